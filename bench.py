@@ -1,0 +1,206 @@
+"""Per-frame reconstruction benchmark on one MI355X.
+
+Workload: the full-rate plain pipeline (640x480 uint16 depth, 5 mm voxels,
+2 cm truncation, 2^20 x 10 hash, 8 analytic primitives, spline trajectory,
+2 % holes).  One step = one frame of
+    depth upload (already in HBM) -> AllocBlockArray -> CollectBlocksInFrustum
+    -> UpdateBlocksSimple -> MarchingCubes -> RayCaster::Cast (640x480).
+All input frames are resident in HBM before the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line.  `value` is voxel-updates/s of the whole chain;
+`ms_per_step` is ms/frame.  Per-stage times come from HIP events recorded
+on the volume's stream inside the timed region.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
+
+
+def algorithmic_bytes(stage, c, frames, w, h):
+    """Algorithmic bytes of one stage over the timed frames."""
+    if stage == "integrate":
+        return 24 * c["voxel_updates"] + 12 * c["blocks_integrated"] + 4 * w * h * frames
+    if stage == "mesh":
+        return (729 * 8 + 512) * c["mc_blocks"] + 40 * c["mc_vertex_writes"] + 12 * c["mc_triangle_writes"]
+    if stage == "raycast":
+        return 16 * w * h * frames + 32 * c["ray_samples"]
+    return None
+
+
+def cpu_baseline(cfg, frames, budget_s=25.0):
+    """The FP64 oracle (1 thread) on the first frames of the same sequence,
+    full chain; stops after the frame that crosses budget_s."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    from meshhashing_amd.engine import synth_depth, synth_pose
+
+    o = orc.Oracle(cfg)
+    t0 = time.perf_counter()
+    done = 0
+    rp = cfg.raycaster_params()
+    for f in range(frames):
+        d, p = synth_depth(cfg, f), synth_pose(cfg, f)
+        o.frame(d, p, mesh=True)
+        o.raycast(np.linalg.inv(p.astype(np.float64)).astype(np.float32), rp)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": o.voxel_updates() / dt, "unit": "voxel-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"first {done} frame(s) of the same 640x480 sequence, full chain "
+                      f"(alloc, collect, TSDF, marching cubes, raycast), FP64 oracle, 1 thread, "
+                      f"{dt:.1f} s, {os.cpu_count()} host cores, {platform.processor() or platform.machine()}",
+            "ms_per_frame": 1e3 * dt / done}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=280)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stage-events", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not "
+                                   "part of it (the sharding design is described in DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    from meshhashing_amd.configs import C2
+    from meshhashing_amd.engine import Volume, synth_depth, synth_pose
+
+    cfg = C2
+    total = args.warmup + args.steps
+    W, H = cfg.width, cfg.height
+    t_gen = time.perf_counter()
+    frames = np.stack([synth_depth(cfg, f % cfg.frames) for f in range(total)])
+    poses = [synth_pose(cfg, f % cfg.frames) for f in range(total)]
+    cTws = [np.linalg.inv(p.astype(np.float64)).astype(np.float32) for p in poses]
+    t_gen = time.perf_counter() - t_gen
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    depth_dev = torch.from_numpy(frames.view(np.int16)).to(dev)  # resident in HBM
+    out_depth = torch.empty(W * H, dtype=torch.float32, device=dev)
+    out_normal = torch.empty(3 * W * H, dtype=torch.float32, device=dev)
+    vol = Volume(cfg)
+    stream = torch.cuda.current_stream()
+    vol.set_stream(stream.cuda_stream)
+    frame_bytes = W * H * 2
+    base_ptr = depth_dev.data_ptr()
+
+    stages = ["alloc", "collect", "integrate", "mesh", "raycast"]
+    ev = {s: [] for s in stages}
+
+    def run_frame(f, record):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(6)] if record else None
+        if record:
+            e[0].record(stream)
+        vol.set_depth_device(base_ptr + f * frame_bytes, poses[f])
+        vol.alloc()
+        if record:
+            e[1].record(stream)
+        vol.collect_visible()
+        if record:
+            e[2].record(stream)
+        vol.integrate_tsdf()
+        if record:
+            e[3].record(stream)
+        vol.marching_cubes()
+        if record:
+            e[4].record(stream)
+        vol.raycast(cTws[f], out_depth.data_ptr(), 0, out_normal.data_ptr())
+        if record:
+            e[5].record(stream)
+            for i, s in enumerate(stages):
+                ev[s].append((e[i], e[i + 1]))
+
+    for f in range(args.warmup):
+        run_frame(f, False)
+    torch.cuda.synchronize()
+    c0 = vol.status()
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    for f in range(args.warmup, total):
+        run_frame(f, bool(args.stage_events))
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_total = t_start.elapsed_time(t_end)
+    c1 = vol.status()
+    cnt = {k: c1[k] - c0[k] for k in ("voxel_updates", "blocks_integrated", "mc_vertex_writes",
+                                      "mc_triangle_writes", "mc_blocks", "ray_samples")}
+    stage_ms = {s: sum(a.elapsed_time(b) for a, b in ev[s]) for s in stages} if args.stage_events else {}
+    K = args.steps
+    ms_per_frame = ms_total / K
+    value = cnt["voxel_updates"] / (ms_total * 1e-3)
+
+    roof = None
+    if stage_ms:
+        dom = max(("integrate", "mesh", "raycast"), key=lambda s: stage_ms[s])
+        rows = {}
+        for s in ("integrate", "mesh", "raycast"):
+            b = algorithmic_bytes(s, cnt, K, W, H)
+            rows[s] = {"ms_per_frame": stage_ms[s] / K, "GB/s": b / (stage_ms[s] * 1e-3) / 1e9,
+                       "frac": b / (stage_ms[s] * 1e-3) / HBM_PEAK}
+        b = algorithmic_bytes(dom, cnt, K, W, H)
+        ach = b / (stage_ms[dom] * 1e-3)
+        roof = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None,
+                "stages": rows}
+
+    st = vol.status()
+    line = {
+        "metric": "ms/frame & voxel-updates/s (640x480, 5 mm) plain vs batched; HBM GB/s; grad err",
+        "value": value,
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_frame,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "c2 full-rate plain pipeline: 640x480 depth, 5 mm voxels, trunc 2 cm, "
+                               "2^20x10 hash, alloc->collect->TSDF->marching cubes->raycast 640x480",
+                   "frames": total, "voxel_size": cfg.voxel_size, "truncation": cfg.truncation,
+                   "bucket_count": cfg.bucket_count, "parallelism": "single GPU"},
+        "roofline": roof,
+        "stage_ms_per_frame": {s: v / K for s, v in stage_ms.items()},
+        "counters_per_frame": {k: v / K for k, v in cnt.items()},
+        "blocks_allocated": st["block_count"],
+        "triangles": st["triangle_count"],
+        "status_flags": st["flag_names"],
+        "host_wall_ms_per_frame": 1e3 * wall / K,
+        "input_generation_s": t_gen,
+    }
+    vol.close()
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(cfg, frames=min(total, 3))
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

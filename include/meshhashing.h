@@ -1,0 +1,267 @@
+/*
+ * meshhashing.h -- C ABI of the MI355X (gfx950) reconstruction hot path.
+ *
+ * Every entry point replaces one free function or lifecycle method of the
+ * reference MeshHashing engine; the reference file:line it stands in for is
+ * given beside each declaration.  Plain C: POD parameter structs, an opaque
+ * volume handle, plain pointers and sizes, int status codes (MH_OK == 0).
+ *
+ * Threading: a volume is used by one host thread at a time.  All kernels of a
+ * volume run on the stream given by mh_volume_set_stream (default: the null
+ * stream).  Entry points that return data to host memory synchronise that
+ * stream; every other entry point only enqueues work.
+ *
+ * Pointer conventions: a parameter named *_dev is device memory (HBM), one
+ * named *_host is host memory.  A NULL output pointer means "not wanted".
+ */
+#ifndef MESHHASHING_H
+#define MESHHASHING_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MH_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  MH_OK = 0,
+  MH_ERR_ARG = -1,      /* bad argument (NULL handle, size, range)           */
+  MH_ERR_HIP = -2,      /* a HIP runtime call failed                         */
+  MH_ERR_CAPACITY = -3, /* a buffer given by the caller is too small         */
+  MH_ERR_STATE = -4,    /* call out of order (e.g. no frame uploaded yet)    */
+  MH_ERR_EMPTY = -5     /* empty worklist; mirrors the reference's -1 return
+                           (mapping/update_bayesian.cu:275-276,
+                            meshing/marching_cubes.cu:336-337)               */
+};
+
+/* ---- device status word bits (mh_status.flags) -------------------------- */
+enum {
+  MH_STATUS_BLOCK_HEAP_EXHAUSTED = 1u << 0,    /* hash_table.h:279-286        */
+  MH_STATUS_INSERT_DROPPED = 1u << 1,          /* list window full, :149-177  */
+  MH_STATUS_DDA_CAP = 1u << 2,                 /* allocate.cu:72 (1024 steps) */
+  MH_STATUS_VERTEX_HEAP_EXHAUSTED = 1u << 3,   /* mesh.h:53-59                */
+  MH_STATUS_TRIANGLE_HEAP_EXHAUSTED = 1u << 4, /* mesh.h:67-73                */
+  MH_STATUS_COORD_RANGE = 1u << 5,  /* block coordinate outside +-2^20        */
+  MH_STATUS_OVERFLOW_QUEUE_FULL = 1u << 6, /* too many list inserts / frame   */
+  MH_STATUS_STENCIL_RANGE = 1u << 7 /* MC stencil left its staged shell       */
+};
+
+/* ---- parameter PODs: same fields as core/params.h:16-74 ----------------- */
+typedef struct mh_hash_params {   /* HashParams, core/params.h:16-24        */
+  uint32_t bucket_count;
+  uint32_t bucket_size;
+  uint32_t entry_count;           /* must equal bucket_count * bucket_size  */
+  uint32_t linked_list_size;
+  uint32_t value_capacity;        /* block pool size                        */
+} mh_hash_params;
+
+typedef struct mh_mesh_params {   /* MeshParams, core/params.h:26-29        */
+  uint32_t max_vertex_count;
+  uint32_t max_triangle_count;
+} mh_mesh_params;
+
+typedef struct mh_volume_params { /* VolumeParams, core/params.h:31-40      */
+  float voxel_size;
+  float truncation_distance_scale;
+  float truncation_distance;
+  float sdf_upper_bound;
+  uint32_t weight_sample;
+  uint32_t weight_upper_bound;    /* unused by the reference as well        */
+} mh_volume_params;
+
+typedef struct mh_sensor_params { /* SensorParams, core/params.h:62-74      */
+  float fx, fy, cx, cy;
+  uint32_t width, height;
+  float min_depth_range;
+  float max_depth_range;
+  float range_factor;             /* metres per raw depth unit              */
+} mh_sensor_params;
+
+typedef struct mh_raycaster_params { /* RayCasterParams, core/params.h:42-57 */
+  float fx, fy, cx, cy;
+  uint32_t width, height;
+  float min_raycast_depth;
+  float max_raycast_depth;
+  float raycast_step;
+  float sample_sdf_threshold;
+  float sdf_threshold;
+  int32_t enable_gradients;
+} mh_raycaster_params;
+
+/* Marching-cubes thresholds.  The reference hard-codes them
+ * (meshing/marching_cubes.cu:20-21,96-99,124-128); they are parameters here.
+ * mh_mc_params_default() returns the reference's values. */
+typedef struct mh_mc_params {
+  float sdf_threshold;   /* |sdf| of every corner must be <= this (0.20)     */
+  float rho_min;         /* a/(a+b) of every corner must be >= this (0.1)    */
+  float weight_min;      /* inv_sigma2 of every corner >= this ((1/vs)^2);
+                            a negative value selects (1/voxel_size)^2         */
+  double snap_distance;  /* vertex snaps to an endpoint with |sdf| < this
+                            (0.008; |sdf| is widened to double and compared
+                            with it, as the reference's double literal)     */
+  int32_t enable_sdf_gradient; /* 0: flat per-triangle normals (mesh.h:98)  */
+} mh_mc_params;
+
+typedef struct mh_status {
+  uint32_t flags;            /* MH_STATUS_* bits, sticky until reset         */
+  uint32_t dropped_inserts;  /* blocks lost to a full linked-list window     */
+  uint32_t dda_cap_hits;     /* pixels whose DDA hit the 1024-step cap       */
+  uint32_t list_inserts;     /* blocks placed in an overflow linked list     */
+  uint32_t block_count;      /* allocated blocks                             */
+  uint32_t candidate_count;  /* blocks in the current worklist               */
+  uint32_t vertex_count;     /* live vertices                                */
+  uint32_t triangle_count;   /* live triangles                               */
+  /* work counters accumulated since the last reset */
+  uint64_t voxel_updates;      /* voxels whose fused state changed          */
+  uint64_t blocks_integrated;  /* worklist blocks visited by fusion         */
+  uint64_t mc_vertex_writes;   /* vertex records written by MC              */
+  uint64_t mc_triangle_writes; /* triangles (re)assigned by MC              */
+  uint64_t mc_blocks;          /* worklist blocks meshed                    */
+  uint64_t ray_samples;        /* valid trilinear samples of the raycaster  */
+} mh_status;
+
+typedef struct mh_volume mh_volume;
+
+/* ---- lifecycle ----------------------------------------------------------
+ * HashTable::{Resize,Reset,Free}   core/hash_table.h:25-33, hash_table.cu:57-143
+ * BlockArray::{Resize,Reset,Free}  core/block_array.h:14-28
+ * EntryArray::Resize               core/entry_array.h:13-23
+ * Mesh::{Resize,Reset,Free}        core/mesh.h:21-24, core/mesh.cu:74-137
+ * Sensor::Sensor                   sensor/rgbd_sensor.cu:15-49
+ * enable_psdf allocates the Beta-inlier fields (a, b) of PSDF fusion.     */
+int mh_volume_create(const mh_hash_params* hash, const mh_volume_params* vol,
+                     const mh_mesh_params* mesh, const mh_sensor_params* sensor,
+                     int enable_psdf, mh_volume** out);
+int mh_volume_destroy(mh_volume* v);
+int mh_volume_reset(mh_volume* v);
+int mh_volume_set_stream(mh_volume* v, void* hip_stream);
+int mh_synchronize(mh_volume* v);
+int mh_get_status(mh_volume* v, mh_status* out);
+void mh_mc_params_default(mh_mc_params* out);
+const char* mh_last_error(void);
+
+/* ---- frame input: Sensor::Process + set_transform ------------------------
+ * sensor/rgbd_sensor.cu:118-137, sensor/preprocess.cu:25-42 (depth
+ * conversion: d = range_factor * (int16)raw, valid iff min <= d <= max, else
+ * -inf), sensor/rgbd_sensor.h:51-54 (cTw = inverse(wTc), FP32).
+ * wTc is camera->world, row-major m11..m44.  The PSDF inlier ratio is reset to
+ * 0.1 (sensor/preprocess.cu:10-23) by every upload.                        */
+int mh_frame_upload_depth_u16(mh_volume* v, const uint16_t* depth_host,
+                              const float wTc[16]);
+int mh_frame_depth_u16_device(mh_volume* v, const uint16_t* depth_dev,
+                              const float wTc[16]);
+int mh_frame_set_pose(mh_volume* v, const float wTc[16]);
+/* Converted metric depth of the current frame (device, W*H floats). */
+int mh_frame_depth_device_ptr(mh_volume* v, const float** depth_dev);
+
+/* ---- mapping ------------------------------------------------------------ */
+/* AllocBlockArray, mapping/allocate.h:16-20 / allocate.cu:8-125.  Lossless:
+ * every block the per-pixel DDA reaches inside the frustum is inserted.   */
+int mh_alloc_blocks(mh_volume* v);
+/* CollectBlocksInFrustum / CollectAllBlocks, core/collect_block_array.h:16-31.
+ * count_host may be NULL (no host synchronisation).                        */
+int mh_collect_visible(mh_volume* v, uint32_t* count_host);
+int mh_collect_all(mh_volume* v, uint32_t* count_host);
+/* UpdateBlocksSimple, mapping/update_simple.h:21-27.                       */
+int mh_integrate_tsdf(mh_volume* v);
+/* PredictOutlierRatio, UpdateBlocksBayesian, mapping/update_bayesian.h:33-47 */
+int mh_predict_inlier(mh_volume* v);
+int mh_integrate_psdf(mh_volume* v);
+/* Inlier ratio image of the current frame (device, W*H floats). */
+int mh_inlier_ratio_device_ptr(mh_volume* v, float** inlier_dev);
+
+/* ---- meshing: MarchingCubes, meshing/marching_cubes.h:18-24 ------------- */
+int mh_marching_cubes(mh_volume* v, const mh_mc_params* p);
+
+/* ---- raycasting: RayCaster::Cast, visualization/ray_caster.h:37-41 -------
+ * cTw is world->camera (the reference's argument); wTc = inverse(cTw) in
+ * FP32 as ray_caster.cu:195.  Outputs (device, caller-owned, W*H pixels):
+ * depth_dev metric depth (-inf: no hit), vertex_dev camera-space point
+ * (3 floats per pixel), normal_dev camera-space normal (3 floats per pixel).
+ * Any output may be NULL.                                                  */
+int mh_raycast(mh_volume* v, const mh_raycaster_params* p, const float cTw[16],
+               float* depth_dev, float* vertex_dev, float* normal_dev);
+
+/* ---- denoising: PrimalDualInit / PrimalDualIterate, optimize/primal_dual.h:14-29
+ * Runs over the current worklist (collect_all first, as the survey'd driver
+ * does).  energy_host (3 doubles, may be NULL): data term, TV term and
+ * err_data*lambda/2 + err_tv of the iteration's pass 1 (primal_dual.cu:207-214). */
+int mh_primal_dual_init(mh_volume* v);
+int mh_primal_dual_iterate(mh_volume* v, float lambda, float sigma, float tau,
+                           double* energy_host);
+
+/* ---- canonical dumps and restore (host buffers) --------------------------
+ * Blocks are returned sorted by (x, y, z).  Pass NULL buffers to query the
+ * count only.  capacity = number of blocks the buffers hold.
+ * pos: 3 int32 per block; fields: 512 values per block (voxel index
+ * x + 8y + 64z, geometry/geometry_helper.h:113-120).  color is RGBA8.     */
+int mh_dump_blocks(mh_volume* v, uint32_t capacity, int32_t* pos_host,
+                   float* sdf_host, float* weight_host, uint32_t* color_host,
+                   float* a_host, float* b_host, uint32_t* count_host);
+/* Upload n blocks (same layout as mh_dump_blocks) into an empty volume.
+ * a_host / b_host may be NULL.  The blocks are inserted into the hash
+ * table with the same insertion rule as mh_alloc_blocks.                   */
+int mh_load_blocks(mh_volume* v, uint32_t n, const int32_t* pos_host,
+                   const float* sdf_host, const float* weight_host,
+                   const uint32_t* color_host, const float* a_host,
+                   const float* b_host);
+/* Hash layout: one record per occupied entry, sorted by entry index:
+ * entry index, home bucket, in-bucket flag (1) or list (0), pos xyz.
+ * rec_host holds 6 int32 per record.                                       */
+int mh_dump_hash(mh_volume* v, uint32_t capacity, int32_t* rec_host,
+                 uint32_t* count_host);
+/* Current worklist as block positions (3 int32 each), sorted.              */
+int mh_dump_candidates(mh_volume* v, uint32_t capacity, int32_t* pos_host,
+                       uint32_t* count_host);
+/* Mesh in canonical form.  Triangles: 9 int32 each -- the owner voxel
+ * (x, y, z) and edge axis of each of its three vertices, in table order --
+ * sorted by (cube voxel, slot).  tri_cube_host (optional): 4 int32 per
+ * triangle: cube voxel x, y, z and triangle slot.  Vertices: one record per
+ * live vertex, key (4 int32: owner voxel xyz, axis) + 10 floats (pos xyz,
+ * normal xyz, color rgb, radius) + ref count, sorted by key.              */
+int mh_dump_mesh(mh_volume* v, uint32_t tri_capacity, int32_t* tri_host,
+                 int32_t* tri_cube_host, uint32_t vtx_capacity,
+                 int32_t* vtx_key_host, float* vtx_val_host,
+                 int32_t* vtx_ref_host, uint32_t* tri_count_host,
+                 uint32_t* vtx_count_host);
+/* Cube indices of every voxel of every allocated block, block order as
+ * mh_dump_blocks: 512 (curr | prev << 8) per block.                        */
+int mh_dump_cube_indices(mh_volume* v, uint32_t capacity, uint16_t* cube_host,
+                         uint32_t* count_host);
+/* Primal-dual variables in mh_dump_blocks order: sdf0, sdf_bar, w_pd, p xyz
+ * (6 floats per voxel, 512 voxels per block, field-major per block) and the
+ * mask (1 byte per voxel).                                                  */
+int mh_dump_primal_dual(mh_volume* v, uint32_t capacity, float* pdv_host,
+                        uint8_t* mask_host, uint32_t* count_host);
+
+/* ---- synthetic depth sequences (seeded, deterministic, CPU) --------------
+ * Analytic scenes: an axis-aligned room seen from inside plus spheres and
+ * boxes.  kind: 0 = circular trajectory around the room centre with 3
+ * spheres; 1 = spline trajectory with 8 spheres/boxes.  The scene and the
+ * trajectory depend on seed only.                                          */
+typedef struct mh_synth_spec {
+  uint64_t seed;
+  int32_t kind;
+  uint32_t width, height;
+  float fx, fy, cx, cy;
+  uint32_t frames;
+  float hole_fraction;     /* pixels set to 0                              */
+  float outlier_fraction;  /* valid pixels replaced by U[0.3, 5] m         */
+  int32_t noise;           /* 1: sigma(z) = 0.0012 + 0.0019 (z - 0.4)^2    */
+} mh_synth_spec;
+/* Pose of frame f (wTc, row-major).                                         */
+int mh_synth_pose(const mh_synth_spec* s, uint32_t frame, float wTc[16]);
+/* Depth of frame f in mm (uint16, W*H, 0 = no measurement).               */
+int mh_synth_depth(const mh_synth_spec* s, uint32_t frame, uint16_t* depth_host);
+/* Exact (noise-free) z-depth in metres, for tests (+inf: no hit).          */
+int mh_synth_depth_exact(const mh_synth_spec* s, uint32_t frame,
+                         const float wTc[16], double* depth_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MESHHASHING_H */

@@ -1,0 +1,549 @@
+// mapping.hip -- frame input, block allocation, worklist compaction and
+// TSDF / PSDF fusion kernels for gfx950.
+//
+// Reference behaviour followed here (file:line in theNded/MeshHashing):
+//   depth conversion        sensor/preprocess.cu:25-42
+//   inlier-ratio reset      sensor/preprocess.cu:10-23
+//   block allocation        mapping/allocate.cu:8-99, core/hash_table.h:96-179
+//   visible-block compaction core/collect_block_array.cu:25-91
+//   TSDF update             mapping/update_simple.cu:13-77, core/voxel.h:107-115
+//   PSDF inlier prediction  mapping/update_bayesian.cu:16-101
+//   PSDF update             mapping/update_bayesian.cu:103-185
+#include "mh_kernels.h"
+
+namespace mh {
+
+// ------------------------------------------------------------------ resets
+__global__ void reset_entries_kernel(Entry* e, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    e[i] = Entry{kEmptyKey, kFreePtr, 0u};
+}
+
+__global__ void reset_pool_kernel(View V) {
+  const uint32_t cap = V.capacity;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
+    V.block_heap[i] = (int32_t)(cap - 1u - i);
+    V.block_keys[i] = kEmptyKey;
+  }
+  const uint32_t nv = V.max_vertices;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
+    V.vertex_heap[i] = (int32_t)(nv - 1u - i);
+    Vertex z{};
+    V.vertices[i] = z;
+  }
+  const uint32_t nt = V.max_triangles;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x) {
+    V.triangle_heap[i] = (int32_t)(nt - 1u - i);
+    V.triangles[3 * i + 0] = -1;
+    V.triangles[3 * i + 1] = -1;
+    V.triangles[3 * i + 2] = -1;
+  }
+}
+
+// Voxel / mesh-unit state of a block, as Block::Clear (core/block.h:27-39).
+__global__ void reset_voxels_kernel(View V, size_t nvox) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox;
+       i += (size_t)gridDim.x * blockDim.x) {
+    V.sdf[i] = 0.0f;
+    V.weight[i] = 0.0f;
+    V.color[i] = 0u;
+    if (V.va) { V.va[i] = 0.0f; V.vb[i] = 0.0f; }
+    V.vptr[3 * i + 0] = kFreePtr;
+    V.vptr[3 * i + 1] = kFreePtr;
+    V.vptr[3 * i + 2] = kFreePtr;
+    for (int t = 0; t < 5; ++t) V.tptr[5 * i + t] = kFreePtr;
+    V.cube[i] = 0;
+  }
+}
+
+// Counters.  The vertex pool reserves index 0: the reference's
+// PredictOutlierRatio skips vertex pointers <= 0 (update_bayesian.cu:84),
+// i.e. whichever vertex happened to be allocated first; reserving the slot
+// keeps that expression while excluding no real vertex.
+__global__ void reset_counters_kernel(View V) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    Counters c{};
+    c.block_top = (int32_t)V.capacity - 1;
+    c.block_hwm = 0;
+    c.vertex_top = (int32_t)V.max_vertices - 2;
+    c.triangle_top = (int32_t)V.max_triangles - 1;
+    *V.ctr = c;
+  }
+}
+
+__global__ void set_u32_kernel(uint32_t* p, uint32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// ------------------------------------------------------------ frame input
+__global__ void convert_depth_kernel(const uint16_t* __restrict__ raw, float* __restrict__ depth,
+                                     float* __restrict__ inlier, uint32_t n, float range_factor,
+                                     float dmin, float dmax) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float d = range_factor * (float)(int16_t)raw[i];
+  depth[i] = (d >= dmin && d <= dmax) ? d : -INFINITY;
+  if (inlier) inlier[i] = 0.1f;
+}
+
+// ------------------------------------------------------------- allocation
+// Plain-load presence test (bucket, then list).  Keys are never removed
+// inside a frame, so a hit is final; a miss is re-checked coherently.
+__device__ __forceinline__ bool key_present(const View& V, uint64_t key, uint32_t first) {
+  for (uint32_t j = 0; j < V.bucket_size; ++j)
+    if (V.entries[first + j].key == key) return true;
+  const uint32_t last = first + V.bucket_size - 1;
+  uint32_t i = last;
+  for (uint32_t it = 0; it < V.linked_list_size; ++it) {
+    const Entry e = V.entries[i];
+    if (e.key == key) return true;
+    if (e.offset == 0) break;
+    i = (last + e.offset) % V.entry_count;
+  }
+  return false;
+}
+
+// Lossless insert.  A free bucket slot is claimed by a 64-bit CAS on its key;
+// losing a race re-scans the bucket with device-coherent loads (each lost
+// CAS means one more slot is taken, so this ends within bucket_size + 1
+// rounds).  A full bucket defers the key to the serial list-insert pass.
+__device__ void insert_block(const View& V, i3 p) {
+  if (!coord_ok(p)) {
+    flag(V, MH_STATUS_COORD_RANGE);
+    return;
+  }
+  const uint64_t key = pack_key(p);
+  const uint32_t first = hash_bucket(p, V.bucket_count) * V.bucket_size;
+  if (key_present(V, key, first)) return;
+  for (uint32_t round = 0; round <= V.bucket_size; ++round) {
+    int empty = -1;
+    for (uint32_t j = 0; j < V.bucket_size; ++j) {
+      const uint64_t k = __hip_atomic_load(&V.entries[first + j].key, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+      if (k == key) return;
+      if (k == kEmptyKey && empty < 0) empty = (int)j;
+    }
+    if (empty < 0) break;
+    Entry* e = &V.entries[first + (uint32_t)empty];
+    const uint64_t old = atomicCAS((unsigned long long*)&e->key, (unsigned long long)kEmptyKey,
+                                   (unsigned long long)key);
+    if (old == kEmptyKey) {
+      const int32_t ptr = alloc_block(V);
+      e->ptr = ptr;
+      if (ptr >= 0) V.block_keys[ptr] = key;
+      return;
+    }
+    if (old == key) return;
+  }
+  const uint32_t q = atomicAdd(&V.ctr->ovf_count, 1u);
+  if (q < kOverflowQueue)
+    V.ovf[q] = key;
+  else
+    flag(V, MH_STATUS_OVERFLOW_QUEUE_FULL);
+}
+
+// AllocBlockArrayKernel, allocate.cu:8-99: per pixel, walk the blocks between
+// depth - trunc and depth + trunc with a 3-D DDA (Amanatides-Woo in block
+// units) and insert every block whose centre lies in the frustum.
+__global__ __launch_bounds__(256) void alloc_kernel(View V) {
+  const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const uint32_t y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= V.cam.width || y >= V.cam.height) return;
+  const Geo& g = V.geo;
+  const float depth = V.depth[y * V.cam.width + x];
+  if (depth == -INFINITY || depth == 0.0f || depth >= g.sdf_upper_bound) return;
+
+  const float tr = truncate_distance(g, depth);
+  const float near_d = fminf(g.sdf_upper_bound, depth - tr);
+  const float far_d = fminf(g.sdf_upper_bound, depth + tr);
+  if (near_d >= far_d) return;
+
+  const f3 cn = reproject(V.cam, x, y, near_d);
+  const f3 cf = reproject(V.cam, x, y, far_d);
+  const f3 wn = xform(V.wTc, cn);
+  const f3 wf = xform(V.wTc, cf);
+  const f3 dir = normalize(wf - wn);
+
+  const i3 bn = world_to_block(g, wn);
+  const i3 bf = world_to_block(g, wf);
+  const f3 step = mk3((float)sgn(dir.x), (float)sgn(dir.y), (float)sgn(dir.z));
+
+  // BlockToWorld(b_near + clamp(step, 0, 1)) - 0.5 vs
+  const i3 bo = i3{bn.x + f2i(fminf(fmaxf(step.x, 0.0f), 1.0f)),
+                   bn.y + f2i(fminf(fmaxf(step.y, 0.0f), 1.0f)),
+                   bn.z + f2i(fminf(fmaxf(step.z, 0.0f), 1.0f))};
+  const f3 bw = block_to_world(g, bo);
+  const float hv = 0.5f * g.voxel_size;
+  const f3 nvc = mk3(bw.x - hv, bw.y - hv, bw.z - hv);
+  f3 t = (nvc - wn) / dir;
+  const float bsz = (float)kBlockSide;
+  f3 dt = mk3(step.x * bsz * g.voxel_size, step.y * bsz * g.voxel_size,
+              step.z * bsz * g.voxel_size) / dir;
+  const i3 bound = i3{f2i((float)bf.x + step.x), f2i((float)bf.y + step.y),
+                      f2i((float)bf.z + step.z)};
+  if (dir.x == 0.0f) { t.x = INFINITY; dt.x = INFINITY; }
+  if (dir.y == 0.0f) { t.y = INFINITY; dt.y = INFINITY; }
+  if (dir.z == 0.0f) { t.z = INFINITY; dt.z = INFINITY; }
+
+  const int sx = (int)step.x, sy = (int)step.y, sz = (int)step.z;
+  i3 cur = bn;
+  for (int it = 0; it < kMaxDdaSteps; ++it) {
+    if (block_in_frustum(g, V.cTw, cur, V.cam)) insert_block(V, cur);
+    if (t.x < t.y && t.x < t.z) {
+      cur.x += sx;
+      if (cur.x == bound.x) return;
+      t.x += dt.x;
+    } else if (t.y < t.z) {
+      cur.y += sy;
+      if (cur.y == bound.y) return;
+      t.y += dt.y;
+    } else {
+      cur.z += sz;
+      if (cur.z == bound.z) return;
+      t.z += dt.z;
+    }
+  }
+  atomicAdd(&V.ctr->dda_cap, 1u);
+  flag(V, MH_STATUS_DDA_CAP);
+}
+
+// Serial list insertion for keys whose bucket was full (hash_table.h:145-177):
+// probe offsets 1..linked_list_size from the bucket's last slot, skipping
+// multiples of bucket_size, and link the first free slot in as list head.
+// Also clears the worklist counter for the collect pass that follows.
+__global__ void list_insert_kernel(View V) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t n = min(V.ctr->ovf_count, kOverflowQueue);
+  for (uint32_t q = 0; q < n; ++q) {
+    const uint64_t key = V.ovf[q];
+    const i3 p = unpack_key(key);
+    const uint32_t first = hash_bucket(p, V.bucket_count) * V.bucket_size;
+    if (key_present(V, key, first)) continue;
+    bool done = false;
+    for (uint32_t j = 0; j < V.bucket_size && !done; ++j) {
+      Entry* e = &V.entries[first + j];
+      if (e->key == kEmptyKey) {
+        e->key = key;
+        const int32_t ptr = alloc_block(V);
+        e->ptr = ptr;
+        if (ptr >= 0) V.block_keys[ptr] = key;
+        done = true;
+      }
+    }
+    if (done) continue;
+    const uint32_t last = first + V.bucket_size - 1;
+    uint32_t offset = 0;
+    for (uint32_t it = 0; it < V.linked_list_size; ++it) {
+      ++offset;
+      if (offset % V.bucket_size == 0) continue;
+      const uint32_t i = (last + offset) % V.entry_count;
+      Entry* e = &V.entries[i];
+      if (e->key == kEmptyKey) {
+        e->key = key;
+        e->offset = V.entries[last].offset;
+        const int32_t ptr = alloc_block(V);
+        e->ptr = ptr;
+        if (ptr >= 0) V.block_keys[ptr] = key;
+        V.entries[last].offset = offset;
+        V.ctr->list_inserts += 1u;
+        done = true;
+        break;
+      }
+    }
+    if (!done) {
+      V.ctr->dropped += 1u;
+      V.ctr->flags |= MH_STATUS_INSERT_DROPPED;
+    }
+  }
+  V.ctr->ovf_count = 0;
+  V.ctr->cand_count = 0;
+}
+
+// --------------------------------------------------------------- compaction
+// Stream compaction of allocated blocks (optionally in-frustum) over the
+// block pool, not over the whole entry table: the pool is dense up to its
+// high-water mark, which is ~100x smaller than entry_count.  One atomic per
+// wave; the worklist order is not deterministic (neither is the reference's).
+__global__ __launch_bounds__(256) void collect_kernel(View V, int frustum) {
+  const uint32_t n = V.ctr->block_hwm;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t p = base + threadIdx.x;
+    bool take = false;
+    if (p < n) {
+      const uint64_t k = V.block_keys[p];
+      if (k != kEmptyKey)
+        take = !frustum || block_in_frustum(V.geo, V.cTw, unpack_key(k), V.cam);
+    }
+    const uint64_t mask = __ballot(take);
+    const uint32_t cnt = (uint32_t)__popcll(mask);
+    const uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t out = 0;
+    if (lane == 0 && cnt) out = atomicAdd(&V.ctr->cand_count, cnt);
+    out = __shfl(out, 0);
+    if (take) V.cand[out + prefix] = (int32_t)p;
+  }
+}
+
+// ------------------------------------------------------------- TSDF fusion
+// UpdateBlocksSimpleKernel (update_simple.cu:13-77) + Voxel::Update
+// (voxel.h:107-115).  One workgroup per worklist block (persistent loop),
+// 2 voxels per lane, coalesced SoA state.  The pixel choice and the skip
+// tests are FP32 decisions; the observed SDF value uses an FP64 camera z.
+__device__ __forceinline__ uint32_t blend_color(uint32_t c, uint32_t d) {
+  uint32_t out = 0;
+  for (int k = 0; k < 3; ++k) {
+    const float cp = (float)((c >> (8 * k)) & 255u);
+    const float cd = (float)((d >> (8 * k)) & 255u);
+    const float cc = 0.5f * cp + 0.5f * cd;
+    out |= ((uint32_t)(uint8_t)(cc + 0.5f)) << (8 * k);
+  }
+  return out;
+}
+
+__global__ __launch_bounds__(256) void integrate_tsdf_kernel(View V) {
+  const uint32_t n = V.ctr->cand_count;
+  const Geo g = V.geo;
+  const Cam c = V.cam;
+  const double vsd = (double)g.voxel_size;
+  const float w10 = 10.0f * g.weight_sample;
+  const uint32_t delta_color = 255u << 8;  // (0, 255, 0): no colour stream
+  uint32_t nupd = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&V.ctr->blocks_integrated, (unsigned long long)n);
+  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
+    const int32_t ptr = V.cand[ci];
+    const i3 b = unpack_key(V.block_keys[ptr]);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x) {
+      const i3 o = devectorize(i);
+      const i3 vp = i3{b.x * kBlockSide + o.x, b.y * kBlockSide + o.y, b.z * kBlockSide + o.z};
+      const f3 cp = xform(V.cTw, voxel_to_world(g, vp));
+      const uint32_t u = (uint32_t)f2i(proj_u(c, cp) + 0.5f);
+      const uint32_t v = (uint32_t)f2i(proj_v(c, cp) + 0.5f);
+      if (u >= c.width || v >= c.height) continue;
+      const float depth = V.depth[v * c.width + u];
+      if (depth == -INFINITY || depth == 0.0f || depth >= g.sdf_upper_bound) continue;
+      const float sdf_dec = depth - cp.z;
+      const float wgt = fmaxf(w10 * (1.0f - normalize_depth(depth, c.min_depth, c.max_depth)), 1.0f);
+      const float tr = truncate_distance(g, depth);
+      if (sdf_dec <= -tr) continue;
+      const double X = (double)vp.x * vsd, Y = (double)vp.y * vsd, Z = (double)vp.z * vsd;
+      const double z64 = fma(V.cTw64.m[8], X, fma(V.cTw64.m[9], Y, fma(V.cTw64.m[10], Z, V.cTw64.m[11])));
+      double s = (double)depth - z64;
+      const double trd = (double)tr;
+      s = s > trd ? trd : (s < -trd ? -trd : s);
+      const float sdf = (float)s;
+      const size_t idx = (size_t)ptr * kBlockVoxels + i;
+      const float so = V.sdf[idx];
+      const float wo = V.weight[idx];
+      V.sdf[idx] = (so * wo + sdf * wgt) / (wo + wgt);
+      V.weight[idx] = wo + wgt;
+      V.color[idx] = blend_color(V.color[idx], delta_color);
+      ++nupd;
+    }
+  }
+  wave_add_u64(&V.ctr->voxel_updates, nupd);
+}
+
+// ------------------------------------------------------------- PSDF fusion
+// gaussian(x, u, s2) = exp(-(x-u)^2/s2) / (sqrt(2 pi) sqrt(s2)), note: no
+// factor 1/2 in the exponent (geometry/geometry_helper.h:30-34).
+__device__ __forceinline__ float gaussian_psdf(float x, float u, float s2) {
+  const float d = x - u;
+  return expf(-(d * d) / s2) / (2.5066282746310002f * sqrtf(s2));
+}
+
+// PredictOutlierRatioKernel, update_bayesian.cu:36-101.  Every voxel of the
+// worklist that owns a vertex of the previous mesh scores the pixel it
+// projects to; the pixel keeps the maximum.  Scores are positive floats, so
+// an integer atomicMax on their bits is an exact, order-free float max.
+__global__ __launch_bounds__(256) void predict_inlier_kernel(View V) {
+  const uint32_t n = V.ctr->cand_count;
+  const Geo g = V.geo;
+  const Cam c = V.cam;
+  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
+    const int32_t ptr = V.cand[ci];
+    const i3 b = unpack_key(V.block_keys[ptr]);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x) {
+      const size_t idx = (size_t)ptr * kBlockVoxels + i;
+      const int32_t p0 = V.vptr[3 * idx + 0], p1 = V.vptr[3 * idx + 1], p2 = V.vptr[3 * idx + 2];
+      if (p0 <= 0 && p1 <= 0 && p2 <= 0) continue;
+      const i3 o = devectorize(i);
+      const i3 vp = i3{b.x * kBlockSide + o.x, b.y * kBlockSide + o.y, b.z * kBlockSide + o.z};
+      const f3 cp = xform(V.cTw, voxel_to_world(g, vp));
+      const uint32_t u = (uint32_t)f2i(proj_u(c, cp) + 0.5f);
+      const uint32_t v = (uint32_t)f2i(proj_v(c, cp) + 0.5f);
+      if (u >= c.width || v >= c.height) continue;
+      const uint32_t pix = u + v * c.width;
+      const float depth = V.depth[pix];
+      if (depth == -INFINITY || depth == 0.0f || depth >= g.sdf_upper_bound) continue;
+      const f3 pc = reproject(c, u, v, depth);
+      const float vsdf = V.sdf[idx];
+      const float vis2 = V.weight[idx];
+      const int32_t ps[3] = {p0, p1, p2};
+      for (int k = 0; k < 3; ++k) {
+        if (ps[k] <= 0) continue;
+        const Vertex vt = V.vertices[ps[k]];
+        const f3 nrm = mk3(vt.nx, vt.ny, vt.nz);
+        const f3 x = mk3(vt.px, vt.py, vt.pz);
+        // The normal goes through the affine transform, translation included
+        // (update_bayesian.cu:78,84).
+        const f3 nc = normalize(xform(V.cTw, nrm));
+        const float cos_a = dot(normalize(-pc), nc);
+        const f3 vec = pc - xform(V.cTw, x);
+        const float pd = dot(vec, nc);
+        const float disk = sqrtf(dot(vec, vec) - pd * pd);
+        const float dd = pd - vsdf;
+        const float w_dist = expf(-(dd * dd) * vis2);
+        const float w_disk = 0.5f + 0.5f / (1.0f + expf(-fabsf(disk / vt.radius)));
+        const float w_ang = fmaxf((cos_a - 0.1736f) / (1.0f - 0.1736f), 0.1f);
+        const float r = fmaxf(w_disk * w_dist * w_ang, 0.01f);
+        atomicMax((int*)&V.inlier[pix], __float_as_int(r));
+      }
+    }
+  }
+}
+
+// UpdateBlocksBayesianKernel, update_bayesian.cu:103-185.
+__global__ __launch_bounds__(256) void integrate_psdf_kernel(View V) {
+  const uint32_t n = V.ctr->cand_count;
+  const Geo g = V.geo;
+  const Cam c = V.cam;
+  const double vsd = (double)g.voxel_size;
+  uint32_t nupd = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&V.ctr->blocks_integrated, (unsigned long long)n);
+  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
+    const int32_t ptr = V.cand[ci];
+    const i3 b = unpack_key(V.block_keys[ptr]);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x) {
+      const i3 o = devectorize(i);
+      const i3 vp = i3{b.x * kBlockSide + o.x, b.y * kBlockSide + o.y, b.z * kBlockSide + o.z};
+      const f3 cp = xform(V.cTw, voxel_to_world(g, vp));
+      const uint32_t u = (uint32_t)f2i(proj_u(c, cp) + 0.5f);
+      const uint32_t v = (uint32_t)f2i(proj_v(c, cp) + 0.5f);
+      if (u >= c.width || v >= c.height) continue;
+      const uint32_t pix = u + v * c.width;
+      const float depth = V.depth[pix];
+      if (depth == -INFINITY || depth == 0.0f || depth >= g.sdf_upper_bound) continue;
+      const float x_dec = depth - cp.z;
+      const float rho = V.inlier[pix];
+      const float tr = truncate_distance(g, depth);
+      if (x_dec <= -tr) continue;
+      ++nupd;
+      const double X = (double)vp.x * vsd, Y = (double)vp.y * vsd, Z = (double)vp.z * vsd;
+      const double z64 = fma(V.cTw64.m[8], X, fma(V.cTw64.m[9], Y, fma(V.cTw64.m[10], Z, V.cTw64.m[11])));
+      double s = (double)depth - z64;
+      const double trd = (double)tr;
+      s = s > trd ? trd : (s < -trd ? -trd : s);
+      const float x = (float)s;
+
+      const float tau = (depth - 0.4f) * 0.012f + 0.019f;
+      const size_t idx = (size_t)ptr * kBlockVoxels + i;
+      const float is2 = V.weight[idx];
+      if (is2 == 0.0f) {
+        V.sdf[idx] = x;
+        V.weight[idx] = 1.0f / (tau * tau);
+        V.va[idx] = rho * 10.0f;
+        V.vb[idx] = (1.0f - rho) * 10.0f;
+        continue;
+      }
+      const float mu = V.sdf[idx];
+      const float s2 = 1.0f / is2;
+      const float t2 = tau * tau;
+      const float ss2 = 1.0f / (1.0f / s2 + 1.0f / t2);
+      const float m = ss2 * (mu / s2 + x / t2);
+      float C1 = rho * gaussian_psdf(x, mu, s2 + t2);
+      float C2 = (depth < c.max_depth && depth >= c.min_depth) ? (1.0f - rho) / (5.0f - 0.1f) : 1.0f;
+      const float csum = C1 + C2;
+      C1 /= csum;
+      C2 /= csum;
+      const float a = V.va[idx], bb = V.vb[idx];
+      const float ab1 = a + bb + 1.0f, ab2 = a + bb + 2.0f;
+      const float f = C1 * (a + 1.0f) / ab1 + C2 * a / ab1;
+      const float e = C1 * (a + 1.0f) * (a + 2.0f) / (ab1 * ab2) + C2 * a * (a + 1.0f) / (ab1 * ab2);
+      const float nsdf = C1 * m + C2 * mu;
+      V.sdf[idx] = nsdf;
+      V.weight[idx] = 1.0f / (C1 * (ss2 + m * m) + C2 * (s2 + mu * mu) - nsdf * nsdf);
+      const float na = (e - f) / (f - e / f);
+      V.va[idx] = na;
+      V.vb[idx] = na * (1.0f - f) / f;
+    }
+  }
+  wave_add_u64(&V.ctr->voxel_updates, nupd);
+}
+
+// ------------------------------------------------------------ block restore
+// Insert explicit block keys with the allocation kernel's insertion rule,
+// then copy their voxel fields in (checkpoint restore, parity fixtures).
+__global__ void load_insert_kernel(View V, const uint64_t* keys, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    insert_block(V, unpack_key(keys[i]));
+}
+
+__global__ __launch_bounds__(256) void load_scatter_kernel(View V, const uint64_t* keys, uint32_t n,
+                                                           const float* sdf, const float* weight,
+                                                           const uint32_t* color, const float* a,
+                                                           const float* b) {
+  for (uint32_t blk = blockIdx.x; blk < n; blk += gridDim.x) {
+    const int32_t p = hash_find(V, unpack_key(keys[blk]));
+    if (p < 0) continue;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x) {
+      const size_t src = (size_t)blk * kBlockVoxels + i;
+      const size_t dst = (size_t)p * kBlockVoxels + i;
+      V.sdf[dst] = sdf[src];
+      V.weight[dst] = weight[src];
+      if (color) V.color[dst] = color[src];
+      if (a && V.va) V.va[dst] = a[src];
+      if (b && V.vb) V.vb[dst] = b[src];
+    }
+  }
+}
+
+void launch_load_insert(const View& V, const uint64_t* keys, uint32_t n, hipStream_t s) {
+  load_insert_kernel<<<(n + 255) / 256, 256, 0, s>>>(V, keys, n);
+  list_insert_kernel<<<1, 64, 0, s>>>(V);
+}
+
+void launch_load_scatter(const View& V, const uint64_t* keys, uint32_t n, const float* sdf,
+                         const float* weight, const uint32_t* color, const float* a,
+                         const float* b, hipStream_t s) {
+  load_scatter_kernel<<<n < 4096u ? n : 4096u, 256, 0, s>>>(V, keys, n, sdf, weight, color, a, b);
+}
+
+// ------------------------------------------------------------------ launchers
+void launch_reset(const View& V, hipStream_t s, int grid) {
+  reset_entries_kernel<<<grid, 256, 0, s>>>(V.entries, V.entry_count);
+  reset_pool_kernel<<<grid, 256, 0, s>>>(V);
+  reset_voxels_kernel<<<grid * 4, 256, 0, s>>>(V, (size_t)V.capacity * kBlockVoxels);
+  reset_counters_kernel<<<1, 64, 0, s>>>(V);
+}
+
+void launch_convert_depth(const View& V, const uint16_t* raw, hipStream_t s) {
+  const uint32_t n = V.cam.width * V.cam.height;
+  convert_depth_kernel<<<(n + 255) / 256, 256, 0, s>>>(raw, V.depth, V.inlier, n, V.range_factor,
+                                                       V.cam.min_depth, V.cam.max_depth);
+}
+
+void launch_alloc(const View& V, hipStream_t s) {
+  dim3 grid((V.cam.width + 15) / 16, (V.cam.height + 15) / 16);
+  alloc_kernel<<<grid, 256, 0, s>>>(V);
+  list_insert_kernel<<<1, 64, 0, s>>>(V);
+}
+
+void launch_collect(const View& V, int frustum, hipStream_t s, int grid) {
+  set_u32_kernel<<<1, 64, 0, s>>>(&V.ctr->cand_count, 0u);
+  collect_kernel<<<grid, 256, 0, s>>>(V, frustum);
+}
+
+void launch_integrate_tsdf(const View& V, hipStream_t s, int grid) {
+  integrate_tsdf_kernel<<<grid, 256, 0, s>>>(V);
+}
+
+void launch_predict_inlier(const View& V, hipStream_t s, int grid) {
+  predict_inlier_kernel<<<grid, 256, 0, s>>>(V);
+}
+
+void launch_integrate_psdf(const View& V, hipStream_t s, int grid) {
+  integrate_psdf_kernel<<<grid, 256, 0, s>>>(V);
+}
+
+}  // namespace mh

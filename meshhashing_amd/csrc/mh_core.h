@@ -1,0 +1,305 @@
+// mh_core.h -- data layout, FP32 decision arithmetic and hash-table access
+// shared by every kernel of the library.
+//
+// Arithmetic convention.  The whole library is compiled with
+// -ffp-contract=off and IEEE division / square root.  Every expression that
+// decides something discrete (which pixel a voxel projects to, which block a
+// DDA step enters, which voxel a world point rounds to, the sign of an
+// interpolated SDF) is written out with the reference's operation order, so
+// that the FP64 oracle, which evaluates the same decision expressions in
+// FP32, makes bit-identical decisions.  Values that are only accumulated
+// (transforms feeding an SDF value, fusion updates) are free to use FP64 or
+// explicit fma.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/meshhashing.h"
+
+#define MH_HD __host__ __device__ __forceinline__
+
+namespace mh {
+
+constexpr int kBlockSide = 8;                 // core/common.h:46-47
+constexpr int kBlockVoxels = 512;
+constexpr int32_t kFreePtr = -2;              // core/common.h:54-58
+constexpr int32_t kLockPtr = -1;
+constexpr uint64_t kEmptyKey = ~0ull;
+constexpr int kMaxDdaSteps = 1024;            // mapping/allocate.cu:72
+constexpr uint32_t kOverflowQueue = 1u << 16; // list inserts per frame
+
+// ---------------------------------------------------------------- vectors
+struct f3 { float x, y, z; };
+struct i3 { int x, y, z; };
+struct d3 { double x, y, z; };
+
+MH_HD f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+MH_HD f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+MH_HD f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+MH_HD f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+MH_HD f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+MH_HD f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+MH_HD f3 operator/(f3 a, f3 b) { return f3{a.x / b.x, a.y / b.y, a.z / b.z}; }
+MH_HD f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+MH_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+MH_HD float length(f3 a) { return sqrtf(dot(a, a)); }
+// v * (1/|v|): the reference multiplies by rsqrtf (helper_math normalize);
+// here the reciprocal square root is the correctly rounded 1/sqrtf so that
+// host and device agree bit for bit.
+MH_HD f3 normalize(f3 v) { return v * (1.0f / sqrtf(dot(v, v))); }
+
+// sign(s) = s < 0 ? -1 : +1 (sign(0) = +1), helper_math.h:1464-1467.
+MH_HD int sgn(float s) { return s < 0.0f ? -1 : 1; }
+
+// float -> int conversion with the GPU's saturating semantics (NaN -> 0),
+// written out so host code (the oracle, the loaders) behaves identically.
+MH_HD int f2i(float f) {
+  if (!(f == f)) return 0;
+  if (f >= 2147483648.0f) return 2147483647;
+  if (f <= -2147483648.0f) return (-2147483647 - 1);
+  return (int)f;
+}
+
+// ---------------------------------------------------------------- poses
+// Row-major 4x4, m[r*4+c]; affine transform of a point (w = 1) as
+// matrix.h:919-926: ((m11 x + m12 y) + m13 z) + m14.
+struct M4 { float m[16]; };
+MH_HD f3 xform(const M4& T, f3 p) {
+  return f3{T.m[0] * p.x + T.m[1] * p.y + T.m[2] * p.z + T.m[3] * 1.0f,
+            T.m[4] * p.x + T.m[5] * p.y + T.m[6] * p.z + T.m[7] * 1.0f,
+            T.m[8] * p.x + T.m[9] * p.y + T.m[10] * p.z + T.m[11] * 1.0f};
+}
+// Direction (w = 0), matrix.h:908-916 with v.w = 0.
+MH_HD f3 xform_dir(const M4& T, f3 d) {
+  return f3{T.m[0] * d.x + T.m[1] * d.y + T.m[2] * d.z + T.m[3] * 0.0f,
+            T.m[4] * d.x + T.m[5] * d.y + T.m[6] * d.z + T.m[7] * 0.0f,
+            T.m[8] * d.x + T.m[9] * d.y + T.m[10] * d.z + T.m[11] * 0.0f};
+}
+// FP64 rows of a pose, used only for SDF values.
+struct D34 { double m[12]; };
+
+// ---------------------------------------------------------------- geometry
+// GeometryHelper, geometry/geometry_helper.h:36-217.
+struct Geo {
+  float voxel_size;
+  float truncation_distance;
+  float truncation_distance_scale;
+  float sdf_upper_bound;
+  float weight_sample;
+};
+struct Cam {  // SensorParams / RayCasterParams intrinsics
+  float fx, fy, cx, cy;
+  uint32_t width, height;
+  float min_depth, max_depth;
+};
+
+MH_HD float truncate_distance(const Geo& g, float z) {
+  return g.truncation_distance + g.truncation_distance_scale * z;
+}
+// WorldToVoxeli: int(p/vs + sign(p)*0.5), half away from zero.
+MH_HD i3 world_to_voxel(const Geo& g, f3 w) {
+  f3 p = w / g.voxel_size;
+  return i3{f2i(p.x + (float)sgn(p.x) * 0.5f), f2i(p.y + (float)sgn(p.y) * 0.5f),
+            f2i(p.z + (float)sgn(p.z) * 0.5f)};
+}
+MH_HD int voxel_to_block1(int v) { return (v - (v < 0) * (kBlockSide - 1)) / kBlockSide; }
+MH_HD i3 voxel_to_block(i3 v) {
+  return i3{voxel_to_block1(v.x), voxel_to_block1(v.y), voxel_to_block1(v.z)};
+}
+MH_HD f3 voxel_to_world(const Geo& g, i3 v) {
+  return f3{(float)v.x * g.voxel_size, (float)v.y * g.voxel_size, (float)v.z * g.voxel_size};
+}
+MH_HD f3 block_to_world(const Geo& g, i3 b) {
+  return voxel_to_world(g, i3{b.x * kBlockSide, b.y * kBlockSide, b.z * kBlockSide});
+}
+MH_HD i3 world_to_block(const Geo& g, f3 w) { return voxel_to_block(world_to_voxel(g, w)); }
+
+// CameraProjectToImagef / ImageReprojectToCamera, geometry_helper.h:141-168.
+MH_HD float proj_u(const Cam& c, f3 p) { return p.x * c.fx / p.z + c.cx; }
+MH_HD float proj_v(const Cam& c, f3 p) { return p.y * c.fy / p.z + c.cy; }
+MH_HD f3 reproject(const Cam& c, uint32_t ux, uint32_t uy, float depth) {
+  const float x = ((float)ux - c.cx) / c.fx;
+  const float y = ((float)uy - c.cy) / c.fy;
+  return f3{depth * x, depth * y, depth};
+}
+MH_HD float normalize_depth(float z, float mn, float mx) { return (z - mn) / (mx - mn); }
+
+// IsPointInCameraFrustum / IsBlockInCameraFrustum, geometry_helper.h:186-216.
+MH_HD bool point_in_frustum(const M4& cTw, f3 w, const Cam& c) {
+  f3 p = xform(cTw, w);
+  float u = proj_u(c, p), v = proj_v(c, p);
+  float wm1 = (float)c.width - 1.0f, hm1 = (float)c.height - 1.0f;
+  float nx = (2.0f * u - wm1) / wm1;
+  float ny = (hm1 - 2.0f * v) / hm1;
+  float nz = normalize_depth(p.z, c.min_depth, c.max_depth);
+  nx *= 0.95f; ny *= 0.95f; nz *= 0.95f;
+  return !(nx < -1.0f || nx > 1.0f || ny < -1.0f || ny > 1.0f || nz < 0.0f || nz > 1.0f);
+}
+MH_HD bool block_in_frustum(const Geo& g, const M4& cTw, i3 b, const Cam& c) {
+  f3 w = block_to_world(g, b);
+  float h = g.voxel_size * 0.5f * ((float)kBlockSide - 1.0f);
+  return point_in_frustum(cTw, f3{w.x + h, w.y + h, w.z + h}, c);
+}
+
+// ---------------------------------------------------------------- hash table
+// Entry: 16 bytes.  The block position is packed into a 64-bit key
+// (21 bits per axis, two's complement) so that a slot is claimed with one
+// 64-bit compare-and-swap; ptr is the block-pool index, offset the list link
+// relative to the bucket's last slot (core/hash_entry.h:11-29).
+struct alignas(16) Entry {
+  uint64_t key;
+  int32_t ptr;
+  uint32_t offset;
+};
+constexpr int kCoordBits = 21;
+constexpr int kCoordMax = (1 << (kCoordBits - 1)) - 1;
+MH_HD bool coord_ok(i3 p) {
+  return p.x >= -kCoordMax && p.x <= kCoordMax && p.y >= -kCoordMax && p.y <= kCoordMax &&
+         p.z >= -kCoordMax && p.z <= kCoordMax;
+}
+MH_HD uint64_t pack_key(i3 p) {
+  const uint64_t m = (1ull << kCoordBits) - 1;
+  return (((uint64_t)(uint32_t)p.x & m) << (2 * kCoordBits)) |
+         (((uint64_t)(uint32_t)p.y & m) << kCoordBits) | ((uint64_t)(uint32_t)p.z & m);
+}
+MH_HD int sext21(uint64_t v) {
+  int x = (int)(v & ((1u << kCoordBits) - 1));
+  return (x ^ (1 << (kCoordBits - 1))) - (1 << (kCoordBits - 1));
+}
+MH_HD i3 unpack_key(uint64_t k) {
+  return i3{sext21(k >> (2 * kCoordBits)), sext21(k >> kCoordBits), sext21(k)};
+}
+// HashBucketForBlockPos, core/hash_table.h:259-269: 32-bit wrapping products,
+// xor, then an *unsigned* modulo (bucket_count is uint, :19).
+MH_HD uint32_t hash_bucket(i3 p, uint32_t bucket_count) {
+  uint32_t h = ((uint32_t)p.x * 73856093u) ^ ((uint32_t)p.y * 19349669u) ^
+               ((uint32_t)p.z * 83492791u);
+  return h % bucket_count;
+}
+
+// Device-wide counters and status, one instance per volume.
+struct Counters {
+  int32_t block_top;       // heap counter (hash_table.h:279-286)
+  uint32_t block_hwm;      // 1 + largest block index ever handed out
+  uint32_t cand_count;     // worklist length
+  uint32_t ovf_count;      // keys queued for list insertion this frame
+  int32_t vertex_top;      // vertex heap counter (mesh.h:53-59)
+  int32_t triangle_top;    // triangle heap counter (mesh.h:67-73)
+  uint32_t flags;          // MH_STATUS_* bits
+  uint32_t dropped;
+  uint32_t dda_cap;
+  uint32_t list_inserts;
+  uint32_t pad0;
+  // work counters (accumulated since reset), for throughput and roofline
+  unsigned long long voxel_updates;      // voxels whose fused state changed
+  unsigned long long blocks_integrated;  // worklist blocks visited by fusion
+  unsigned long long mc_vertex_writes;   // vertex records written by MC pass 1
+  unsigned long long mc_triangle_writes; // triangles assigned by MC pass 2
+  unsigned long long mc_blocks;          // worklist blocks meshed
+  unsigned long long ray_samples;        // valid trilinear samples of the raycaster
+};
+
+// Sum v over the wave and add it to *dst once.  Every lane of the wave must
+// reach the call.
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63u) == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
+// Vertex record (core/vertex.h:11-26), 48 bytes: three 16-byte stores.
+struct alignas(16) Vertex {
+  float px, py, pz, radius;
+  float nx, ny, nz;
+  int32_t ref_count;
+  float cr, cg, cb, pad;
+};
+
+// Everything a kernel needs, passed by value.
+struct View {
+  // hash table
+  Entry* entries;
+  uint32_t bucket_count, bucket_size, entry_count, linked_list_size;
+  uint64_t* block_keys;  // key of the block at each pool index (kEmptyKey: free)
+  int32_t* block_heap;
+  uint32_t capacity;
+  Counters* ctr;
+  uint64_t* ovf;         // overflow queue
+  // voxels, pool-index-major, 512 per block
+  float* sdf;
+  float* weight;         // inv_sigma2 of the reference's Voxel
+  uint32_t* color;       // RGBA8
+  float* va;             // Beta a (PSDF only)
+  float* vb;             // Beta b (PSDF only)
+  // mesh units (core/voxel.h:27-60), 512 per block
+  int32_t* vptr;         // 3 per voxel: x, y, z edge vertex
+  int32_t* tptr;         // 5 per voxel
+  uint16_t* cube;        // curr | prev << 8
+  int32_t* surfel;       // per block: inner, boundary surfel counts (block.h:17-18)
+  // mesh pools
+  Vertex* vertices;
+  int32_t* vertex_heap;
+  uint32_t max_vertices;
+  int32_t* triangles;    // 3 per triangle
+  int32_t* triangle_heap;
+  uint32_t max_triangles;
+  // worklist
+  int32_t* cand;
+  // primal-dual state (lazily allocated), 512 per block
+  float* pd_sdf0;
+  float* pd_sdf_bar;
+  float* pd_w;
+  float* pd_p;           // 3 per voxel, interleaved
+  uint8_t* pd_mask;
+  // geometry and frame
+  Geo geo;
+  Cam cam;
+  float range_factor;
+  float* depth;          // W*H metric depth of the current frame
+  float* inlier;         // W*H PSDF inlier ratio
+  M4 wTc, cTw;           // FP32 pose (decisions)
+  D34 cTw64;             // FP64 world->camera (values)
+};
+
+// GetEntry, hash_table.h:57-93: bucket scan, then the list that starts at
+// the bucket's last slot.  Returns the pool index or kFreePtr.
+__device__ __forceinline__ int32_t hash_find(const View& V, i3 p) {
+  const uint64_t key = pack_key(p);
+  const uint32_t first = hash_bucket(p, V.bucket_count) * V.bucket_size;
+  for (uint32_t j = 0; j < V.bucket_size; ++j) {
+    const Entry e = V.entries[first + j];
+    if (e.key == key) return e.ptr;
+  }
+  const uint32_t last = first + V.bucket_size - 1;
+  uint32_t i = last;
+  for (uint32_t it = 0; it < V.linked_list_size; ++it) {
+    const Entry e = V.entries[i];
+    if (e.key == key) return e.ptr;
+    if (e.offset == 0) break;
+    i = (last + e.offset) % V.entry_count;
+  }
+  return kFreePtr;
+}
+
+__device__ __forceinline__ void flag(const View& V, uint32_t bit) {
+  atomicOr(&V.ctr->flags, bit);
+}
+
+// Block-pool pop (hash_table.h:279-286): the heap holds free indices,
+// initialised so that pops return 0, 1, 2, ...
+__device__ __forceinline__ int32_t alloc_block(const View& V) {
+  const int32_t addr = atomicSub(&V.ctr->block_top, 1);
+  if (addr < 0) {
+    flag(V, MH_STATUS_BLOCK_HEAP_EXHAUSTED);
+    return kFreePtr;
+  }
+  const int32_t p = V.block_heap[addr];
+  atomicMax(&V.ctr->block_hwm, (uint32_t)p + 1u);
+  return p;
+}
+
+// Local voxel index <-> offset, geometry_helper.h:111-129.
+MH_HD i3 devectorize(uint32_t i) {
+  return i3{(int)(i % 8u), (int)((i % 64u) / 8u), (int)(i / 64u)};
+}
+
+}  // namespace mh

@@ -1,0 +1,260 @@
+"""Python view of the C ABI: a Volume handle plus the reference engine's
+per-frame driver (MainEngine::Mapping / Meshing / Visualize,
+engine/main_engine.cc:100-174,231-290).
+
+Every method calls libmeshhashing_amd.so; nothing here computes on the CPU
+except array bookkeeping for dumps.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as A
+from .configs import Config
+
+
+def synth_pose(cfg: Config, frame: int) -> np.ndarray:
+    lib = A.load()
+    out = (C.c_float * 16)()
+    A.check(lib.mh_synth_pose(C.byref(cfg.synth_spec()), frame, out), "mh_synth_pose")
+    return np.array(out[:], dtype=np.float32).reshape(4, 4)
+
+
+def synth_depth(cfg: Config, frame: int) -> np.ndarray:
+    lib = A.load()
+    d = np.zeros((cfg.height, cfg.width), dtype=np.uint16)
+    A.check(lib.mh_synth_depth(C.byref(cfg.synth_spec()), frame, A.ptr(d, C.c_uint16)),
+            "mh_synth_depth")
+    return d
+
+
+def synth_depth_exact(cfg: Config, wTc: np.ndarray) -> np.ndarray:
+    lib = A.load()
+    d = np.zeros((cfg.height, cfg.width), dtype=np.float64)
+    A.check(lib.mh_synth_depth_exact(C.byref(cfg.synth_spec()), 0, A.f32x16(wTc),
+                                     A.ptr(d, C.c_double)), "mh_synth_depth_exact")
+    return d
+
+
+class Volume:
+    """Hash table + block pool + mesh + frame buffers on the current GPU."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self.lib = A.load()
+        self.h = C.c_void_p()
+        A.check(self.lib.mh_volume_create(C.byref(cfg.hash_params()), C.byref(cfg.volume_params()),
+                                          C.byref(cfg.mesh_params()), C.byref(cfg.sensor_params()),
+                                          1 if cfg.psdf else 0, C.byref(self.h)),
+                "mh_volume_create")
+
+    def close(self):
+        if self.h:
+            self.lib.mh_volume_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- frame -------------------------------------------------------------
+    def upload_depth(self, depth_u16: np.ndarray, wTc: np.ndarray):
+        d = np.ascontiguousarray(depth_u16, dtype=np.uint16)
+        A.check(self.lib.mh_frame_upload_depth_u16(self.h, A.ptr(d, C.c_uint16), A.f32x16(wTc)),
+                "mh_frame_upload_depth_u16")
+
+    def set_depth_device(self, dev_ptr: int, wTc: np.ndarray):
+        A.check(self.lib.mh_frame_depth_u16_device(self.h, C.c_void_p(dev_ptr), A.f32x16(wTc)),
+                "mh_frame_depth_u16_device")
+
+    def set_stream(self, stream_ptr: int):
+        A.check(self.lib.mh_volume_set_stream(self.h, C.c_void_p(stream_ptr)), "set_stream")
+
+    # -- stages --------------------------------------------------------------
+    def alloc(self):
+        A.check(self.lib.mh_alloc_blocks(self.h), "mh_alloc_blocks")
+
+    def collect_visible(self, sync: bool = False) -> int | None:
+        n = C.c_uint32(0)
+        A.check(self.lib.mh_collect_visible(self.h, C.byref(n) if sync else None),
+                "mh_collect_visible")
+        return n.value if sync else None
+
+    def collect_all(self, sync: bool = False) -> int | None:
+        n = C.c_uint32(0)
+        A.check(self.lib.mh_collect_all(self.h, C.byref(n) if sync else None), "mh_collect_all")
+        return n.value if sync else None
+
+    def integrate_tsdf(self):
+        A.check(self.lib.mh_integrate_tsdf(self.h), "mh_integrate_tsdf")
+
+    def predict_inlier(self):
+        A.check(self.lib.mh_predict_inlier(self.h), "mh_predict_inlier")
+
+    def integrate_psdf(self):
+        A.check(self.lib.mh_integrate_psdf(self.h), "mh_integrate_psdf")
+
+    def marching_cubes(self, params: A.McParams | None = None):
+        p = params if params is not None else self.cfg.mc_params()
+        A.check(self.lib.mh_marching_cubes(self.h, C.byref(p)), "mh_marching_cubes")
+
+    def raycast(self, cTw: np.ndarray, depth_ptr: int, vertex_ptr: int = 0, normal_ptr: int = 0,
+                params: A.RayCasterParams | None = None):
+        p = params if params is not None else self.cfg.raycaster_params()
+        A.check(self.lib.mh_raycast(self.h, C.byref(p), A.f32x16(cTw), C.c_void_p(depth_ptr or None),
+                                    C.c_void_p(vertex_ptr or None), C.c_void_p(normal_ptr or None)),
+                "mh_raycast")
+
+    def pd_init(self):
+        A.check(self.lib.mh_primal_dual_init(self.h), "mh_primal_dual_init")
+
+    def pd_iterate(self, lam: float, sigma: float, tau: float, energy: bool = False):
+        e = (C.c_double * 3)()
+        A.check(self.lib.mh_primal_dual_iterate(self.h, lam, sigma, tau, e if energy else None),
+                "mh_primal_dual_iterate")
+        return tuple(e) if energy else None
+
+    def synchronize(self):
+        A.check(self.lib.mh_synchronize(self.h), "mh_synchronize")
+
+    def status(self) -> dict:
+        s = A.Status()
+        A.check(self.lib.mh_get_status(self.h, C.byref(s)), "mh_get_status")
+        d = {k: getattr(s, k) for k, _ in A.Status._fields_}
+        d["flag_names"] = [n for n, b in A.STATUS_BITS.items() if s.flags & b]
+        return d
+
+    def inlier_ptr(self) -> int:
+        p = C.c_void_p()
+        A.check(self.lib.mh_inlier_ratio_device_ptr(self.h, C.byref(p)), "inlier ptr")
+        return p.value
+
+    def depth_ptr(self) -> int:
+        p = C.c_void_p()
+        A.check(self.lib.mh_frame_depth_device_ptr(self.h, C.byref(p)), "depth ptr")
+        return p.value
+
+    # -- dumps -------------------------------------------------------------------
+    def dump_blocks(self) -> dict:
+        n = C.c_uint32(0)
+        A.check(self.lib.mh_dump_blocks(self.h, 0, None, None, None, None, None, None,
+                                        C.byref(n)), "count")
+        nb = n.value
+        pos = np.zeros((nb, 3), np.int32)
+        sdf = np.zeros((nb, 512), np.float32)
+        w = np.zeros((nb, 512), np.float32)
+        col = np.zeros((nb, 512), np.uint32)
+        a = np.zeros((nb, 512), np.float32) if self.cfg.psdf else None
+        b = np.zeros((nb, 512), np.float32) if self.cfg.psdf else None
+        A.check(self.lib.mh_dump_blocks(self.h, nb, A.ptr(pos, C.c_int32), A.ptr(sdf, C.c_float),
+                                        A.ptr(w, C.c_float), A.ptr(col, C.c_uint32),
+                                        A.ptr(a, C.c_float), A.ptr(b, C.c_float), C.byref(n)),
+                "mh_dump_blocks")
+        out = {"pos": pos, "sdf": sdf, "weight": w, "color": col}
+        if self.cfg.psdf:
+            out["a"], out["b"] = a, b
+        return out
+
+    def load_blocks(self, pos, sdf, weight, color=None, a=None, b=None):
+        pos = np.ascontiguousarray(pos, np.int32)
+        sdf = np.ascontiguousarray(sdf, np.float32)
+        weight = np.ascontiguousarray(weight, np.float32)
+        color = None if color is None else np.ascontiguousarray(color, np.uint32)
+        a = None if a is None else np.ascontiguousarray(a, np.float32)
+        b = None if b is None else np.ascontiguousarray(b, np.float32)
+        A.check(self.lib.mh_load_blocks(self.h, len(pos), A.ptr(pos, C.c_int32),
+                                        A.ptr(sdf, C.c_float), A.ptr(weight, C.c_float),
+                                        A.ptr(color, C.c_uint32), A.ptr(a, C.c_float),
+                                        A.ptr(b, C.c_float)), "mh_load_blocks")
+
+    def dump_hash(self) -> np.ndarray:
+        n = C.c_uint32(0)
+        A.check(self.lib.mh_dump_hash(self.h, 0, None, C.byref(n)), "count")
+        rec = np.zeros((n.value, 6), np.int32)
+        A.check(self.lib.mh_dump_hash(self.h, n.value, A.ptr(rec, C.c_int32), C.byref(n)),
+                "mh_dump_hash")
+        return rec
+
+    def dump_candidates(self) -> np.ndarray:
+        n = C.c_uint32(0)
+        A.check(self.lib.mh_dump_candidates(self.h, 0, None, C.byref(n)), "count")
+        pos = np.zeros((n.value, 3), np.int32)
+        A.check(self.lib.mh_dump_candidates(self.h, n.value, A.ptr(pos, C.c_int32), C.byref(n)),
+                "mh_dump_candidates")
+        return pos
+
+    def dump_cube_indices(self) -> np.ndarray:
+        n = C.c_uint32(0)
+        A.check(self.lib.mh_dump_cube_indices(self.h, 0, None, C.byref(n)), "count")
+        cube = np.zeros((n.value, 512), np.uint16)
+        A.check(self.lib.mh_dump_cube_indices(self.h, n.value, A.ptr(cube, C.c_uint16),
+                                              C.byref(n)), "mh_dump_cube_indices")
+        return cube
+
+    def dump_mesh(self) -> dict:
+        nt, nv = C.c_uint32(0), C.c_uint32(0)
+        A.check(self.lib.mh_dump_mesh(self.h, 0, None, None, 0, None, None, None, C.byref(nt),
+                                      C.byref(nv)), "count")
+        tri = np.zeros((nt.value, 9), np.int32)
+        tcube = np.zeros((nt.value, 4), np.int32)
+        vkey = np.zeros((nv.value, 4), np.int32)
+        vval = np.zeros((nv.value, 10), np.float32)
+        vref = np.zeros(nv.value, np.int32)
+        A.check(self.lib.mh_dump_mesh(self.h, nt.value, A.ptr(tri, C.c_int32),
+                                      A.ptr(tcube, C.c_int32), nv.value, A.ptr(vkey, C.c_int32),
+                                      A.ptr(vval, C.c_float), A.ptr(vref, C.c_int32),
+                                      C.byref(nt), C.byref(nv)), "mh_dump_mesh")
+        return {"tri": tri, "tri_cube": tcube, "vkey": vkey, "vval": vval, "vref": vref}
+
+    def dump_primal_dual(self) -> dict:
+        n = C.c_uint32(0)
+        A.check(self.lib.mh_dump_primal_dual(self.h, 0, None, None, C.byref(n)), "count")
+        pdv = np.zeros((n.value, 6, 512), np.float32)
+        mask = np.zeros((n.value, 512), np.uint8)
+        A.check(self.lib.mh_dump_primal_dual(self.h, n.value, A.ptr(pdv, C.c_float),
+                                             A.ptr(mask, C.c_uint8), C.byref(n)),
+                "mh_dump_primal_dual")
+        return {"sdf0": pdv[:, 0], "sdf_bar": pdv[:, 1], "w_pd": pdv[:, 2], "p": pdv[:, 3:6],
+                "mask": mask}
+
+
+class MainEngine:
+    """Per-frame driver with the reference's stage order.
+
+    Mapping: AllocBlockArray -> CollectBlocksInFrustum -> UpdateBlocksSimple
+    (or PredictOutlierRatio + UpdateBlocksBayesian); Meshing: MarchingCubes on
+    the same worklist; Visualize: RayCaster::Cast.
+    """
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self.vol = Volume(cfg)
+
+    def mapping(self):
+        v = self.vol
+        v.alloc()
+        v.collect_visible()
+        if self.cfg.psdf:
+            v.predict_inlier()
+            v.integrate_psdf()
+        else:
+            v.integrate_tsdf()
+
+    def meshing(self):
+        self.vol.marching_cubes()
+
+    def frame(self, depth_u16: np.ndarray, wTc: np.ndarray, mesh: bool = True):
+        self.vol.upload_depth(depth_u16, wTc)
+        self.mapping()
+        if mesh:
+            self.meshing()
