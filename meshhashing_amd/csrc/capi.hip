@@ -89,6 +89,7 @@ struct mh_volume {
   double* pd_partials = nullptr;
   double* pd_energy = nullptr;
   uint32_t pd_npartials = 0;
+  uint32_t mc_pass = 0;
   std::vector<void*> owned;
 
   int grid_wg256() const { return cus * 8; }
@@ -225,6 +226,7 @@ int mh_volume_create(const mh_hash_params* hash, const mh_volume_params* vol,
   A(&V.entries, hash->entry_count);
   A(&V.block_keys, cap);
   A(&V.block_heap, cap);
+  A(&V.occ, (size_t)1 << (kOccBits - 5));
   A(&V.ctr, 1);
   A(&V.ovf, kOverflowQueue);
   A(&V.sdf, nvox);
@@ -238,6 +240,7 @@ int mh_volume_create(const mh_hash_params* hash, const mh_volume_params* vol,
   A(&V.tptr, nvox * 5);
   A(&V.cube, nvox);
   A(&V.surfel, cap * 2);
+  A(&V.block_mark, cap);
   A(&V.vertices, mesh->max_vertex_count);
   A(&V.vertex_heap, mesh->max_vertex_count);
   A(&V.triangles, (size_t)mesh->max_triangle_count * 3);
@@ -424,6 +427,7 @@ int mh_marching_cubes(mh_volume* v, const mh_mc_params* p) {
   c.weight_min = p->weight_min < 0.0f ? inv * inv : p->weight_min;
   c.snap = p->snap_distance;
   c.flat_normals = 0;
+  c.pass = ++v->mc_pass;
   launch_marching_cubes(v->V, c, v->stream, v->grid_wg512());
   HIP_TRY(hipGetLastError());
   return MH_OK;
@@ -446,6 +450,7 @@ int mh_raycast(mh_volume* v, const mh_raycaster_params* p, const float cTw[16], 
   R.sample_sdf_threshold = p->sample_sdf_threshold;
   R.sdf_threshold = p->sdf_threshold;
   R.enable_gradients = p->enable_gradients;
+  R.inv_vs = 1.0f / v->vp.voxel_size;
   std::memcpy(R.cTw.m, cTw, sizeof(float) * 16);
   invert4<float>(cTw, R.wTc.m);
   launch_raycast(v->V, R, depth_dev, vertex_dev, normal_dev, v->stream);

@@ -24,7 +24,13 @@ __global__ void reset_pool_kernel(View V) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
     V.block_heap[i] = (int32_t)(cap - 1u - i);
     V.block_keys[i] = kEmptyKey;
+    V.block_mark[i] = 0u;
+    V.surfel[2 * i] = 0;
+    V.surfel[2 * i + 1] = 0;
   }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (1u << (kOccBits - 5));
+       i += gridDim.x * blockDim.x)
+    V.occ[i] = 0u;
   const uint32_t nv = V.max_vertices;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
     V.vertex_heap[i] = (int32_t)(nv - 1u - i);
@@ -128,6 +134,8 @@ __device__ void insert_block(const View& V, i3 p) {
     const uint64_t old = atomicCAS((unsigned long long*)&e->key, (unsigned long long)kEmptyKey,
                                    (unsigned long long)key);
     if (old == kEmptyKey) {
+      const uint32_t ob = occ_bit(key);
+      atomicOr(&V.occ[ob >> 5], 1u << (ob & 31u));
       const int32_t ptr = alloc_block(V);
       e->ptr = ptr;
       if (ptr >= 0) V.block_keys[ptr] = key;
@@ -219,6 +227,8 @@ __global__ void list_insert_kernel(View V) {
     const i3 p = unpack_key(key);
     const uint32_t first = hash_bucket(p, V.bucket_count) * V.bucket_size;
     if (key_present(V, key, first)) continue;
+    const uint32_t ob = occ_bit(key);
+    V.occ[ob >> 5] |= 1u << (ob & 31u);
     bool done = false;
     for (uint32_t j = 0; j < V.bucket_size && !done; ++j) {
       Entry* e = &V.entries[first + j];

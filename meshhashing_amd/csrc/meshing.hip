@@ -88,7 +88,7 @@ struct SVal {
 // the block's voxel origin.  Corner voxels come from rounding the corner's
 // world position, exactly as the reference, so ties and negative
 // coordinates pick the same voxels.
-template <bool PSDF>
+template <bool PSDF, bool FULL = true>
 __device__ bool shell_value(const Shell<PSDF>& S, const Geo& g, i3 base, f3 pos, SVal* out,
                             bool* oob) {
   const float vs = g.voxel_size;
@@ -100,6 +100,7 @@ __device__ bool shell_value(const Shell<PSDF>& S, const Geo& g, i3 base, f3 pos,
   const int cy[2] = {vox1(pcy + 0.0f, vs) - base.y, vox1(pcy + vs, vs) - base.y};
   const int cz[2] = {vox1(pcz + 0.0f, vs) - base.z, vox1(pcz + vs, vs) - base.z};
   float sdf = 0.0f, a = 0.0f, b = 0.0f, rad = 0.0f;
+#pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int mx = (i >> 2) & 1, my = (i >> 1) & 1, mz = i & 1;
     const int lx = cx[mx], ly = cy[my], lz = cz[mz];
@@ -111,11 +112,13 @@ __device__ bool shell_value(const Shell<PSDF>& S, const Geo& g, i3 base, f3 pos,
     if (!S.ok[s]) return false;
     const float w = (mx ? rx : 1.0f - rx) * (my ? ry : 1.0f - ry) * (mz ? rz : 1.0f - rz);
     sdf += w * S.sdf[s];
-    if (PSDF) {
-      a += w * S.a[s];
-      b += w * S.b[s];
+    if (FULL) {
+      if (PSDF) {
+        a += w * S.a[s];
+        b += w * S.b[s];
+      }
+      rad += w * sqrtf(1.0f / S.w[s]);
     }
-    rad += w * sqrtf(1.0f / S.w[s]);
   }
   out->sdf = sdf;
   out->a = a;
@@ -173,13 +176,14 @@ __device__ void vertex_payload(const Shell<PSDF>& S, const Geo& g, i3 base, f3 p
   const float h = 0.5f * g.voxel_size;
   bool valid = true;
   float sp[3] = {0, 0, 0}, sn[3] = {0, 0, 0};
+#pragma unroll
   for (int k = 0; k < 3 && valid; ++k) {
     const f3 dp = mk3(k == 0 ? h : 0.0f, k == 1 ? h : 0.0f, k == 2 ? h : 0.0f);
     SVal q{};
-    valid = shell_value(S, g, base, p - dp, &q, oob);
+    valid = shell_value<PSDF, false>(S, g, base, p - dp, &q, oob);
     sn[k] = q.sdf;
     if (!valid) break;
-    valid = shell_value(S, g, base, p + dp, &q, oob);
+    valid = shell_value<PSDF, false>(S, g, base, p + dp, &q, oob);
     sp[k] = q.sdf;
   }
   const float vs = g.voxel_size;
@@ -191,117 +195,193 @@ __device__ void vertex_payload(const Shell<PSDF>& S, const Geo& g, i3 base, f3 p
   vt->nz = n.z;
 }
 
-__device__ __forceinline__ int32_t pop_heap(int32_t* top, const int32_t* heap, const View& V,
-                                            uint32_t bit) {
-  const int32_t a = atomicSub(top, 1);
-  if (a < 0) {
-    atomicAdd(top, 1);
-    flag(V, bit);
-    return kFreePtr;
+// ---------------------------------------------------------------- pass 1
+// Corner checks of the cube at local voxel (x, y, z) (marching_cubes.cu:
+// 96-130): every corner must exist, |sdf| <= threshold, rho >= rho_min and
+// weight >= weight_min; returns validity and the cube index.
+template <bool PSDF>
+__device__ __forceinline__ bool cube_index(const Shell<PSDF>& S, int x, int y, int z,
+                                           const McConst& mc, int* cube) {
+  int c = 0;
+  for (int k = 0; k < 8; ++k) {
+    const int s = sidx(x + corner_off(k, 0), y + corner_off(k, 1), z + corner_off(k, 2));
+    if (!S.ok[s]) return false;
+    const float d = S.sdf[s];
+    if (fabsf(d) > mc.sdf_threshold) return false;
+    const float a = PSDF ? S.a[s] : 0.0f, bb = PSDF ? S.b[s] : 0.0f;
+    const float rho = a / (a + bb);
+    if (rho < mc.rho_min || S.w[s] < mc.weight_min) return false;
+    if (d < 0.0f) c |= 1 << k;
   }
-  return heap[a];
-}
-__device__ __forceinline__ void push_heap(int32_t* top, int32_t* heap, int32_t p) {
-  const int32_t a = atomicAdd(top, 1);
-  heap[a + 1] = p;
+  *cube = c;
+  return true;
 }
 
-// ---------------------------------------------------------------- pass 1
+// Cubes at local coordinates -1..8 (10^3), for the writer rule below.
+constexpr int kCubeHalo = 10;
+__device__ __forceinline__ int cidx(int x, int y, int z) {
+  return (x + 1) + kCubeHalo * (y + 1) + kCubeHalo * kCubeHalo * (z + 1);
+}
+
+// Payload of vertex (owner voxel at local (key & 15, key >> 4 & 15,
+// key >> 8 & 15), axis key >> 12) into vertex record vp: VertexIntersection
+// (marching_cubes.cu:16-28) in the canonical endpoint order, then
+// AllocateVertexWithMutex's payload (:45-72).
+constexpr uint32_t kQueue = 2048;
+template <bool PSDF>
+__device__ void write_vertex(const Shell<PSDF>& S, const View& V, const McConst& mc, i3 base,
+                             uint32_t key, int32_t vp, bool* oob) {
+  const Geo& g = V.geo;
+  const int ox = (int)(key & 15u), oy = (int)((key >> 4) & 15u), oz = (int)((key >> 8) & 15u);
+  const int ax = (int)(key >> 12);
+  const float vA = S.sdf[sidx(ox, oy, oz)];
+  const float vB = S.sdf[sidx(ox + (ax == 0), oy + (ax == 1), oz + (ax == 2))];
+  const f3 pA = voxel_to_world(g, i3{base.x + ox, base.y + oy, base.z + oz});
+  const f3 pB = mk3(pA.x + (ax == 0 ? g.voxel_size : 0.0f), pA.y + (ax == 1 ? g.voxel_size : 0.0f),
+                    pA.z + (ax == 2 ? g.voxel_size : 0.0f));
+  f3 p;
+  if ((double)fabsf(vA) < mc.snap) {
+    p = pA;
+  } else if ((double)fabsf(vB) < mc.snap) {
+    p = pB;
+  } else {
+    const float mu = (0.0f - vA) / (vB - vA);
+    p = mk3(pA.x + mu * (pB.x - pA.x), pA.y + mu * (pB.y - pA.y), pA.z + mu * (pB.z - pA.z));
+  }
+  Vertex vt;
+  vertex_payload<PSDF>(S, g, base, p, &vt, oob);
+  Vertex* dst = &V.vertices[vp];
+  *reinterpret_cast<float4*>(&dst->px) = make_float4(vt.px, vt.py, vt.pz, vt.radius);
+  dst->nx = vt.nx;
+  dst->ny = vt.ny;
+  dst->nz = vt.nz;
+  dst->cr = vt.cr;
+  dst->cg = vt.cg;
+  dst->cb = vt.cb;
+}
+
 // SurfelExtractionKernel (marching_cubes.cu:75-170): cube index of every
-// cube of the block, and the vertex of every active edge.
+// cube of the block, and the vertex of every active edge.  A vertex is
+// shared by up to four cubes.  Its payload is written once per pass, by a
+// designated cube: the first, in the order (0,0), (1,0), (0,1), (1,1) of
+// the offsets across the edge, that is valid and belongs to a block meshed
+// in this pass.  Every cube can evaluate that rule from the staged shell (a
+// 10^3 halo of cube validities and the meshed flags of the 27 blocks), so
+// no atomics are needed; the other cubes only make sure the vertex slot is
+// allocated.  Any writer would store the same bits (canonical vertices).
 template <bool PSDF>
 __global__ __launch_bounds__(512) void mc_surfel_kernel(View V, McConst mc) {
   __shared__ Shell<PSDF> S;
   __shared__ int32_t nb[27];
+  __shared__ uint8_t meshed[27];
+  __shared__ uint8_t cv[kCubeHalo * kCubeHalo * kCubeHalo];
+  __shared__ int32_t qptr[kQueue];
+  __shared__ uint16_t qkey[kQueue];
+  __shared__ uint32_t qn;
   const uint32_t n = V.ctr->cand_count;
   const Geo g = V.geo;
   uint32_t nwrites = 0;
+  bool oob = false;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&V.ctr->mc_blocks, (unsigned long long)n);
   for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
     const int32_t ptr = V.cand[ci];
     const i3 b = unpack_key(V.block_keys[ptr]);
     stage_shell<PSDF>(V, b, ptr, S, nb);
+    if (threadIdx.x < 27) meshed[threadIdx.x] = nb[threadIdx.x] >= 0 && V.block_mark[nb[threadIdx.x]] == mc.pass;
+    if (threadIdx.x == 0) qn = 0;
+    __syncthreads();
+    for (int s = (int)threadIdx.x; s < kCubeHalo * kCubeHalo * kCubeHalo; s += (int)blockDim.x) {
+      const int x = s % kCubeHalo - 1, y = (s / kCubeHalo) % kCubeHalo - 1, z = s / (kCubeHalo * kCubeHalo) - 1;
+      const int dx = x < 0 ? -1 : (x >= 8 ? 1 : 0), dy = y < 0 ? -1 : (y >= 8 ? 1 : 0),
+                dz = z < 0 ? -1 : (z >= 8 ? 1 : 0);
+      int c;
+      cv[s] = meshed[nb27(dx, dy, dz)] && cube_index(S, x, y, z, mc, &c);
+    }
+    __syncthreads();
     const i3 base = i3{b.x * kBlockSide, b.y * kBlockSide, b.z * kBlockSide};
     const uint32_t i = threadIdx.x;
     const i3 o = devectorize(i);
     const size_t idx = (size_t)ptr * kBlockVoxels + i;
     const uint32_t prev = V.cube[idx] & 255u;
     int cube = 0;
-    bool valid = true;
-    for (int k = 0; k < 8; ++k) {
-      const int s = sidx(o.x + corner_off(k, 0), o.y + corner_off(k, 1), o.z + corner_off(k, 2));
-      if (!S.ok[s]) { valid = false; break; }
-      const float d = S.sdf[s];
-      if (fabsf(d) > mc.sdf_threshold) { valid = false; break; }
-      const float a = PSDF ? S.a[s] : 0.0f, bb = PSDF ? S.b[s] : 0.0f;
-      const float rho = a / (a + bb);
-      if (rho < mc.rho_min || S.w[s] < mc.weight_min) { valid = false; break; }
-      if (d < 0.0f) cube |= 1 << k;
-    }
-    if (!valid) cube = 0;
+    if (!cube_index(S, o.x, o.y, o.z, mc, &cube)) cube = 0;
     V.cube[idx] = (uint16_t)((uint32_t)cube | (prev << 8));
-    bool oob = false;
-    if (cube != 0 && cube != 255) {
-      const unsigned em = kMc.edges[cube];
-      for (int e = 0; e < 12; ++e) {
-        if (!(em & (1u << e))) continue;
-        const int ax = edge_axis(e);
-        const int ox = o.x + edge_owner_off(e, 0), oy = o.y + edge_owner_off(e, 1),
-                  oz = o.z + edge_owner_off(e, 2);
-        const float vA = S.sdf[sidx(ox, oy, oz)];
-        const float vB = S.sdf[sidx(ox + (ax == 0), oy + (ax == 1), oz + (ax == 2))];
-        const f3 pA = voxel_to_world(g, i3{base.x + ox, base.y + oy, base.z + oz});
-        const f3 pB = mk3(pA.x + (ax == 0 ? g.voxel_size : 0.0f), pA.y + (ax == 1 ? g.voxel_size : 0.0f),
-                          pA.z + (ax == 2 ? g.voxel_size : 0.0f));
-        // VertexIntersection (marching_cubes.cu:16-28), canonical order.
-        f3 p;
-        if ((double)fabsf(vA) < mc.snap) {
-          p = pA;
-        } else if ((double)fabsf(vB) < mc.snap) {
-          p = pB;
-        } else {
-          const float mu = (0.0f - vA) / (vB - vA);
-          p = mk3(pA.x + mu * (pB.x - pA.x), pA.y + mu * (pB.y - pA.y), pA.z + mu * (pB.z - pA.z));
-        }
+    const unsigned em = (cube != 0 && cube != 255) ? kMc.edges[cube] : 0u;
+    // uniform loop: every lane takes part in the wave-aggregated pops
+    for (int e = 0; e < 12; ++e) {
+      const bool active = (em >> e) & 1u;
+      const int ax = edge_axis(e);
+      const int offx = edge_owner_off(e, 0), offy = edge_owner_off(e, 1), offz = edge_owner_off(e, 2);
+      const int ox = o.x + offx, oy = o.y + offy, oz = o.z + offz;
+      int32_t* slot = nullptr;
+      int32_t cur = kFreePtr;
+      bool claimed = false;
+      if (active) {
         const int dx = ox >> 3, dy = oy >> 3, dz = oz >> 3;
         const int32_t optr = nb[nb27(dx, dy, dz)];
         const uint32_t oi = (uint32_t)((ox - 8 * dx) + 8 * (oy - 8 * dy) + 64 * (oz - 8 * dz));
-        int32_t* slot = &V.vptr[((size_t)optr * kBlockVoxels + oi) * 3 + ax];
-        int32_t cur = *slot;
-        int32_t vp = cur;
-        bool write = cur >= 0;
+        slot = &V.vptr[((size_t)optr * kBlockVoxels + oi) * 3 + ax];
+        cur = *slot;
         if (cur == kFreePtr) {
-          const int32_t old = atomicCAS(slot, kFreePtr, kLockPtr);
-          if (old == kFreePtr) {
-            vp = pop_heap(&V.ctr->vertex_top, V.vertex_heap, V, MH_STATUS_VERTEX_HEAP_EXHAUSTED);
-            atomicExch(slot, vp);
-            write = vp >= 0;
-          } else {
-            vp = old;
-            write = old >= 0;
-          }
-        }
-        if (write) {
-          Vertex vt;
-          vertex_payload<PSDF>(S, g, base, p, &vt, &oob);
-          Vertex* dst = &V.vertices[vp];
-          *reinterpret_cast<float4*>(&dst->px) = make_float4(vt.px, vt.py, vt.pz, vt.radius);
-          dst->nx = vt.nx;
-          dst->ny = vt.ny;
-          dst->nz = vt.nz;
-          *reinterpret_cast<float4*>(&dst->cr) = make_float4(vt.cr, vt.cg, vt.cb, 0.0f);
-          ++nwrites;
+          cur = atomicCAS(slot, kFreePtr, kLockPtr);
+          claimed = cur == kFreePtr;
         }
       }
+      const int32_t fresh = heap_pop_wave(&V.ctr->vertex_top, V.vertex_heap, claimed,
+                                          &V.ctr->flags, MH_STATUS_VERTEX_HEAP_EXHAUSTED);
+      if (claimed) {
+        atomicExch(slot, fresh);
+        cur = fresh;
+      }
+      if (!active || cur < 0) continue;
+      // writer rule: offsets across the edge in the two other axes (b1 < b2)
+      const int b1 = ax == 0 ? 1 : 0, b2 = ax == 2 ? 1 : 2;
+      const int own[3] = {offx, offy, offz};
+      const int kself = own[b1] + 2 * own[b2];
+      // A slot claimed in this pass is also written by its claimer: the
+      // designated cube may have read it while still locked and skipped it.
+      bool writer = true;
+      for (int k = 0; k < kself && writer && !claimed; ++k) {
+        int q[3] = {ox, oy, oz};
+        q[b1] -= k & 1;
+        q[b2] -= k >> 1;
+        if (cv[cidx(q[0], q[1], q[2])]) writer = false;
+      }
+      if (!writer) continue;
+      // queue the payload; all lanes of the block work the queue below
+      const uint32_t q = atomicAdd(&qn, 1u);
+      const uint32_t key = (uint32_t)ox | ((uint32_t)oy << 4) | ((uint32_t)oz << 8) | ((uint32_t)ax << 12);
+      if (q < kQueue) {
+        qptr[q] = cur;
+        qkey[q] = (uint16_t)key;
+      } else {
+        write_vertex<PSDF>(S, V, mc, base, key, cur, &oob);
+        ++nwrites;
+      }
     }
-    if (oob) flag(V, MH_STATUS_STENCIL_RANGE);
+    __syncthreads();
+    const uint32_t nq = qn < kQueue ? qn : kQueue;
+    for (uint32_t j = threadIdx.x; j < nq; j += blockDim.x) {
+      write_vertex<PSDF>(S, V, mc, base, qkey[j], qptr[j], &oob);
+      ++nwrites;
+    }
     __syncthreads();
   }
+  if (oob) flag(V, MH_STATUS_STENCIL_RANGE);
   wave_add_u64(&V.ctr->mc_vertex_writes, nwrites);
 }
 
+// Meshed-in-this-pass marks for the writer rule.
+__global__ void mc_mark_kernel(View V, uint32_t pass) {
+  const uint32_t n = V.ctr->cand_count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    V.block_mark[V.cand[i]] = pass;
+}
+
 // ---------------------------------------------------------------- pass 2
-// TriangleExtractionKernel (marching_cubes.cu:180-275).
+// TriangleExtractionKernel (marching_cubes.cu:180-275).  A triangle slot
+// whose vertex triple is unchanged is left alone (release + re-assign of
+// the same triple leaves every ref count where it was).
 __global__ __launch_bounds__(512) void mc_triangle_kernel(View V) {
   __shared__ int32_t nb[8];
   __shared__ int counts[2];
@@ -324,8 +404,9 @@ __global__ __launch_bounds__(512) void mc_triangle_kernel(View V) {
       atomicAdd(&counts[inner ? 0 : 1], nv);
     }
     const int cube = V.cube[idx] & 255;
-    if (cube != 0 && cube != 255) {
-      int32_t vps[12];
+    const int nt = (cube != 0 && cube != 255) ? kMc.ntri[cube] : 0;
+    int32_t vps[12];
+    if (nt) {
       const unsigned em = kMc.edges[cube];
       for (int e = 0; e < 12; ++e) {
         vps[e] = kFreePtr;
@@ -338,29 +419,40 @@ __global__ __launch_bounds__(512) void mc_triangle_kernel(View V) {
         const uint32_t oi = (uint32_t)((ox - 8 * dx) + 8 * (oy - 8 * dy) + 64 * (oz - 8 * dz));
         vps[e] = optr < 0 ? kFreePtr : V.vptr[((size_t)optr * kBlockVoxels + oi) * 3 + ax];
       }
-      const int nt = kMc.ntri[cube];
-      for (int t = 0; t < nt; ++t) {
-        const int32_t v0 = vps[kMc.tri[cube][3 * t]], v1 = vps[kMc.tri[cube][3 * t + 1]],
-                      v2 = vps[kMc.tri[cube][3 * t + 2]];
-        if (v0 < 0 || v1 < 0 || v2 < 0) continue;  // vertex pool exhausted in pass 1
-        int32_t tp = V.tptr[5 * idx + t];
-        if (tp == kFreePtr) {
-          tp = pop_heap(&V.ctr->triangle_top, V.triangle_heap, V, MH_STATUS_TRIANGLE_HEAP_EXHAUSTED);
-          if (tp < 0) continue;
-        } else {
-          atomicSub(&V.vertices[V.triangles[3 * tp + 0]].ref_count, 1);
-          atomicSub(&V.vertices[V.triangles[3 * tp + 1]].ref_count, 1);
-          atomicSub(&V.vertices[V.triangles[3 * tp + 2]].ref_count, 1);
+    }
+    for (int t = 0; t < 5; ++t) {  // uniform: wave-aggregated pops below
+      int32_t v0 = kFreePtr, v1 = kFreePtr, v2 = kFreePtr, tp = kFreePtr;
+      bool assign = false;
+      if (t < nt) {
+        v0 = vps[kMc.tri[cube][3 * t]];
+        v1 = vps[kMc.tri[cube][3 * t + 1]];
+        v2 = vps[kMc.tri[cube][3 * t + 2]];
+        if (v0 >= 0 && v1 >= 0 && v2 >= 0) {  // else: vertex pool exhausted in pass 1
+          tp = V.tptr[5 * idx + t];
+          assign = !(tp != kFreePtr && V.triangles[3 * tp + 0] == v0 &&
+                     V.triangles[3 * tp + 1] == v1 && V.triangles[3 * tp + 2] == v2);
         }
-        V.tptr[5 * idx + t] = tp;
-        V.triangles[3 * tp + 0] = v0;
-        V.triangles[3 * tp + 1] = v1;
-        V.triangles[3 * tp + 2] = v2;
-        atomicAdd(&V.vertices[v0].ref_count, 1);
-        atomicAdd(&V.vertices[v1].ref_count, 1);
-        atomicAdd(&V.vertices[v2].ref_count, 1);
-        ++nassign;
       }
+      const bool need = assign && tp == kFreePtr;
+      const int32_t fresh = heap_pop_wave(&V.ctr->triangle_top, V.triangle_heap, need,
+                                          &V.ctr->flags, MH_STATUS_TRIANGLE_HEAP_EXHAUSTED);
+      if (!assign) continue;
+      if (need) {
+        if (fresh < 0) continue;
+        tp = fresh;
+      } else {
+        atomicSub(&V.vertices[V.triangles[3 * tp + 0]].ref_count, 1);
+        atomicSub(&V.vertices[V.triangles[3 * tp + 1]].ref_count, 1);
+        atomicSub(&V.vertices[V.triangles[3 * tp + 2]].ref_count, 1);
+      }
+      V.tptr[5 * idx + t] = tp;
+      V.triangles[3 * tp + 0] = v0;
+      V.triangles[3 * tp + 1] = v1;
+      V.triangles[3 * tp + 2] = v2;
+      atomicAdd(&V.vertices[v0].ref_count, 1);
+      atomicAdd(&V.vertices[v1].ref_count, 1);
+      atomicAdd(&V.vertices[v2].ref_count, 1);
+      ++nassign;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -380,18 +472,20 @@ __global__ __launch_bounds__(256) void mc_recycle_triangles_kernel(View V) {
     const int32_t ptr = V.cand[ci];
     for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x) {
       const size_t idx = (size_t)ptr * kBlockVoxels + i;
-      const int cube = V.cube[idx] & 255;
-      for (int t = kMc.ntri[cube]; t < 5; ++t) {
-        const int32_t tp = V.tptr[5 * idx + t];
-        if (tp == kFreePtr) continue;
-        atomicSub(&V.vertices[V.triangles[3 * tp + 0]].ref_count, 1);
-        atomicSub(&V.vertices[V.triangles[3 * tp + 1]].ref_count, 1);
-        atomicSub(&V.vertices[V.triangles[3 * tp + 2]].ref_count, 1);
-        V.triangles[3 * tp + 0] = -1;  // Triangle::Clear (triangle.h:14-16)
-        V.triangles[3 * tp + 1] = -1;
-        V.triangles[3 * tp + 2] = -1;
-        push_heap(&V.ctr->triangle_top, V.triangle_heap, tp);
-        V.tptr[5 * idx + t] = kFreePtr;
+      const int keep = kMc.ntri[V.cube[idx] & 255];
+      for (int t = 0; t < 5; ++t) {  // uniform: wave-aggregated pushes
+        const int32_t tp = t >= keep ? V.tptr[5 * idx + t] : kFreePtr;
+        const bool release = tp != kFreePtr;
+        if (release) {
+          atomicSub(&V.vertices[V.triangles[3 * tp + 0]].ref_count, 1);
+          atomicSub(&V.vertices[V.triangles[3 * tp + 1]].ref_count, 1);
+          atomicSub(&V.vertices[V.triangles[3 * tp + 2]].ref_count, 1);
+          V.triangles[3 * tp + 0] = -1;  // Triangle::Clear (triangle.h:14-16)
+          V.triangles[3 * tp + 1] = -1;
+          V.triangles[3 * tp + 2] = -1;
+          V.tptr[5 * idx + t] = kFreePtr;
+        }
+        heap_push_wave(&V.ctr->triangle_top, V.triangle_heap, release, tp);
       }
     }
   }
@@ -404,20 +498,22 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(View V) {
     const int32_t ptr = V.cand[ci];
     for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x) {
       const size_t idx = (size_t)ptr * kBlockVoxels + i;
-      for (int ax = 0; ax < 3; ++ax) {
+      for (int ax = 0; ax < 3; ++ax) {  // uniform: wave-aggregated pushes
         const int32_t vp = V.vptr[3 * idx + ax];
-        if (vp < 0) continue;
-        if (V.vertices[vp].ref_count != 0) continue;
-        Vertex z{};
-        V.vertices[vp] = z;
-        push_heap(&V.ctr->vertex_top, V.vertex_heap, vp);
-        V.vptr[3 * idx + ax] = kFreePtr;
+        const bool release = vp >= 0 && V.vertices[vp].ref_count == 0;
+        if (release) {
+          Vertex z{};
+          V.vertices[vp] = z;
+          V.vptr[3 * idx + ax] = kFreePtr;
+        }
+        heap_push_wave(&V.ctr->vertex_top, V.vertex_heap, release, vp);
       }
     }
   }
 }
 
 void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int grid) {
+  mc_mark_kernel<<<grid / 4, 256, 0, s>>>(V, p.pass);
   if (V.va)
     mc_surfel_kernel<true><<<grid, 512, 0, s>>>(V, p);
   else

@@ -222,6 +222,7 @@ struct View {
   uint64_t* block_keys;  // key of the block at each pool index (kEmptyKey: free)
   int32_t* block_heap;
   uint32_t capacity;
+  uint32_t* occ;         // occupancy filter, 2^kOccBits bits
   Counters* ctr;
   uint64_t* ovf;         // overflow queue
   // voxels, pool-index-major, 512 per block
@@ -235,6 +236,7 @@ struct View {
   int32_t* tptr;         // 5 per voxel
   uint16_t* cube;        // curr | prev << 8
   int32_t* surfel;       // per block: inner, boundary surfel counts (block.h:17-18)
+  uint32_t* block_mark;  // per block: last meshing pass that processed it
   // mesh pools
   Vertex* vertices;
   int32_t* vertex_heap;
@@ -260,24 +262,64 @@ struct View {
   D34 cTw64;             // FP64 world->camera (values)
 };
 
+// Occupancy filter: one bit per hashed block key (2^kOccBits bits, 2 MiB,
+// L2-resident).  Set on every insertion, never cleared within a volume's
+// life, so a clear bit proves a block is absent; a set bit falls through to
+// the hash table.  Most lookups of the raycaster and of the stencil kernels
+// are for empty space and end here with one cached load.
+constexpr int kOccBits = 24;
+MH_HD uint32_t occ_bit(uint64_t key) {
+  return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - kOccBits));
+}
+__device__ __forceinline__ bool occ_test(const uint32_t* occ, uint64_t key) {
+  const uint32_t b = occ_bit(key);
+  return (occ[b >> 5] >> (b & 31u)) & 1u;
+}
+
 // GetEntry, hash_table.h:57-93: bucket scan, then the list that starts at
-// the bucket's last slot.  Returns the pool index or kFreePtr.
-__device__ __forceinline__ int32_t hash_find(const View& V, i3 p) {
-  const uint64_t key = pack_key(p);
+// the bucket's last slot.  Returns the pool index or kFreePtr.  The bucket's
+// slots are loaded together (independent 16-byte loads) rather than one by
+// one, so a probe costs one memory round trip.
+__device__ __forceinline__ int32_t hash_find_key(const View& V, i3 p, uint64_t key) {
   const uint32_t first = hash_bucket(p, V.bucket_count) * V.bucket_size;
-  for (uint32_t j = 0; j < V.bucket_size; ++j) {
-    const Entry e = V.entries[first + j];
-    if (e.key == key) return e.ptr;
-  }
   const uint32_t last = first + V.bucket_size - 1;
-  uint32_t i = last;
-  for (uint32_t it = 0; it < V.linked_list_size; ++it) {
+  uint32_t last_offset;
+  if (V.bucket_size == 10) {
+    Entry e[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) e[j] = V.entries[first + j];
+    int32_t found = kFreePtr;
+#pragma unroll
+    for (int j = 0; j < 10; ++j)
+      if (e[j].key == key) found = e[j].ptr;
+    if (e[0].key == key || e[1].key == key || e[2].key == key || e[3].key == key ||
+        e[4].key == key || e[5].key == key || e[6].key == key || e[7].key == key ||
+        e[8].key == key || e[9].key == key)
+      return found;
+    last_offset = e[9].offset;
+  } else {
+    for (uint32_t j = 0; j < V.bucket_size; ++j) {
+      const Entry e = V.entries[first + j];
+      if (e.key == key) return e.ptr;
+    }
+    last_offset = V.entries[last].offset;
+  }
+  // list: its head is the last slot (already compared above)
+  if (V.linked_list_size == 0 || last_offset == 0) return kFreePtr;
+  uint32_t i = (last + last_offset) % V.entry_count;
+  for (uint32_t it = 1; it < V.linked_list_size; ++it) {
     const Entry e = V.entries[i];
     if (e.key == key) return e.ptr;
     if (e.offset == 0) break;
     i = (last + e.offset) % V.entry_count;
   }
   return kFreePtr;
+}
+
+__device__ __forceinline__ int32_t hash_find(const View& V, i3 p) {
+  const uint64_t key = pack_key(p);
+  if (!occ_test(V.occ, key)) return kFreePtr;
+  return hash_find_key(V, p, key);
 }
 
 __device__ __forceinline__ void flag(const View& V, uint32_t bit) {
@@ -295,6 +337,48 @@ __device__ __forceinline__ int32_t alloc_block(const View& V) {
   const int32_t p = V.block_heap[addr];
   atomicMax(&V.ctr->block_hwm, (uint32_t)p + 1u);
   return p;
+}
+
+// Wave-aggregated LIFO heap operations on a free-index stack whose top is
+// *top (the reference's Alloc/Free, hash_table.h:279-292, mesh.h:53-78, one
+// atomic per thread on one counter).  Here one lane per wave does a single
+// atomic for all lanes that need an index; lanes get consecutive stack
+// slots by rank.  Every lane of the wave must make the call (need = false
+// for lanes that want nothing).  Pops and pushes never share a kernel.
+__device__ __forceinline__ int32_t heap_pop_wave(int32_t* top, const int32_t* heap, bool need,
+                                                 uint32_t* flags, uint32_t bit) {
+  const unsigned long long mask = __ballot(need);
+  if (mask == 0ull) return kFreePtr;
+  const uint32_t lane = threadIdx.x & 63u;
+  const int leader = __ffsll((long long)mask) - 1;
+  const int cnt = __popcll(mask);
+  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+  int32_t old = 0;
+  if ((int)lane == leader) {
+    old = atomicSub(top, cnt);
+    const int32_t avail = old + 1 > 0 ? old + 1 : 0;
+    if (avail < cnt) {
+      atomicAdd(top, cnt - avail);
+      atomicOr(flags, bit);
+    }
+  }
+  old = __shfl(old, leader);
+  if (!need) return kFreePtr;
+  const int32_t idx = old - rank;
+  return idx >= 0 ? heap[idx] : kFreePtr;
+}
+
+__device__ __forceinline__ void heap_push_wave(int32_t* top, int32_t* heap, bool need, int32_t p) {
+  const unsigned long long mask = __ballot(need);
+  if (mask == 0ull) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const int leader = __ffsll((long long)mask) - 1;
+  const int cnt = __popcll(mask);
+  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+  int32_t old = 0;
+  if ((int)lane == leader) old = atomicAdd(top, cnt);
+  old = __shfl(old, leader);
+  if (need) heap[old + 1 + rank] = p;
 }
 
 // Local voxel index <-> offset, geometry_helper.h:111-129.

@@ -24,6 +24,7 @@ struct McConst {
   float weight_min;
   double snap;
   int flat_normals;
+  uint32_t pass;  // 1, 2, ...: per-call stamp for single-writer vertex payloads
 };
 void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int grid);
 
@@ -33,6 +34,7 @@ struct RayConst {
   float min_depth, max_depth, step;
   float sample_sdf_threshold, sdf_threshold;
   int enable_gradients;
+  float inv_vs;  // 1 / voxel_size (fast empty-space test only)
   M4 cTw, wTc;
 };
 void launch_raycast(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,

@@ -46,33 +46,69 @@ __device__ __forceinline__ int vox1(float w, float vs) {
   return f2i(p + (float)sgn(p) * 0.5f);
 }
 
+// Lowest-corner voxel of a sample along one axis from a multiply by 1/vs
+// instead of the IEEE division; accepted only when the quotient is more than
+// 1e-3 voxel away from every rounding tie (and |q| < 2048, where the
+// multiply is within 2 ulp of the quotient), so it equals the exact result.
+__device__ __forceinline__ bool fast_corner(float pc, float inv_vs, int* c) {
+  const float q = pc * inv_vs;
+  const float aq = fabsf(q);
+  if (!(aq < 2048.0f) || aq < 1e-3f) return false;
+  const float f = aq - floorf(aq);
+  if (fabsf(f - 0.5f) < 1e-3f) return false;
+  *c = f2i(q + (float)sgn(q) * 0.5f);
+  return true;
+}
+
 // GetSpatialValue restricted to the SDF (the only field the march uses).
-__device__ bool ray_sdf(const View& V, BlockCache& cache, f3 pos, float* out) {
+__device__ __forceinline__ bool ray_sdf(const View& V, BlockCache& cache, f3 pos, float inv_vs,
+                                        float* out) {
   const float vs = V.geo.voxel_size;
   const float hv = 0.5f * vs;
   const float pcx = pos.x - hv, pcy = pos.y - hv, pcz = pos.z - hv;
+  {
+    // empty-space fast path: the lowest corner's block is missing
+    int c0x, c0y, c0z;
+    if (fast_corner(pcx, inv_vs, &c0x) && fast_corner(pcy, inv_vs, &c0y) &&
+        fast_corner(pcz, inv_vs, &c0z) &&
+        cached_find(V, cache, i3{voxel_to_block1(c0x), voxel_to_block1(c0y), voxel_to_block1(c0z)}) < 0)
+      return false;
+  }
   const float qx = pos.x / vs, qy = pos.y / vs, qz = pos.z / vs;
   const float rx = qx - floorf(qx), ry = qy - floorf(qy), rz = qz - floorf(qz);
   const int cx[2] = {vox1(pcx + 0.0f, vs), vox1(pcx + vs, vs)};
   const int cy[2] = {vox1(pcy + 0.0f, vs), vox1(pcy + vs, vs)};
   const int cz[2] = {vox1(pcz + 0.0f, vs), vox1(pcz + vs, vs)};
-  float sdf = 0.0f;
+  const int bx[2] = {voxel_to_block1(cx[0]), voxel_to_block1(cx[1])};
+  const int by[2] = {voxel_to_block1(cy[0]), voxel_to_block1(cy[1])};
+  const int bz[2] = {voxel_to_block1(cz[0]), voxel_to_block1(cz[1])};
+  // The sample is invalid if any corner's block is missing; in empty space
+  // the lowest corner's block already says so (one cached probe).
+  const int32_t p000 = cached_find(V, cache, i3{bx[0], by[0], bz[0]});
+  if (p000 < 0) return false;
+  int32_t pb[8];
+#pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int mx = (i >> 2) & 1, my = (i >> 1) & 1, mz = i & 1;
-    const i3 v = i3{cx[mx], cy[my], cz[mz]};
-    const i3 b = voxel_to_block(v);
-    const int32_t p = cached_find(V, cache, b);
-    if (p < 0) return false;
-    const uint32_t li = (uint32_t)((v.x - b.x * 8) + 8 * (v.y - b.y * 8) + 64 * (v.z - b.z * 8));
+    const bool same = bx[mx] == bx[0] && by[my] == by[0] && bz[mz] == bz[0];
+    pb[i] = same ? p000 : cached_find(V, cache, i3{bx[mx], by[my], bz[mz]});
+    if (pb[i] < 0) return false;
+  }
+  float sdf = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int mx = (i >> 2) & 1, my = (i >> 1) & 1, mz = i & 1;
+    const uint32_t li = (uint32_t)((cx[mx] - bx[mx] * 8) + 8 * (cy[my] - by[my] * 8) +
+                                   64 * (cz[mz] - bz[mz] * 8));
     const float w = (mx ? rx : 1.0f - rx) * (my ? ry : 1.0f - ry) * (mz ? rz : 1.0f - rz);
-    sdf += w * V.sdf[(size_t)p * kBlockVoxels + li];
+    sdf += w * V.sdf[(size_t)pb[i] * kBlockVoxels + li];
   }
   *out = sdf;
   return true;
 }
 
 // March one pixel; returns the number of valid samples taken.
-__device__ uint32_t cast_pixel(const View& V, const RayConst& R, uint32_t x, uint32_t y,
+__device__ __forceinline__ uint32_t cast_pixel(const View& V, const RayConst& R, uint32_t x, uint32_t y,
                                float* res_d, f3* res_v, f3* res_n) {
   uint32_t nsamp = 0;
   float out_d = -INFINITY;
@@ -91,7 +127,7 @@ __device__ uint32_t cast_pixel(const View& V, const RayConst& R, uint32_t x, uin
   for (float t = t_min; t < t_max; t += R.step) {
     const f3 p = mk3(o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z);
     float s;
-    if (!ray_sdf(V, cache, p, &s)) continue;
+    if (!ray_sdf(V, cache, p, R.inv_vs, &s)) continue;
     ++nsamp;
     if (prev_ok && prev_sdf > 0.0f && s < 0.0f) {
       // BisectionIntersection: three regula-falsi steps.
@@ -101,7 +137,8 @@ __device__ uint32_t cast_pixel(const View& V, const RayConst& R, uint32_t x, uin
       for (int k = 0; k < 3; ++k) {
         m = l + (l_sdf / (l_sdf - r_sdf)) * (r - l);
         float m_sdf;
-        if (!ray_sdf(V, cache, mk3(o.x + m * dir.x, o.y + m * dir.y, o.z + m * dir.z), &m_sdf)) {
+        if (!ray_sdf(V, cache, mk3(o.x + m * dir.x, o.y + m * dir.y, o.z + m * dir.z), R.inv_vs,
+                     &m_sdf)) {
           found = false;
           break;
         }
@@ -122,11 +159,12 @@ __device__ uint32_t cast_pixel(const View& V, const RayConst& R, uint32_t x, uin
           const float h = 0.5f * V.geo.voxel_size;
           bool valid = true;
           float sp[3] = {0, 0, 0}, sn[3] = {0, 0, 0};
+#pragma unroll
           for (int k = 0; k < 3 && valid; ++k) {
             const f3 dp = mk3(k == 0 ? h : 0.0f, k == 1 ? h : 0.0f, k == 2 ? h : 0.0f);
-            valid = ray_sdf(V, cache, iso - dp, &sn[k]);
+            valid = ray_sdf(V, cache, iso - dp, R.inv_vs, &sn[k]);
             if (!valid) break;
-            valid = ray_sdf(V, cache, iso + dp, &sp[k]);
+            valid = ray_sdf(V, cache, iso + dp, R.inv_vs, &sp[k]);
           }
           const float vs = V.geo.voxel_size;
           const f3 grad = mk3((sp[0] - sn[0]) / vs, (sp[1] - sn[1]) / vs, (sp[2] - sn[2]) / vs);
