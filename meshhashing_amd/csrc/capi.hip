@@ -8,8 +8,11 @@
 #include <vector>
 
 #include "mh_kernels.h"
+#include "mh_mc_tables.h"
 
 using namespace mh;
+
+static constexpr McTables kHostMc = build_mc_tables();
 
 namespace {
 
@@ -237,14 +240,11 @@ int mh_volume_create(const mh_hash_params* hash, const mh_volume_params* vol,
     A(&V.vb, nvox);
   }
   A(&V.vptr, nvox * 3);
-  A(&V.tptr, nvox * 5);
   A(&V.cube, nvox);
   A(&V.surfel, cap * 2);
   A(&V.block_mark, cap);
   A(&V.vertices, mesh->max_vertex_count);
   A(&V.vertex_heap, mesh->max_vertex_count);
-  A(&V.triangles, (size_t)mesh->max_triangle_count * 3);
-  A(&V.triangle_heap, mesh->max_triangle_count);
   A(&V.cand, cap);
   A(&V.depth, npix);
   A(&V.inlier, npix);
@@ -317,7 +317,8 @@ int mh_get_status(mh_volume* v, mh_status* out) {
   out->block_count = (uint32_t)((int64_t)v->hp.value_capacity - 1 - c.block_top);
   out->candidate_count = c.cand_count;
   out->vertex_count = (uint32_t)((int64_t)v->mp.max_vertex_count - 2 - c.vertex_top);
-  out->triangle_count = (uint32_t)((int64_t)v->mp.max_triangle_count - 1 - c.triangle_top);
+  out->triangle_count = (uint32_t)c.tri_live;
+  if (c.tri_live > v->mp.max_triangle_count) out->flags |= MH_STATUS_TRIANGLE_HEAP_EXHAUSTED;
   out->voxel_updates = c.voxel_updates;
   out->blocks_integrated = c.blocks_integrated;
   out->mc_vertex_writes = c.mc_vertex_writes;
@@ -651,22 +652,20 @@ int mh_dump_mesh(mh_volume* v, uint32_t tri_capacity, int32_t* tri_host, int32_t
   if (rc) return rc;
   const uint32_t hwm = read_counters(v).block_hwm;
   const size_t nvox = (size_t)hwm * kBlockVoxels;
-  std::vector<int32_t> vptr(nvox * 3), tptr(nvox * 5);
+  std::vector<int32_t> vptr(nvox * 3);
+  std::vector<uint16_t> cubes(nvox);
   if (nvox) {
+    HIP_TRY(hipMemcpy(cubes.data(), v->V.cube, nvox * 2, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(vptr.data(), v->V.vptr, vptr.size() * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(tptr.data(), v->V.tptr, tptr.size() * 4, hipMemcpyDeviceToHost));
   }
   std::vector<Vertex> verts(v->mp.max_vertex_count);
   HIP_TRY(hipMemcpy(verts.data(), v->V.vertices, verts.size() * sizeof(Vertex),
                     hipMemcpyDeviceToHost));
-  std::vector<int32_t> tris((size_t)v->mp.max_triangle_count * 3);
-  HIP_TRY(hipMemcpy(tris.data(), v->V.triangles, tris.size() * 4, hipMemcpyDeviceToHost));
   // vertex pointer -> canonical key (owner voxel, axis)
   struct VKey {
     int32_t k[4];
     int32_t vp;
   };
-  std::vector<int32_t> key_of(verts.size() * 4, INT32_MIN);
   std::vector<VKey> vkeys;
   for (const BlockRec& b : blocks) {
     for (int i = 0; i < kBlockVoxels; ++i) {
@@ -674,9 +673,7 @@ int mh_dump_mesh(mh_volume* v, uint32_t tri_capacity, int32_t* tri_host, int32_t
       for (int ax = 0; ax < 3; ++ax) {
         const int32_t vp = vptr[((size_t)b.ptr * kBlockVoxels + i) * 3 + ax];
         if (vp < 0) continue;
-        VKey k{{b.pos.x * 8 + o.x, b.pos.y * 8 + o.y, b.pos.z * 8 + o.z, ax}, vp};
-        std::memcpy(&key_of[(size_t)vp * 4], k.k, sizeof(k.k));
-        vkeys.push_back(k);
+        vkeys.push_back(VKey{{b.pos.x * 8 + o.x, b.pos.y * 8 + o.y, b.pos.z * 8 + o.z, ax}, vp});
       }
     }
   }
@@ -696,7 +693,26 @@ int mh_dump_mesh(mh_volume* v, uint32_t tri_capacity, int32_t* tri_host, int32_t
         d[6] = vt.cr; d[7] = vt.cg; d[8] = vt.cb;
         d[9] = vt.radius;
       }
-      if (vtx_ref_host) vtx_ref_host[i] = vt.ref_count;
+      if (vtx_ref_host) {
+        // reference count from the four cube configurations around the edge
+        const int32_t* k = vkeys[i].k;
+        const int ax = k[3], b1 = ax == 0 ? 1 : 0, b2 = ax == 2 ? 1 : 2;
+        int refs = 0;
+        for (int q = 0; q < 4; ++q) {
+          int d[3] = {0, 0, 0};
+          d[b1] = q & 1;
+          d[b2] = q >> 1;
+          const i3 c{k[0] - d[0], k[1] - d[1], k[2] - d[2]};
+          const i3 cb = voxel_to_block(c);
+          auto it = std::lower_bound(blocks.begin(), blocks.end(), cb,
+                                     [](const BlockRec& r, const i3& p) { return pos_less(r.pos, p); });
+          if (it == blocks.end() || !(it->pos.x == cb.x && it->pos.y == cb.y && it->pos.z == cb.z))
+            continue;
+          const uint32_t li = (uint32_t)((c.x - cb.x * 8) + 8 * (c.y - cb.y * 8) + 64 * (c.z - cb.z * 8));
+          refs += edge_refs(kHostMc, cubes[(size_t)it->ptr * kBlockVoxels + li] & 255, d[0], d[1], d[2], ax);
+        }
+        vtx_ref_host[i] = refs;
+      }
     }
   }
   // triangles in (cube voxel, slot) order: blocks are sorted by position and
@@ -705,19 +721,36 @@ int mh_dump_mesh(mh_volume* v, uint32_t tri_capacity, int32_t* tri_host, int32_t
     int32_t cube[4];
     int32_t v[12];
   };
+  // The live triangles of a cube are its configuration's table row over the
+  // vertices of its edges; a vertex is named by its owner voxel and axis.
   std::vector<TRec> trecs;
+  auto slot_of = [&](const i3& vox, int ax) -> int32_t {
+    const i3 vb = voxel_to_block(vox);
+    auto it = std::lower_bound(blocks.begin(), blocks.end(), vb,
+                               [](const BlockRec& r, const i3& p) { return pos_less(r.pos, p); });
+    if (it == blocks.end() || !(it->pos.x == vb.x && it->pos.y == vb.y && it->pos.z == vb.z))
+      return kFreePtr;
+    const uint32_t li = (uint32_t)((vox.x - vb.x * 8) + 8 * (vox.y - vb.y * 8) + 64 * (vox.z - vb.z * 8));
+    return vptr[((size_t)it->ptr * kBlockVoxels + li) * 3 + ax];
+  };
   for (const BlockRec& b : blocks) {
     for (int i = 0; i < kBlockVoxels; ++i) {
+      const int cube = cubes[(size_t)b.ptr * kBlockVoxels + i] & 255;
+      const int nt = kHostMc.ntri[cube];
+      if (cube == 0 || cube == 255 || nt == 0) continue;
       const i3 o = devectorize((uint32_t)i);
-      for (int t = 0; t < 5; ++t) {
-        const int32_t tp = tptr[((size_t)b.ptr * kBlockVoxels + i) * 5 + t];
-        if (tp < 0) continue;
-        TRec r{{b.pos.x * 8 + o.x, b.pos.y * 8 + o.y, b.pos.z * 8 + o.z, t}, {}};
+      const i3 c{b.pos.x * 8 + o.x, b.pos.y * 8 + o.y, b.pos.z * 8 + o.z};
+      for (int t = 0; t < nt; ++t) {
+        TRec r{{c.x, c.y, c.z, t}, {}};
         for (int k = 0; k < 3; ++k) {
-          const int32_t vp = tris[(size_t)tp * 3 + k];
-          for (int j = 0; j < 4; ++j)
-            r.v[4 * k + j] = (vp >= 0 && (size_t)vp < verts.size()) ? key_of[(size_t)vp * 4 + j]
-                                                                    : INT32_MIN;
+          const int e = kHostMc.tri[cube][3 * t + k];
+          const int ax = edge_axis(e);
+          const i3 ov{c.x + edge_owner_off(e, 0), c.y + edge_owner_off(e, 1), c.z + edge_owner_off(e, 2)};
+          const bool live = slot_of(ov, ax) >= 0;
+          r.v[4 * k + 0] = live ? ov.x : INT32_MIN;
+          r.v[4 * k + 1] = live ? ov.y : INT32_MIN;
+          r.v[4 * k + 2] = live ? ov.z : INT32_MIN;
+          r.v[4 * k + 3] = ax;
         }
         trecs.push_back(r);
       }

@@ -37,13 +37,6 @@ __global__ void reset_pool_kernel(View V) {
     Vertex z{};
     V.vertices[i] = z;
   }
-  const uint32_t nt = V.max_triangles;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x) {
-    V.triangle_heap[i] = (int32_t)(nt - 1u - i);
-    V.triangles[3 * i + 0] = -1;
-    V.triangles[3 * i + 1] = -1;
-    V.triangles[3 * i + 2] = -1;
-  }
 }
 
 // Voxel / mesh-unit state of a block, as Block::Clear (core/block.h:27-39).
@@ -57,7 +50,6 @@ __global__ void reset_voxels_kernel(View V, size_t nvox) {
     V.vptr[3 * i + 0] = kFreePtr;
     V.vptr[3 * i + 1] = kFreePtr;
     V.vptr[3 * i + 2] = kFreePtr;
-    for (int t = 0; t < 5; ++t) V.tptr[5 * i + t] = kFreePtr;
     V.cube[i] = 0;
   }
 }
@@ -72,7 +64,6 @@ __global__ void reset_counters_kernel(View V) {
     c.block_top = (int32_t)V.capacity - 1;
     c.block_hwm = 0;
     c.vertex_top = (int32_t)V.max_vertices - 2;
-    c.triangle_top = (int32_t)V.max_triangles - 1;
     *V.ctr = c;
   }
 }

@@ -99,17 +99,21 @@ __device__ bool shell_value(const Shell<PSDF>& S, const Geo& g, i3 base, f3 pos,
   const int cx[2] = {vox1(pcx + 0.0f, vs) - base.x, vox1(pcx + vs, vs) - base.x};
   const int cy[2] = {vox1(pcy + 0.0f, vs) - base.y, vox1(pcy + vs, vs) - base.y};
   const int cz[2] = {vox1(pcz + 0.0f, vs) - base.z, vox1(pcz + vs, vs) - base.z};
+  // Straight-line evaluation of all 8 corners (no early exit: a missing
+  // corner makes the result invalid, and the sums are then discarded).
+  const bool in = cx[0] >= -1 && cx[1] <= 9 && cy[0] >= -1 && cy[1] <= 9 && cz[0] >= -1 && cz[1] <= 9 &&
+                  cx[1] >= -1 && cx[0] <= 9 && cy[1] >= -1 && cy[0] <= 9 && cz[1] >= -1 && cz[0] <= 9;
+  if (!in) {
+    *oob = true;
+    return false;
+  }
+  bool ok = true;
   float sdf = 0.0f, a = 0.0f, b = 0.0f, rad = 0.0f;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int mx = (i >> 2) & 1, my = (i >> 1) & 1, mz = i & 1;
-    const int lx = cx[mx], ly = cy[my], lz = cz[mz];
-    if (lx < -1 || lx > 9 || ly < -1 || ly > 9 || lz < -1 || lz > 9) {
-      *oob = true;
-      return false;
-    }
-    const int s = sidx(lx, ly, lz);
-    if (!S.ok[s]) return false;
+    const int s = sidx(cx[mx], cy[my], cz[mz]);
+    ok = ok && S.ok[s];
     const float w = (mx ? rx : 1.0f - rx) * (my ? ry : 1.0f - ry) * (mz ? rz : 1.0f - rz);
     sdf += w * S.sdf[s];
     if (FULL) {
@@ -124,7 +128,7 @@ __device__ bool shell_value(const Shell<PSDF>& S, const Geo& g, i3 base, f3 pos,
   out->a = a;
   out->b = b;
   out->radius = rad;
-  return true;
+  return ok;
 }
 
 // ValToRGB (color_util.h:13-37); the 0.25/0.5/0.75 literals are doubles
@@ -173,17 +177,18 @@ __device__ void vertex_payload(const Shell<PSDF>& S, const Geo& g, i3 base, f3 p
   }
   // GetSpatialSDFGradient (spatial_query.h:68-97): central differences at
   // +-vs/2, with the reference's short-circuit on the first invalid sample.
+  // (all six samples are taken; any invalid one zeroes the normal, which is
+  // what the reference's short-circuit produces)
   const float h = 0.5f * g.voxel_size;
   bool valid = true;
   float sp[3] = {0, 0, 0}, sn[3] = {0, 0, 0};
 #pragma unroll
-  for (int k = 0; k < 3 && valid; ++k) {
+  for (int k = 0; k < 3; ++k) {
     const f3 dp = mk3(k == 0 ? h : 0.0f, k == 1 ? h : 0.0f, k == 2 ? h : 0.0f);
     SVal q{};
-    valid = shell_value<PSDF, false>(S, g, base, p - dp, &q, oob);
+    valid = shell_value<PSDF, false>(S, g, base, p - dp, &q, oob) && valid;
     sn[k] = q.sdf;
-    if (!valid) break;
-    valid = shell_value<PSDF, false>(S, g, base, p + dp, &q, oob);
+    valid = shell_value<PSDF, false>(S, g, base, p + dp, &q, oob) && valid;
     sp[k] = q.sdf;
   }
   const float vs = g.voxel_size;
@@ -280,7 +285,8 @@ __global__ __launch_bounds__(512) void mc_surfel_kernel(View V, McConst mc) {
   __shared__ uint32_t qn;
   const uint32_t n = V.ctr->cand_count;
   const Geo g = V.geo;
-  uint32_t nwrites = 0;
+  uint32_t nwrites = 0, tri_assigned = 0;
+  int tri_delta = 0;
   bool oob = false;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&V.ctr->mc_blocks, (unsigned long long)n);
   for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
@@ -306,6 +312,9 @@ __global__ __launch_bounds__(512) void mc_surfel_kernel(View V, McConst mc) {
     int cube = 0;
     if (!cube_index(S, o.x, o.y, o.z, mc, &cube)) cube = 0;
     V.cube[idx] = (uint16_t)((uint32_t)cube | (prev << 8));
+    // live triangles of a cube = its configuration's table row
+    tri_delta += kMc.ntri[cube] - kMc.ntri[prev];
+    if ((uint32_t)cube != prev) tri_assigned += (uint32_t)kMc.ntri[cube];
     const unsigned em = (cube != 0 && cube != 255) ? kMc.edges[cube] : 0u;
     // uniform loop: every lane takes part in the wave-aggregated pops
     for (int e = 0; e < 12; ++e) {
@@ -369,6 +378,10 @@ __global__ __launch_bounds__(512) void mc_surfel_kernel(View V, McConst mc) {
   }
   if (oob) flag(V, MH_STATUS_STENCIL_RANGE);
   wave_add_u64(&V.ctr->mc_vertex_writes, nwrites);
+  wave_add_u64(&V.ctr->mc_triangle_writes, tri_assigned);
+  for (int off = 32; off > 0; off >>= 1) tri_delta += __shfl_xor(tri_delta, off);
+  if ((threadIdx.x & 63u) == 0 && tri_delta)
+    atomicAdd(&V.ctr->tri_live, (unsigned long long)(long long)tri_delta);
 }
 
 // Meshed-in-this-pass marks for the writer rule.
@@ -379,128 +392,67 @@ __global__ void mc_mark_kernel(View V, uint32_t pass) {
 }
 
 // ---------------------------------------------------------------- pass 2
-// TriangleExtractionKernel (marching_cubes.cu:180-275).  A triangle slot
-// whose vertex triple is unchanged is left alone (release + re-assign of
-// the same triple leaves every ref count where it was).
-__global__ __launch_bounds__(512) void mc_triangle_kernel(View V) {
-  __shared__ int32_t nb[8];
+// The reference's TriangleExtractionKernel (marching_cubes.cu:180-275) and
+// RecycleTrianglesKernel (:278-301) maintain a triangle pool: after a pass
+// every processed cube holds exactly the triangles of its configuration's
+// table row, over the vertices of its active edges, and keeps them until it
+// is processed again.  That state is already held by the cube configuration
+// and the per-voxel vertex slots, so the mesh is kept implicitly: no pool,
+// no per-triangle writes, no reference-count atomics.  The triangle list
+// is materialised only when the mesh is read out (dumps, compaction).  A
+// vertex's reference count is likewise the sum over the four cubes around
+// its edge of how many of their table triangles use that edge (edge_refs).
+//
+// RecycleVerticesKernel (marching_cubes.cu:303-321) then frees the block's
+// vertices with no references; it also produces the block's inner/boundary
+// surfel counts (TriangleExtractionKernel counts the allocated vertex slots
+// before recycling, :187-205).  Cube configurations of the block and of its
+// -x/-y/-z neighbours (voxels -1..7) are staged in LDS.
+constexpr int kCfg = 9;  // local cube coordinates -1..7
+__global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(View V) {
+  __shared__ int32_t nb[8];  // blocks at offsets (dx, dy, dz) in {-1, 0}^3
+  __shared__ uint8_t cfg[kCfg * kCfg * kCfg];
   __shared__ int counts[2];
   const uint32_t n = V.ctr->cand_count;
-  uint32_t nassign = 0;
   for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
     const int32_t ptr = V.cand[ci];
     const i3 b = unpack_key(V.block_keys[ptr]);
-    const int t8 = (int)threadIdx.x;
-    if (t8 < 8)
-      nb[t8] = t8 == 0 ? ptr : hash_find(V, i3{b.x + (t8 & 1), b.y + ((t8 >> 1) & 1), b.z + ((t8 >> 2) & 1)});
-    if (t8 == 0) { counts[0] = 0; counts[1] = 0; }
+    if (threadIdx.x < 8) {
+      const int t = (int)threadIdx.x;
+      nb[t] = t == 0 ? ptr : hash_find(V, i3{b.x - (t & 1), b.y - ((t >> 1) & 1), b.z - ((t >> 2) & 1)});
+    }
+    if (threadIdx.x == 0) { counts[0] = 0; counts[1] = 0; }
     __syncthreads();
-    const uint32_t i = threadIdx.x;
-    const i3 o = devectorize(i);
-    const size_t idx = (size_t)ptr * kBlockVoxels + i;
-    const int nv = (V.vptr[3 * idx] >= 0) + (V.vptr[3 * idx + 1] >= 0) + (V.vptr[3 * idx + 2] >= 0);
-    if (nv) {
-      const bool inner = o.x >= 1 && o.y >= 1 && o.z >= 1 && o.x < 7 && o.y < 7 && o.z < 7;
-      atomicAdd(&counts[inner ? 0 : 1], nv);
-    }
-    const int cube = V.cube[idx] & 255;
-    const int nt = (cube != 0 && cube != 255) ? kMc.ntri[cube] : 0;
-    int32_t vps[12];
-    if (nt) {
-      const unsigned em = kMc.edges[cube];
-      for (int e = 0; e < 12; ++e) {
-        vps[e] = kFreePtr;
-        if (!(em & (1u << e))) continue;
-        const int ax = edge_axis(e);
-        const int ox = o.x + edge_owner_off(e, 0), oy = o.y + edge_owner_off(e, 1),
-                  oz = o.z + edge_owner_off(e, 2);
-        const int dx = ox >> 3, dy = oy >> 3, dz = oz >> 3;
-        const int32_t optr = nb[dx + 2 * dy + 4 * dz];
-        const uint32_t oi = (uint32_t)((ox - 8 * dx) + 8 * (oy - 8 * dy) + 64 * (oz - 8 * dz));
-        vps[e] = optr < 0 ? kFreePtr : V.vptr[((size_t)optr * kBlockVoxels + oi) * 3 + ax];
-      }
-    }
-    for (int t = 0; t < 5; ++t) {  // uniform: wave-aggregated pops below
-      int32_t v0 = kFreePtr, v1 = kFreePtr, v2 = kFreePtr, tp = kFreePtr;
-      bool assign = false;
-      if (t < nt) {
-        v0 = vps[kMc.tri[cube][3 * t]];
-        v1 = vps[kMc.tri[cube][3 * t + 1]];
-        v2 = vps[kMc.tri[cube][3 * t + 2]];
-        if (v0 >= 0 && v1 >= 0 && v2 >= 0) {  // else: vertex pool exhausted in pass 1
-          tp = V.tptr[5 * idx + t];
-          assign = !(tp != kFreePtr && V.triangles[3 * tp + 0] == v0 &&
-                     V.triangles[3 * tp + 1] == v1 && V.triangles[3 * tp + 2] == v2);
-        }
-      }
-      const bool need = assign && tp == kFreePtr;
-      const int32_t fresh = heap_pop_wave(&V.ctr->triangle_top, V.triangle_heap, need,
-                                          &V.ctr->flags, MH_STATUS_TRIANGLE_HEAP_EXHAUSTED);
-      if (!assign) continue;
-      if (need) {
-        if (fresh < 0) continue;
-        tp = fresh;
-      } else {
-        atomicSub(&V.vertices[V.triangles[3 * tp + 0]].ref_count, 1);
-        atomicSub(&V.vertices[V.triangles[3 * tp + 1]].ref_count, 1);
-        atomicSub(&V.vertices[V.triangles[3 * tp + 2]].ref_count, 1);
-      }
-      V.tptr[5 * idx + t] = tp;
-      V.triangles[3 * tp + 0] = v0;
-      V.triangles[3 * tp + 1] = v1;
-      V.triangles[3 * tp + 2] = v2;
-      atomicAdd(&V.vertices[v0].ref_count, 1);
-      atomicAdd(&V.vertices[v1].ref_count, 1);
-      atomicAdd(&V.vertices[v2].ref_count, 1);
-      ++nassign;
+    for (int s = (int)threadIdx.x; s < kCfg * kCfg * kCfg; s += (int)blockDim.x) {
+      const int x = s % kCfg - 1, y = (s / kCfg) % kCfg - 1, z = s / (kCfg * kCfg) - 1;
+      const int nx = x < 0, ny = y < 0, nz = z < 0;
+      const int32_t cp = nb[nx + 2 * ny + 4 * nz];
+      cfg[s] = cp < 0 ? 0 : (uint8_t)(V.cube[(size_t)cp * kBlockVoxels + (x + 8 * nx) + 8 * (y + 8 * ny) +
+                                             64 * (z + 8 * nz)] & 255);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      V.surfel[2 * ptr + 0] = counts[0];
-      V.surfel[2 * ptr + 1] = counts[1];
-    }
-    __syncthreads();
-  }
-  wave_add_u64(&V.ctr->mc_triangle_writes, nassign);
-}
-
-// ---------------------------------------------------------------- recycling
-// RecycleTrianglesKernel (marching_cubes.cu:278-301).
-__global__ __launch_bounds__(256) void mc_recycle_triangles_kernel(View V) {
-  const uint32_t n = V.ctr->cand_count;
-  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
-    const int32_t ptr = V.cand[ci];
+    int inner = 0, boundary = 0;
     for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x) {
       const size_t idx = (size_t)ptr * kBlockVoxels + i;
-      const int keep = kMc.ntri[V.cube[idx] & 255];
-      for (int t = 0; t < 5; ++t) {  // uniform: wave-aggregated pushes
-        const int32_t tp = t >= keep ? V.tptr[5 * idx + t] : kFreePtr;
-        const bool release = tp != kFreePtr;
-        if (release) {
-          atomicSub(&V.vertices[V.triangles[3 * tp + 0]].ref_count, 1);
-          atomicSub(&V.vertices[V.triangles[3 * tp + 1]].ref_count, 1);
-          atomicSub(&V.vertices[V.triangles[3 * tp + 2]].ref_count, 1);
-          V.triangles[3 * tp + 0] = -1;  // Triangle::Clear (triangle.h:14-16)
-          V.triangles[3 * tp + 1] = -1;
-          V.triangles[3 * tp + 2] = -1;
-          V.tptr[5 * idx + t] = kFreePtr;
-        }
-        heap_push_wave(&V.ctr->triangle_top, V.triangle_heap, release, tp);
-      }
-    }
-  }
-}
-
-// RecycleVerticesKernel (marching_cubes.cu:303-321).
-__global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(View V) {
-  const uint32_t n = V.ctr->cand_count;
-  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
-    const int32_t ptr = V.cand[ci];
-    for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x) {
-      const size_t idx = (size_t)ptr * kBlockVoxels + i;
+      const i3 o = devectorize(i);
+      const bool is_inner = o.x >= 1 && o.y >= 1 && o.z >= 1 && o.x < 7 && o.y < 7 && o.z < 7;
       for (int ax = 0; ax < 3; ++ax) {  // uniform: wave-aggregated pushes
         const int32_t vp = V.vptr[3 * idx + ax];
-        const bool release = vp >= 0 && V.vertices[vp].ref_count == 0;
+        int refs = 0;
+        if (vp >= 0) {
+          if (is_inner) ++inner; else ++boundary;
+          const int b1 = ax == 0 ? 1 : 0, b2 = ax == 2 ? 1 : 2;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            int d[3] = {0, 0, 0};
+            d[b1] = k & 1;
+            d[b2] = k >> 1;
+            const int c = cfg[(o.x - d[0] + 1) + kCfg * (o.y - d[1] + 1) + kCfg * kCfg * (o.z - d[2] + 1)];
+            refs += edge_refs(kMc, c, d[0], d[1], d[2], ax);
+          }
+        }
+        // (the count itself is not stored: readers derive it the same way)
+        const bool release = vp >= 0 && refs == 0;
         if (release) {
           Vertex z{};
           V.vertices[vp] = z;
@@ -509,6 +461,14 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(View V) {
         heap_push_wave(&V.ctr->vertex_top, V.vertex_heap, release, vp);
       }
     }
+    if (inner) atomicAdd(&counts[0], inner);
+    if (boundary) atomicAdd(&counts[1], boundary);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      V.surfel[2 * ptr + 0] = counts[0];
+      V.surfel[2 * ptr + 1] = counts[1];
+    }
+    __syncthreads();
   }
 }
 
@@ -518,9 +478,7 @@ void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int g
     mc_surfel_kernel<true><<<grid, 512, 0, s>>>(V, p);
   else
     mc_surfel_kernel<false><<<grid, 512, 0, s>>>(V, p);
-  mc_triangle_kernel<<<grid, 512, 0, s>>>(V);
-  mc_recycle_triangles_kernel<<<grid, 256, 0, s>>>(V);
-  mc_recycle_vertices_kernel<<<grid, 256, 0, s>>>(V);
+  mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(V);
 }
 
 }  // namespace mh

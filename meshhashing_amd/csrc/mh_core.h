@@ -184,7 +184,7 @@ struct Counters {
   uint32_t cand_count;     // worklist length
   uint32_t ovf_count;      // keys queued for list insertion this frame
   int32_t vertex_top;      // vertex heap counter (mesh.h:53-59)
-  int32_t triangle_top;    // triangle heap counter (mesh.h:67-73)
+  int32_t pad_top;
   uint32_t flags;          // MH_STATUS_* bits
   uint32_t dropped;
   uint32_t dda_cap;
@@ -194,9 +194,10 @@ struct Counters {
   unsigned long long voxel_updates;      // voxels whose fused state changed
   unsigned long long blocks_integrated;  // worklist blocks visited by fusion
   unsigned long long mc_vertex_writes;   // vertex records written by MC pass 1
-  unsigned long long mc_triangle_writes; // triangles assigned by MC pass 2
+  unsigned long long mc_triangle_writes; // triangles of cubes whose configuration changed
   unsigned long long mc_blocks;          // worklist blocks meshed
   unsigned long long ray_samples;        // valid trilinear samples of the raycaster
+  unsigned long long tri_live;           // live triangles (sum of table rows)
 };
 
 // Sum v over the wave and add it to *dst once.  Every lane of the wave must
@@ -233,7 +234,6 @@ struct View {
   float* vb;             // Beta b (PSDF only)
   // mesh units (core/voxel.h:27-60), 512 per block
   int32_t* vptr;         // 3 per voxel: x, y, z edge vertex
-  int32_t* tptr;         // 5 per voxel
   uint16_t* cube;        // curr | prev << 8
   int32_t* surfel;       // per block: inner, boundary surfel counts (block.h:17-18)
   uint32_t* block_mark;  // per block: last meshing pass that processed it
@@ -241,9 +241,7 @@ struct View {
   Vertex* vertices;
   int32_t* vertex_heap;
   uint32_t max_vertices;
-  int32_t* triangles;    // 3 per triangle
-  int32_t* triangle_heap;
-  uint32_t max_triangles;
+  uint32_t max_triangles;  // bound on live triangles (MeshParams)
   // worklist
   int32_t* cand;
   // primal-dual state (lazily allocated), 512 per block

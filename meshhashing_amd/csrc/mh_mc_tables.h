@@ -74,9 +74,11 @@ constexpr const char* kTriTableHex[256] = {
 };
 
 struct McTables {
-  int8_t tri[256][16];   // edge indices, -1 terminated
-  uint16_t edges[256];   // 12-bit edge mask per configuration
-  int8_t ntri[256];      // triangle count per configuration
+  int8_t tri[256][16];    // edge indices, -1 terminated
+  uint16_t edges[256];    // 12-bit edge mask per configuration
+  int8_t ntri[256];       // triangle count per configuration
+  uint8_t ecount[256][12];  // triangles of a configuration that use edge e
+  int8_t edge_at[2][2][2][3];  // edge with owner offset (x,y,z) and axis, or -1
 };
 
 constexpr int hex_digit(char c) { return c <= '9' ? c - '0' : c - 'a' + 10; }
@@ -95,8 +97,27 @@ constexpr McTables build_mc_tables() {
     for (int k = n; k < 16; ++k) t.tri[i][k] = -1;
     t.edges[i] = (uint16_t)m;
     t.ntri[i] = (int8_t)(n / 3);
+    for (int e = 0; e < 12; ++e) t.ecount[i][e] = 0;
+    for (int k = 0; k < n; ++k) t.ecount[i][hex_digit(s[k])] += 1;
   }
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < 2; ++c)
+        for (int ax = 0; ax < 3; ++ax) t.edge_at[a][b][c][ax] = -1;
+  for (int e = 0; e < 12; ++e)
+    t.edge_at[edge_owner_off(e, 0)][edge_owner_off(e, 1)][edge_owner_off(e, 2)][edge_axis(e)] =
+        (int8_t)e;
   return t;
+}
+
+// References to vertex (owner voxel, axis) held by the cube whose owner
+// offset for that edge is (dx, dy, dz) and whose current configuration is
+// cube: after a meshing pass a cube's live triangles are exactly its
+// configuration's table row (marching_cubes.cu:219-301), so a vertex's
+// reference count is the sum of this over the four cubes around its edge.
+__host__ __device__ constexpr int edge_refs(const McTables& t, int cube, int dx, int dy, int dz,
+                                            int axis) {
+  return (cube == 0 || cube == 255) ? 0 : t.ecount[cube][t.edge_at[dx][dy][dz][axis]];
 }
 
 }  // namespace mh
