@@ -217,8 +217,9 @@ int mh_dump_hash(mh_volume* v, uint32_t capacity, int32_t* rec_host,
 /* Current worklist as block positions (3 int32 each), sorted.              */
 int mh_dump_candidates(mh_volume* v, uint32_t capacity, int32_t* pos_host,
                        uint32_t* count_host);
-/* Mesh in canonical form.  Triangles: 9 int32 each -- the owner voxel
- * (x, y, z) and edge axis of each of its three vertices, in table order --
+/* Mesh in canonical form.  Triangles: 12 int32 each -- the owner voxel
+ * (x, y, z) and edge axis of each of its three vertices, in table order;
+ * a vertex whose slot is not allocated has x = y = z = INT32_MIN --
  * sorted by (cube voxel, slot).  tri_cube_host (optional): 4 int32 per
  * triangle: cube voxel x, y, z and triangle slot.  Vertices: one record per
  * live vertex, key (4 int32: owner voxel xyz, axis) + 10 floats (pos xyz,

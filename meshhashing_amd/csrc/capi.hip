@@ -763,12 +763,7 @@ int mh_dump_mesh(mh_volume* v, uint32_t tri_capacity, int32_t* tri_host, int32_t
   if (tri_host || tri_cube_host) {
     if (tri_capacity < trecs.size()) return fail(MH_ERR_CAPACITY, "triangle buffer too small");
     for (size_t i = 0; i < trecs.size(); ++i) {
-      if (tri_host)
-        for (int k = 0; k < 3; ++k) {
-          tri_host[9 * i + 3 * k + 0] = trecs[i].v[4 * k + 0];
-          tri_host[9 * i + 3 * k + 1] = trecs[i].v[4 * k + 1];
-          tri_host[9 * i + 3 * k + 2] = trecs[i].v[4 * k + 2];
-        }
+      if (tri_host) std::memcpy(tri_host + 12 * i, trecs[i].v, sizeof(trecs[i].v));
       if (tri_cube_host) std::memcpy(tri_cube_host + 4 * i, trecs[i].cube, sizeof(trecs[i].cube));
     }
   }

@@ -205,7 +205,7 @@ class Volume:
         nt, nv = C.c_uint32(0), C.c_uint32(0)
         A.check(self.lib.mh_dump_mesh(self.h, 0, None, None, 0, None, None, None, C.byref(nt),
                                       C.byref(nv)), "count")
-        tri = np.zeros((nt.value, 9), np.int32)
+        tri = np.zeros((nt.value, 12), np.int32)
         tcube = np.zeros((nt.value, 4), np.int32)
         vkey = np.zeros((nv.value, 4), np.int32)
         vval = np.zeros((nv.value, 10), np.float32)

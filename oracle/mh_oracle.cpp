@@ -1150,7 +1150,7 @@ int orc_dump_mesh(const orc_volume* v, int32_t* tri, int32_t* tri_cube, int32_t*
   for (const auto& kv : v->tris) {
     if (tri)
       for (int k = 0; k < 3; ++k)
-        for (int j = 0; j < 3; ++j) tri[9 * n + 3 * k + j] = kv.second[k].k[j];
+        for (int j = 0; j < 4; ++j) tri[12 * n + 4 * k + j] = kv.second[k].k[j];
     if (tri_cube) std::memcpy(tri_cube + 4 * n, kv.first.k, sizeof(kv.first.k));
     ++n;
   }

@@ -220,7 +220,7 @@ class Oracle:
     def dump_mesh(self) -> dict:
         nt = self.L.orc_tri_count(self.h)
         nv = self.L.orc_vtx_count(self.h)
-        tri = np.zeros((nt, 9), np.int32)
+        tri = np.zeros((nt, 12), np.int32)
         tcube = np.zeros((nt, 4), np.int32)
         vkey = np.zeros((nv, 4), np.int32)
         vval = np.zeros((nv, 10), np.float64)

@@ -86,6 +86,89 @@ def test_mc_tables_brute_force(oracle_mod):
         assert L.orc_mc_ntri(c) <= 5
 
 
+def field_volume(oracle_mod, cfg, fn, lo, hi):
+    """Oracle volume whose blocks cover voxels [lo, hi) on every axis, with
+    sdf = fn(world position of each voxel) in FP32 and weight 50."""
+    b0, b1 = lo // 8, -(-hi // 8)
+    r = np.arange(b0, b1)
+    pos = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3).astype(np.int32)
+    i = np.arange(512)
+    off = np.stack([i % 8, (i // 8) % 8, i // 64], -1)
+    vox = pos[:, None, :] * 8 + off[None]
+    vs = float(np.float32(cfg.voxel_size))
+    sdf = fn(vox * vs).astype(np.float32)
+    o = oracle_mod.Oracle(cfg)
+    o.load_blocks(pos, sdf, np.full(sdf.shape, 50.0, np.float32))
+    o.collect_all()
+    o.marching_cubes()
+    return o, vs
+
+
+def test_mc_plane_vertices_closed_form(oracle_mod):
+    """A linear SDF makes the edge interpolation exact: every vertex lies on
+    the plane, except where an endpoint's |sdf| < 0.008 snaps the vertex to
+    that endpoint (marching_cubes.cu:20-21), checked in the canonical order
+    (owner voxel first)."""
+    cfg = plane_cfg()
+    n = np.array([0.3, -0.5, 0.81])
+    n /= np.linalg.norm(n)
+    c0 = 0.2137
+    o, vs = field_volume(oracle_mod, cfg, lambda p: p @ n - c0, 0, 32)
+    m = o.dump_mesh()
+    assert len(m["vkey"]) > 300
+    snapped = 0
+    for key, val in zip(m["vkey"], m["vval"]):
+        a = key[:3].astype(np.float64)
+        b = a.copy()
+        b[key[3]] += 1
+        pa, pb = a * vs, b * vs
+        sa = float(np.float32(pa @ n - c0))
+        sb = float(np.float32(pb @ n - c0))
+        assert (sa < 0) != (sb < 0)
+        if abs(sa) < 0.008:
+            assert np.allclose(val[:3], pa, atol=1e-6)
+            snapped += 1
+        elif abs(sb) < 0.008:
+            assert np.allclose(val[:3], pb, atol=1e-6)
+            snapped += 1
+        else:
+            assert abs(val[:3] @ n - c0) < 2e-6
+        # the normal is the normalised SDF gradient (its +-vs/2 stencil needs
+        # every trilinear corner, so skip vertices next to the region's faces)
+        if key[:3].min() >= 2 and key[:3].max() <= 29:
+            assert np.allclose(val[3:6], n, atol=1e-5)
+    assert 0 < snapped < len(m["vkey"])
+
+
+def test_mc_sphere_is_closed_surface(oracle_mod):
+    """A fully observed sphere gives a closed 2-manifold: every mesh edge is
+    shared by exactly two triangles and V - E + F = 2."""
+    cfg = plane_cfg()
+    ctr = np.array([0.331, 0.272, 0.419])
+    o, vs = field_volume(oracle_mod, cfg, lambda p: np.linalg.norm(p - ctr, axis=-1) - 0.15, 0, 48)
+    m = o.dump_mesh()
+    tri = m["tri"].reshape(-1, 3, 4)
+    assert (tri[:, :, 0] != np.iinfo(np.int32).min).all()
+    assert len(tri) > 500
+    keys = {tuple(v) for t in tri for v in t}
+    assert keys == {tuple(k) for k in m["vkey"]}
+    edges = {}
+    for t in tri:
+        vs3 = [tuple(v) for v in t]
+        assert len(set(vs3)) == 3
+        for a, b in ((0, 1), (1, 2), (2, 0)):
+            e = tuple(sorted((vs3[a], vs3[b])))
+            edges[e] = edges.get(e, 0) + 1
+    assert set(edges.values()) == {2}
+    assert len(keys) - len(edges) + len(tri) == 2
+    # reference counts are the number of triangles using each vertex
+    use = {}
+    for t in tri:
+        for v in t:
+            use[tuple(v)] = use.get(tuple(v), 0) + 1
+    assert all(use[tuple(k)] == r for k, r in zip(m["vkey"], m["vref"]))
+
+
 # ------------------------------------------------------------------ TSDF
 def plane_cfg(**kw):
     base = replace(C1, width=64, height=48, fx=60.0, fy=60.0, cx=32.0, cy=24.0,
@@ -181,6 +264,49 @@ def test_hash_invariants(oracle_mod):
         assert (rec[:, 0] // 10 == b).sum() == 10
     for r in rec:
         assert r[1] == oracle_mod.hash_bucket(int(r[3]), int(r[4]), int(r[5]), cfg.bucket_count)
+
+
+def test_collect_matches_frustum_filter(oracle_mod):
+    """CollectBlocksInFrustum (collect_block_array.cu:25-151) keeps exactly
+    the allocated blocks whose centre lies in the 0.95-shrunk frustum of the
+    current pose; CollectAllBlocks keeps every allocated block.  The filter is
+    recomputed here in FP64; blocks within 1e-4 of a frustum face are
+    skipped (FP32 rounding decides them)."""
+    cfg = plane_cfg()
+    rng = np.random.default_rng(11)
+    depth = rng.integers(600, 3000, (cfg.height, cfg.width)).astype(np.uint16)
+    o = oracle_mod.Oracle(cfg)
+    o.set_frame(depth, np.eye(4))
+    o.alloc()
+    th = 0.25
+    pose = np.eye(4)
+    pose[:3, :3] = [[np.cos(th), 0, np.sin(th)], [0, 1, 0], [-np.sin(th), 0, np.cos(th)]]
+    pose[:3, 3] = [0.1, 0.05, 0.2]
+    o.set_frame(depth, pose)  # a second pose: some blocks leave the frustum
+    n = o.collect_visible()
+    got = {tuple(p) for p in o.dump_candidates()}
+    assert len(got) == n
+    allb = {tuple(p) for p in o.dump_blocks()["pos"]}
+    cTw = np.linalg.inv(pose.astype(np.float32).astype(np.float64))
+    vs = float(np.float32(0.02))
+    expect, ambiguous = set(), set()
+    for b in allb:
+        cen = (np.array(b) * 8 + 3.5) * vs
+        pc = cTw[:3, :3] @ cen + cTw[:3, 3]
+        u = pc[0] * cfg.fx / pc[2] + cfg.cx
+        v = pc[1] * cfg.fy / pc[2] + cfg.cy
+        nx = (2 * u - (cfg.width - 1)) / (cfg.width - 1) * 0.95
+        ny = ((cfg.height - 1) - 2 * v) / (cfg.height - 1) * 0.95
+        nz = (pc[2] - 0.1) / 4.9 * 0.95
+        margin = min(abs(abs(nx) - 1), abs(abs(ny) - 1), abs(nz), abs(nz - 1))
+        if margin < 1e-4:
+            ambiguous.add(b)
+        elif -1 <= nx <= 1 and -1 <= ny <= 1 and 0 <= nz <= 1:
+            expect.add(b)
+    assert 0 < len(expect) < len(allb)
+    assert got - ambiguous == expect
+    assert o.collect_all() == len(allb)
+    assert {tuple(p) for p in o.dump_candidates()} == allb
 
 
 # ------------------------------------------------------------------ PSDF
