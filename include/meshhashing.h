@@ -239,6 +239,74 @@ int mh_dump_cube_indices(mh_volume* v, uint32_t capacity, uint16_t* cube_host,
 int mh_dump_primal_dual(mh_volume* v, uint32_t capacity, float* pdv_host,
                         uint8_t* mask_host, uint32_t* count_host);
 
+/* FP32 pose of the current frame: wTc as given, cTw its inverse as used by
+ * every decision (sensor/rgbd_sensor.h:51-54).  Either output may be NULL. */
+int mh_frame_get_pose(mh_volume* v, float wTc[16], float cTw[16]);
+
+/* ---- integration autotuning (no counterpart in the reference) ------------
+ * UpdateBlocksSimple's tiling -- lanes per workgroup, worklist blocks fused
+ * per workgroup (hence voxels per lane) and persistent workgroups per CU --
+ * is chosen by timing every candidate on the current worklist.  Results do
+ * not depend on the tiling: each voxel is updated by one lane with the same
+ * arithmetic.  mh_tune_integrate runs after mh_collect_* and before
+ * mh_integrate_tsdf; it saves and restores the worklist's fused state and
+ * the counters around every trial, so it changes no output.               */
+typedef struct mh_tune_result {
+  int32_t tiling;               /* index of the chosen tiling               */
+  int32_t workgroup_size;       /* lanes per workgroup                      */
+  int32_t blocks_per_workgroup; /* worklist blocks per workgroup at a time  */
+  int32_t voxels_per_lane;
+  int32_t workgroups_per_cu;    /* persistent grid = CUs x this             */
+  int32_t candidates;           /* configurations timed                     */
+  float best_ms;                /* fastest trial of the chosen tiling       */
+  float default_ms;             /* fastest trial of the default tiling      */
+} mh_tune_result;
+#define MH_NUM_TILINGS 10
+int mh_tune_integrate(mh_volume* v, uint32_t reps, mh_tune_result* out);
+int mh_set_integrate_tiling(mh_volume* v, int tiling, int workgroups_per_cu);
+
+/* ---- batched scenes (no counterpart in the reference) --------------------
+ * Many independent scenes (uneven block counts and voxel sizes, one common
+ * image size) fused per stage: every stage of a frame is one launch over
+ * all scenes (pixel stages index scenes by grid z, block stages run over
+ * the concatenated (scene, block) worklist).  Each scene's results are
+ * those of running the same frames through the single-volume entry points
+ * (mh_frame_upload_depth_u16, mh_alloc_blocks, mh_collect_visible,
+ * mh_integrate_tsdf, mh_marching_cubes, mh_raycast with the frame's cTw);
+ * mh_batch_volume exposes each scene's volume for status and dumps.  TSDF
+ * scenes only.  The batch owns a HIP stream (replaceable, never NULL).     */
+typedef struct mh_batch mh_batch;
+enum {
+  MH_RUN_MESH = 1u << 0,    /* marching cubes after fusion                 */
+  MH_RUN_RAYCAST = 1u << 1, /* raycast after fusion (mh_batch_set_raycast) */
+  MH_RUN_GRAPH = 1u << 2,   /* replay the captured per-frame HIP graph
+                               (captured on first use or shape change)      */
+  MH_RUN_TUNE = 1u << 3     /* autotune integration on this frame first    */
+};
+/* Arrays of n_scenes parameter structs; all sensors share width/height.    */
+int mh_batch_create(uint32_t n_scenes, const mh_hash_params* hash,
+                    const mh_volume_params* vol, const mh_mesh_params* mesh,
+                    const mh_sensor_params* sensor, mh_batch** out);
+int mh_batch_destroy(mh_batch* b);
+int mh_batch_set_stream(mh_batch* b, void* hip_stream);
+int mh_batch_volume(mh_batch* b, uint32_t scene, mh_volume** out);
+/* Depth of every scene (n*W*H uint16, scene-major; host memory, or device
+ * memory that stays valid until the run completes when on_device != 0) and
+ * the poses wTc (n*16 floats).                                             */
+int mh_batch_frame_u16(mh_batch* b, const uint16_t* depth, int on_device,
+                       const float* wTc);
+/* Raycaster params per scene and device outputs (n*W*H depth, n*W*H*3
+ * vertex and normal floats, scene-major; any may be NULL).                 */
+int mh_batch_set_raycast(mh_batch* b, const mh_raycaster_params* p,
+                         float* depth_dev, float* vertex_dev, float* normal_dev);
+int mh_batch_set_integrate_tiling(mh_batch* b, int tiling, int workgroups_per_cu);
+/* One frame of every scene: convert -> alloc -> collect visible -> TSDF
+ * [-> marching cubes (mc: n structs, or NULL for defaults)] [-> raycast].
+ * tune_out receives the tuning result with MH_RUN_TUNE (may be NULL).     */
+int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags,
+                 mh_tune_result* tune_out);
+int mh_batch_synchronize(mh_batch* b);
+
 /* ---- synthetic depth sequences (seeded, deterministic, CPU) --------------
  * Analytic scenes: an axis-aligned room seen from inside plus spheres and
  * boxes.  kind: 0 = circular trajectory around the room centre with 3

@@ -78,6 +78,20 @@ class Status(C.Structure):
                 ("mc_blocks", C.c_uint64), ("ray_samples", C.c_uint64)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("tiling", C.c_int32), ("workgroup_size", C.c_int32),
+                ("blocks_per_workgroup", C.c_int32), ("voxels_per_lane", C.c_int32),
+                ("workgroups_per_cu", C.c_int32), ("candidates", C.c_int32),
+                ("best_ms", C.c_float), ("default_ms", C.c_float)]
+
+
+MH_NUM_TILINGS = 10
+MH_RUN_MESH = 1 << 0
+MH_RUN_RAYCAST = 1 << 1
+MH_RUN_GRAPH = 1 << 2
+MH_RUN_TUNE = 1 << 3
+
+
 class SynthSpec(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("kind", C.c_int32), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("fx", C.c_float), ("fy", C.c_float),
@@ -130,6 +144,20 @@ SIGNATURES = {
                                P_U32, P_U32]),
     "mh_dump_cube_indices": (C.c_int, [VP, C.c_uint32, P_U16, P_U32]),
     "mh_dump_primal_dual": (C.c_int, [VP, C.c_uint32, P_F32, P_U8, P_U32]),
+    "mh_frame_get_pose": (C.c_int, [VP, P_F32, P_F32]),
+    "mh_tune_integrate": (C.c_int, [VP, C.c_uint32, C.POINTER(TuneResult)]),
+    "mh_set_integrate_tiling": (C.c_int, [VP, C.c_int, C.c_int]),
+    "mh_batch_create": (C.c_int, [C.c_uint32, C.POINTER(HashParams), C.POINTER(VolumeParams),
+                                  C.POINTER(MeshParams), C.POINTER(SensorParams),
+                                  C.POINTER(VP)]),
+    "mh_batch_destroy": (C.c_int, [VP]),
+    "mh_batch_set_stream": (C.c_int, [VP, VP]),
+    "mh_batch_volume": (C.c_int, [VP, C.c_uint32, C.POINTER(VP)]),
+    "mh_batch_frame_u16": (C.c_int, [VP, VP, C.c_int, P_F32]),
+    "mh_batch_set_raycast": (C.c_int, [VP, C.POINTER(RayCasterParams), VP, VP, VP]),
+    "mh_batch_set_integrate_tiling": (C.c_int, [VP, C.c_int, C.c_int]),
+    "mh_batch_run": (C.c_int, [VP, C.POINTER(McParams), C.c_uint32, C.POINTER(TuneResult)]),
+    "mh_batch_synchronize": (C.c_int, [VP]),
     "mh_synth_pose": (C.c_int, [C.POINTER(SynthSpec), C.c_uint32, P_F32]),
     "mh_synth_depth": (C.c_int, [C.POINTER(SynthSpec), C.c_uint32, P_U16]),
     "mh_synth_depth_exact": (C.c_int, [C.POINTER(SynthSpec), C.c_uint32, P_F32, P_F64]),

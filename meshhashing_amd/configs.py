@@ -102,6 +102,27 @@ C5 = replace(C1, name="c5", width=640, height=480, fx=525.0, fy=525.0, cx=319.5,
 
 CONFIGS = {c.name: c for c in (C1, C2, C3, C5)}
 
+# Batched form: independent rooms of 320x240 frames, voxel sizes drawn from
+# {1, 2, 3, 4} cm (seeded), truncation 4 voxels, each with its own seed.
+C4_VOXEL_SIZES = (0.01, 0.02, 0.03, 0.04)
+C4_CAPACITY = {0.01: 1 << 15, 0.02: 1 << 14, 0.03: 1 << 13, 0.04: 1 << 13}
+C4_VERTICES = {0.01: 1 << 21, 0.02: 1 << 20, 0.03: 1 << 19, 0.04: 1 << 18}
+
+
+def c4_scenes(n: int = 64, frames: int = 4, seed: int = 4) -> list[Config]:
+    import numpy as np
+
+    pick = np.random.default_rng(seed).integers(0, len(C4_VOXEL_SIZES), n)
+    out = []
+    for i in range(n):
+        vs = C4_VOXEL_SIZES[int(pick[i])]
+        out.append(replace(C1, name=f"c4_{i}", voxel_size=vs, truncation=4 * vs,
+                           bucket_count=1 << 15, value_capacity=C4_CAPACITY[vs],
+                           max_vertices=C4_VERTICES[vs], max_triangles=2 * C4_VERTICES[vs],
+                           frames=frames,
+                           seed=1000 + i))
+    return out
+
 
 def small(cfg: Config, **kw) -> Config:
     """A reduced copy for quick tests (same geometry rules, fewer resources)."""

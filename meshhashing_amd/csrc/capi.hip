@@ -93,6 +93,8 @@ struct mh_volume {
   double* pd_energy = nullptr;
   uint32_t pd_npartials = 0;
   uint32_t mc_pass = 0;
+  int tiling = kDefaultIntegrateTiling;  // integration tiling (kIntegrateTilings)
+  int grid_per_cu = 8;                   // integration workgroups per CU
   std::vector<void*> owned;
 
   int grid_wg256() const { return cus * 8; }
@@ -122,6 +124,44 @@ void set_pose(mh_volume* v, const float wTc[16]) {
   std::memcpy(v->V.wTc.m, wTc, sizeof(float) * 16);
   std::memcpy(v->V.cTw.m, c32, sizeof(float) * 16);
   for (int i = 0; i < 12; ++i) v->V.cTw64.m[i] = c64[i];
+}
+
+// MarchingCubes constants of one call (thresholds, then the next pass stamp).
+int build_mc(mh_volume* v, const mh_mc_params* p, McConst* c) {
+  mh_mc_params d;
+  mh_mc_params_default(&d);
+  if (!p) p = &d;
+  if (!p->enable_sdf_gradient)
+    return fail(MH_ERR_ARG, "flat per-triangle normals are not supported");
+  c->sdf_threshold = p->sdf_threshold;
+  c->rho_min = p->rho_min;
+  const float inv = 1.0f / v->vp.voxel_size;
+  c->weight_min = p->weight_min < 0.0f ? inv * inv : p->weight_min;
+  c->snap = p->snap_distance;
+  c->flat_normals = 0;
+  c->pass = ++v->mc_pass;
+  return MH_OK;
+}
+
+// RayCaster constants (ray_caster.cu:152-224): wTc is the FP32 inverse of cTw.
+int build_ray(const mh_volume* v, const mh_raycaster_params* p, const float cTw[16], RayConst* R) {
+  if (p->width == 0 || p->height == 0 || p->width > 16384 || p->height > 16384 ||
+      !(p->raycast_step > 0.0f) || !(p->min_raycast_depth > 0.0f) ||
+      !(p->max_raycast_depth >= p->min_raycast_depth) ||
+      (p->max_raycast_depth - p->min_raycast_depth) / p->raycast_step > 65536.0f)
+    return fail(MH_ERR_ARG, "bad raycaster params (sizes, depth range, or > 65536 steps per ray)");
+  R->cam = Cam{p->fx, p->fy, p->cx, p->cy, p->width, p->height, p->min_raycast_depth,
+               p->max_raycast_depth};
+  R->min_depth = p->min_raycast_depth;
+  R->max_depth = p->max_raycast_depth;
+  R->step = p->raycast_step;
+  R->sample_sdf_threshold = p->sample_sdf_threshold;
+  R->sdf_threshold = p->sdf_threshold;
+  R->enable_gradients = p->enable_gradients;
+  R->inv_vs = 1.0f / v->vp.voxel_size;
+  std::memcpy(R->cTw.m, cTw, sizeof(float) * 16);
+  invert4<float>(cTw, R->wTc.m);
+  return MH_OK;
 }
 
 }  // namespace
@@ -391,7 +431,7 @@ int mh_collect_all(mh_volume* v, uint32_t* count_host) {
 int mh_integrate_tsdf(mh_volume* v) {
   if (check_vol(v)) return MH_ERR_ARG;
   if (!v->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
-  launch_integrate_tsdf(v->V, v->stream, v->grid_wg256());
+  launch_integrate_tsdf(v->V, v->stream, v->cus * v->grid_per_cu, v->tiling);
   HIP_TRY(hipGetLastError());
   return MH_OK;
 }
@@ -416,19 +456,9 @@ int mh_integrate_psdf(mh_volume* v) {
 
 int mh_marching_cubes(mh_volume* v, const mh_mc_params* p) {
   if (check_vol(v)) return MH_ERR_ARG;
-  mh_mc_params d;
-  mh_mc_params_default(&d);
-  if (!p) p = &d;
-  if (!p->enable_sdf_gradient)
-    return fail(MH_ERR_ARG, "flat per-triangle normals are not supported");
   McConst c;
-  c.sdf_threshold = p->sdf_threshold;
-  c.rho_min = p->rho_min;
-  const float inv = 1.0f / v->vp.voxel_size;
-  c.weight_min = p->weight_min < 0.0f ? inv * inv : p->weight_min;
-  c.snap = p->snap_distance;
-  c.flat_normals = 0;
-  c.pass = ++v->mc_pass;
+  const int rc = build_mc(v, p, &c);
+  if (rc) return rc;
   launch_marching_cubes(v->V, c, v->stream, v->grid_wg512());
   HIP_TRY(hipGetLastError());
   return MH_OK;
@@ -437,23 +467,9 @@ int mh_marching_cubes(mh_volume* v, const mh_mc_params* p) {
 int mh_raycast(mh_volume* v, const mh_raycaster_params* p, const float cTw[16], float* depth_dev,
                float* vertex_dev, float* normal_dev) {
   if (check_vol(v) || !p || !cTw) return fail(MH_ERR_ARG, "null argument");
-  if (p->width == 0 || p->height == 0 || p->width > 16384 || p->height > 16384 ||
-      !(p->raycast_step > 0.0f) || !(p->min_raycast_depth > 0.0f) ||
-      !(p->max_raycast_depth >= p->min_raycast_depth) ||
-      (p->max_raycast_depth - p->min_raycast_depth) / p->raycast_step > 65536.0f)
-    return fail(MH_ERR_ARG, "bad raycaster params (sizes, depth range, or > 65536 steps per ray)");
   RayConst R;
-  R.cam = Cam{p->fx, p->fy, p->cx, p->cy, p->width, p->height, p->min_raycast_depth,
-              p->max_raycast_depth};
-  R.min_depth = p->min_raycast_depth;
-  R.max_depth = p->max_raycast_depth;
-  R.step = p->raycast_step;
-  R.sample_sdf_threshold = p->sample_sdf_threshold;
-  R.sdf_threshold = p->sdf_threshold;
-  R.enable_gradients = p->enable_gradients;
-  R.inv_vs = 1.0f / v->vp.voxel_size;
-  std::memcpy(R.cTw.m, cTw, sizeof(float) * 16);
-  invert4<float>(cTw, R.wTc.m);
+  const int rc = build_ray(v, p, cTw, &R);
+  if (rc) return rc;
   launch_raycast(v->V, R, depth_dev, vertex_dev, normal_dev, v->stream);
   HIP_TRY(hipGetLastError());
   return MH_OK;
@@ -811,6 +827,443 @@ int mh_load_blocks(mh_volume* v, uint32_t n, const int32_t* pos_host, const floa
   HIP_TRY(hipStreamSynchronize(v->stream));
   for (void* p : {(void*)dkeys, (void*)dsdf, (void*)dw, (void*)dc, (void*)da, (void*)db})
     if (p) (void)hipFree(p);
+  return MH_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ autotuning
+// Every integration tiling (kIntegrateTilings) x persistent-grid size is
+// timed on the current worklist.  Integration is stateful, so the worklist
+// blocks' fused state and the counters are saved once and restored after
+// each trial: tuning changes no output, and every tiling gives bit-identical
+// results (each voxel is updated by one lane with the same arithmetic).
+namespace {
+
+constexpr int kGridPerCu[] = {2, 4, 8, 16, 32};
+constexpr int kDefaultGridPerCu = 8;
+
+struct Scratch {
+  std::vector<void*> ptrs;
+  ~Scratch() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  hipError_t get(T** p, size_t n) {
+    const hipError_t e = dalloc(p, n);
+    if (e == hipSuccess) ptrs.push_back((void*)*p);
+    return e;
+  }
+};
+
+struct Events {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~Events() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
+template <class Run, class Restore>
+int tune_loop(hipStream_t s, int cus, uint32_t reps, Run run, Restore restore, mh_tune_result* out,
+              int* best_t, int* best_g) {
+  Events ev;
+  HIP_TRY(hipEventCreate(&ev.a));
+  HIP_TRY(hipEventCreate(&ev.b));
+  float best = INFINITY, def = INFINITY;
+  int cands = 0;
+  *best_t = kDefaultIntegrateTiling;
+  *best_g = kDefaultGridPerCu;
+  for (int t = 0; t < kNumIntegrateTilings; ++t) {
+    for (int g : kGridPerCu) {
+      float tmin = INFINITY;
+      for (uint32_t r = 0; r < reps; ++r) {
+        HIP_TRY(hipEventRecord(ev.a, s));
+        run(t, cus * g);
+        HIP_TRY(hipEventRecord(ev.b, s));
+        HIP_TRY(hipGetLastError());
+        const int rc = restore();
+        if (rc) return rc;
+        HIP_TRY(hipEventSynchronize(ev.b));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+        tmin = fminf(tmin, ms);
+      }
+      ++cands;
+      if (tmin < best) {
+        best = tmin;
+        *best_t = t;
+        *best_g = g;
+      }
+      if (t == kDefaultIntegrateTiling && g == kDefaultGridPerCu) def = tmin;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  if (out) {
+    const Tiling tl = kIntegrateTilings[*best_t];
+    out->tiling = *best_t;
+    out->workgroup_size = tl.nt;
+    out->blocks_per_workgroup = tl.bpw;
+    out->voxels_per_lane = kBlockVoxels * tl.bpw / tl.nt;
+    out->workgroups_per_cu = *best_g;
+    out->candidates = cands;
+    out->best_ms = best;
+    out->default_ms = def;
+  }
+  return MH_OK;
+}
+
+int check_tiling(int tiling, int grid_per_cu) {
+  if (tiling < 0 || tiling >= kNumIntegrateTilings) return fail(MH_ERR_ARG, "tiling out of range");
+  if (grid_per_cu < 1 || grid_per_cu > 64) return fail(MH_ERR_ARG, "grid_per_cu out of range");
+  return MH_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ batches
+// S independent scenes of one image size.  Each scene is an ordinary volume;
+// a device table holds every scene's View, MarchingCubes and RayCaster
+// constants for the current frame, so each stage is one launch over all
+// scenes and the per-frame chain can be captured once as a HIP graph and
+// replayed (every count it needs lives on the device).
+struct mh_batch {
+  std::vector<mh_volume*> vols;
+  uint32_t n = 0, W = 0, H = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int cus = 256;
+  SceneFrame* d_frames = nullptr;
+  std::vector<SceneFrame> h_frames;
+  static constexpr int kRing = 4;  // pinned staging slots for the frame table
+  SceneFrame* pinned[kRing] = {};
+  hipEvent_t ev[kRing] = {};
+  int ring = 0;
+  uint32_t* d_work = nullptr;
+  uint32_t* d_work_count = nullptr;
+  uint16_t* d_raw = nullptr;       // staging for host depth uploads
+  const uint16_t* raw = nullptr;   // depth of the current frame (n*W*H)
+  bool have_frame = false;
+  std::vector<mh_raycaster_params> rp;
+  float *out_depth = nullptr, *out_vertex = nullptr, *out_normal = nullptr;
+  bool have_ray = false;
+  int tiling = kDefaultIntegrateTiling;
+  int grid_per_cu = kDefaultGridPerCu;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t graph_flags = 0;
+
+  BatchDev dev() const { return BatchDev{d_frames, d_work, d_work_count, n, W, H, 0}; }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
+namespace {
+
+void batch_free(mh_batch* b) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  b->drop_graph();
+  for (int k = 0; k < mh_batch::kRing; ++k) {
+    if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
+    if (b->pinned[k]) (void)hipHostFree(b->pinned[k]);
+  }
+  for (void* p : {(void*)b->d_frames, (void*)b->d_work, (void*)b->d_work_count, (void*)b->d_raw})
+    if (p) (void)hipFree(p);
+  for (mh_volume* v : b->vols) mh_volume_destroy(v);
+  if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// The per-frame chain: convert -> alloc -> collect visible -> TSDF ->
+// [marching cubes] -> [raycast], one launch per stage over all scenes.
+void batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
+  const BatchDev B = b->dev();
+  launch_batch_convert_depth(B, b->stream);
+  launch_batch_alloc(B, b->stream);
+  launch_batch_collect(B, 1, b->stream, b->cus * 4);
+  if (!integrate) return;
+  launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
+  if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->stream, b->cus * 4);
+  if (flags & MH_RUN_RAYCAST) launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream);
+}
+
+int batch_tune(mh_batch* b, uint32_t reps, mh_tune_result* out) {
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  std::vector<size_t> off(b->n + 1, 0);
+  for (uint32_t s = 0; s < b->n; ++s) off[s + 1] = off[s] + read_counters(b->vols[s]).cand_count;
+  const size_t total = off[b->n];
+  if (total == 0) return fail(MH_ERR_EMPTY, "empty worklists");
+  Scratch sc;
+  float *ss = nullptr, *sw = nullptr;
+  uint32_t* scol = nullptr;
+  Counters* sctr = nullptr;
+  HIP_TRY(sc.get(&ss, total * kBlockVoxels));
+  HIP_TRY(sc.get(&sw, total * kBlockVoxels));
+  HIP_TRY(sc.get(&scol, total * kBlockVoxels));
+  HIP_TRY(sc.get(&sctr, b->n));
+  for (uint32_t s = 0; s < b->n; ++s) {
+    const size_t o = off[s] * kBlockVoxels;
+    launch_state_save(b->vols[s]->V, ss + o, sw + o, scol + o, b->stream, b->cus);
+    HIP_TRY(hipMemcpyAsync(sctr + s, b->vols[s]->V.ctr, sizeof(Counters), hipMemcpyDeviceToDevice,
+                           b->stream));
+  }
+  const BatchDev B = b->dev();
+  auto run = [&](int t, int g) { launch_batch_integrate_tsdf(B, b->stream, g, t); };
+  auto restore = [&]() -> int {
+    for (uint32_t s = 0; s < b->n; ++s) {
+      const size_t o = off[s] * kBlockVoxels;
+      launch_state_restore(b->vols[s]->V, ss + o, sw + o, scol + o, b->stream, b->cus);
+      HIP_TRY(hipMemcpyAsync(b->vols[s]->V.ctr, sctr + s, sizeof(Counters),
+                             hipMemcpyDeviceToDevice, b->stream));
+    }
+    return MH_OK;
+  };
+  int bt = 0, bg = 0;
+  const int rc = tune_loop(b->stream, b->cus, reps, run, restore, out, &bt, &bg);
+  if (rc) return rc;
+  b->tiling = bt;
+  b->grid_per_cu = bg;
+  return MH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mh_set_integrate_tiling(mh_volume* v, int tiling, int grid_per_cu) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  const int rc = check_tiling(tiling, grid_per_cu);
+  if (rc) return rc;
+  v->tiling = tiling;
+  v->grid_per_cu = grid_per_cu;
+  return MH_OK;
+}
+
+int mh_tune_integrate(mh_volume* v, uint32_t reps, mh_tune_result* out) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  if (!v->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
+  if (reps == 0) reps = 3;
+  const uint32_t n = read_counters(v).cand_count;
+  if (n == 0) return fail(MH_ERR_EMPTY, "empty worklist");
+  Scratch sc;
+  float *ss = nullptr, *sw = nullptr;
+  uint32_t* scol = nullptr;
+  Counters* sctr = nullptr;
+  HIP_TRY(sc.get(&ss, (size_t)n * kBlockVoxels));
+  HIP_TRY(sc.get(&sw, (size_t)n * kBlockVoxels));
+  HIP_TRY(sc.get(&scol, (size_t)n * kBlockVoxels));
+  HIP_TRY(sc.get(&sctr, 1));
+  launch_state_save(v->V, ss, sw, scol, v->stream, v->cus * 4);
+  HIP_TRY(hipMemcpyAsync(sctr, v->V.ctr, sizeof(Counters), hipMemcpyDeviceToDevice, v->stream));
+  auto run = [&](int t, int g) { launch_integrate_tsdf(v->V, v->stream, g, t); };
+  auto restore = [&]() -> int {
+    launch_state_restore(v->V, ss, sw, scol, v->stream, v->cus * 4);
+    HIP_TRY(hipMemcpyAsync(v->V.ctr, sctr, sizeof(Counters), hipMemcpyDeviceToDevice, v->stream));
+    return MH_OK;
+  };
+  int bt = 0, bg = 0;
+  const int rc = tune_loop(v->stream, v->cus, reps, run, restore, out, &bt, &bg);
+  if (rc) return rc;
+  v->tiling = bt;
+  v->grid_per_cu = bg;
+  return MH_OK;
+}
+
+int mh_frame_get_pose(mh_volume* v, float wTc[16], float cTw[16]) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  if (wTc) std::memcpy(wTc, v->V.wTc.m, sizeof(float) * 16);
+  if (cTw) std::memcpy(cTw, v->V.cTw.m, sizeof(float) * 16);
+  return MH_OK;
+}
+
+int mh_batch_create(uint32_t n_scenes, const mh_hash_params* hash, const mh_volume_params* vol,
+                    const mh_mesh_params* mesh, const mh_sensor_params* sensor, mh_batch** out) {
+  if (!hash || !vol || !mesh || !sensor || !out) return fail(MH_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (n_scenes == 0 || n_scenes > kMaxScenes) return fail(MH_ERR_ARG, "n_scenes out of range");
+  for (uint32_t s = 0; s < n_scenes; ++s) {
+    if (sensor[s].width != sensor[0].width || sensor[s].height != sensor[0].height)
+      return fail(MH_ERR_ARG, "all scenes of a batch need the same image size");
+    if (hash[s].value_capacity > kMaxBatchCapacity)
+      return fail(MH_ERR_ARG, "value_capacity above 2^24 in a batch");
+  }
+  mh_batch* b = new mh_batch();
+  b->n = n_scenes;
+  b->W = sensor[0].width;
+  b->H = sensor[0].height;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+    b->cus = cus;
+  auto bail = [&](int rc) {
+    batch_free(b);
+    return rc;
+  };
+  if (hipStreamCreate(&b->stream) != hipSuccess) {
+    b->stream = nullptr;
+    return bail(fail(MH_ERR_HIP, "hipStreamCreate failed"));
+  }
+  b->own_stream = true;
+  size_t work_cap = 0;
+  for (uint32_t s = 0; s < n_scenes; ++s) {
+    mh_volume* v = nullptr;
+    const int rc = mh_volume_create(&hash[s], &vol[s], &mesh[s], &sensor[s], 0, &v);
+    if (rc) return bail(rc);
+    v->stream = b->stream;
+    b->vols.push_back(v);
+    work_cap += hash[s].value_capacity;
+  }
+  const size_t npix = (size_t)b->W * b->H * n_scenes;
+  if (dalloc(&b->d_frames, n_scenes) != hipSuccess || dalloc(&b->d_work, work_cap) != hipSuccess ||
+      dalloc(&b->d_work_count, 1) != hipSuccess || dalloc(&b->d_raw, npix) != hipSuccess)
+    return bail(fail(MH_ERR_HIP, "device allocation failed"));
+  for (int k = 0; k < mh_batch::kRing; ++k) {
+    if (hipHostMalloc((void**)&b->pinned[k], sizeof(SceneFrame) * n_scenes, hipHostMallocDefault) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(b->ev[k], b->stream) != hipSuccess)
+      return bail(fail(MH_ERR_HIP, "pinned staging allocation failed"));
+  }
+  b->h_frames.resize(n_scenes);
+  b->rp.resize(n_scenes);
+  *out = b;
+  return MH_OK;
+}
+
+int mh_batch_destroy(mh_batch* b) {
+  if (b) batch_free(b);
+  return MH_OK;
+}
+
+int mh_batch_set_stream(mh_batch* b, void* hip_stream) {
+  if (!b || !hip_stream) return fail(MH_ERR_ARG, "null argument (a batch needs a non-null stream)");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (b->own_stream) HIP_TRY(hipStreamDestroy(b->stream));
+  b->own_stream = false;
+  b->stream = (hipStream_t)hip_stream;
+  for (mh_volume* v : b->vols) v->stream = b->stream;
+  b->drop_graph();
+  return MH_OK;
+}
+
+int mh_batch_volume(mh_batch* b, uint32_t scene, mh_volume** out) {
+  if (!b || !out || scene >= b->n) return fail(MH_ERR_ARG, "bad batch, scene or output");
+  *out = b->vols[scene];
+  return MH_OK;
+}
+
+int mh_batch_frame_u16(mh_batch* b, const uint16_t* depth, int on_device, const float* wTc) {
+  if (!b || !depth || !wTc) return fail(MH_ERR_ARG, "null argument");
+  const size_t npix = (size_t)b->W * b->H * b->n;
+  if (on_device) {
+    b->raw = depth;
+  } else {
+    HIP_TRY(hipMemcpyAsync(b->d_raw, depth, npix * sizeof(uint16_t), hipMemcpyHostToDevice, b->stream));
+    b->raw = b->d_raw;
+  }
+  for (uint32_t s = 0; s < b->n; ++s) set_pose(b->vols[s], wTc + 16 * s);
+  b->have_frame = true;
+  return MH_OK;
+}
+
+int mh_batch_set_raycast(mh_batch* b, const mh_raycaster_params* p, float* depth_dev,
+                         float* vertex_dev, float* normal_dev) {
+  if (!b || !p) return fail(MH_ERR_ARG, "null argument");
+  for (uint32_t s = 0; s < b->n; ++s) {
+    if (p[s].width != b->W || p[s].height != b->H)
+      return fail(MH_ERR_ARG, "raycast image size must equal the batch's image size");
+    RayConst R;
+    const int rc = build_ray(b->vols[s], &p[s], b->vols[s]->V.cTw.m, &R);
+    if (rc) return rc;
+  }
+  b->rp.assign(p, p + b->n);
+  b->out_depth = depth_dev;
+  b->out_vertex = vertex_dev;
+  b->out_normal = normal_dev;
+  b->have_ray = true;
+  b->drop_graph();  // output pointers are baked into a captured graph
+  return MH_OK;
+}
+
+int mh_batch_set_integrate_tiling(mh_batch* b, int tiling, int grid_per_cu) {
+  if (!b) return fail(MH_ERR_ARG, "null batch");
+  const int rc = check_tiling(tiling, grid_per_cu);
+  if (rc) return rc;
+  b->tiling = tiling;
+  b->grid_per_cu = grid_per_cu;
+  b->drop_graph();
+  return MH_OK;
+}
+
+int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_result* tune_out) {
+  if (!b) return fail(MH_ERR_ARG, "null batch");
+  if (!b->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
+  if ((flags & MH_RUN_RAYCAST) && !b->have_ray) return fail(MH_ERR_STATE, "mh_batch_set_raycast first");
+  if ((flags & MH_RUN_GRAPH) && (flags & MH_RUN_TUNE))
+    return fail(MH_ERR_ARG, "tuning runs eagerly; tune first, then replay the graph");
+  // this frame's table: Views (pose, raw depth), MC pass stamps, raycast poses
+  for (uint32_t s = 0; s < b->n; ++s) {
+    mh_volume* v = b->vols[s];
+    SceneFrame& f = b->h_frames[s];
+    std::memset(&f, 0, sizeof(f));
+    if (flags & MH_RUN_MESH) {
+      const int rc = build_mc(v, mc ? &mc[s] : nullptr, &f.mc);
+      if (rc) return rc;
+    }
+    if (flags & MH_RUN_RAYCAST) {
+      const int rc = build_ray(v, &b->rp[s], v->V.cTw.m, &f.ray);
+      if (rc) return rc;
+    }
+    f.V = v->V;
+    f.V.raw = b->raw + (size_t)s * b->W * b->H;
+    v->have_frame = true;
+  }
+  const int k = b->ring;
+  b->ring = (k + 1) % mh_batch::kRing;
+  HIP_TRY(hipEventSynchronize(b->ev[k]));  // the slot's previous upload has finished
+  std::memcpy(b->pinned[k], b->h_frames.data(), sizeof(SceneFrame) * b->n);
+  HIP_TRY(hipMemcpyAsync(b->d_frames, b->pinned[k], sizeof(SceneFrame) * b->n,
+                         hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipEventRecord(b->ev[k], b->stream));
+  if (flags & MH_RUN_TUNE) {
+    batch_chain(b, flags, false);
+    HIP_TRY(hipGetLastError());
+    const int rc = batch_tune(b, 3, tune_out);
+    if (rc) return rc;
+    b->drop_graph();
+    const BatchDev B = b->dev();
+    launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
+    if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->stream, b->cus * 4);
+    if (flags & MH_RUN_RAYCAST)
+      launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream);
+  } else if (flags & MH_RUN_GRAPH) {
+    const uint32_t shape = flags & (MH_RUN_MESH | MH_RUN_RAYCAST);
+    if (!b->exec || b->graph_flags != shape) {
+      b->drop_graph();
+      HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      batch_chain(b, flags, true);
+      const hipError_t le = hipGetLastError();
+      const hipError_t ee = hipStreamEndCapture(b->stream, &b->graph);
+      HIP_TRY(le);
+      HIP_TRY(ee);
+      HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->graph_flags = shape;
+    }
+    HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  } else {
+    batch_chain(b, flags, true);
+  }
+  HIP_TRY(hipGetLastError());
+  return MH_OK;
+}
+
+int mh_batch_synchronize(mh_batch* b) {
+  if (!b) return fail(MH_ERR_ARG, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->stream));
   return MH_OK;
 }
 

@@ -83,6 +83,16 @@ __global__ void convert_depth_kernel(const uint16_t* __restrict__ raw, float* __
   if (inlier) inlier[i] = 0.1f;
 }
 
+// The same for every scene of a batch (scene = blockIdx.y).
+__global__ void batch_convert_depth_kernel(Many src, uint32_t n) {
+  const View& V = src.view(blockIdx.y);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float d = V.range_factor * (float)(int16_t)V.raw[i];
+  V.depth[i] = (d >= V.cam.min_depth && d <= V.cam.max_depth) ? d : -INFINITY;
+  if (V.inlier) V.inlier[i] = 0.1f;
+}
+
 // ------------------------------------------------------------- allocation
 // Plain-load presence test (bucket, then list).  Keys are never removed
 // inside a frame, so a hit is final; a miss is re-checked coherently.
@@ -144,7 +154,9 @@ __device__ void insert_block(const View& V, i3 p) {
 // AllocBlockArrayKernel, allocate.cu:8-99: per pixel, walk the blocks between
 // depth - trunc and depth + trunc with a 3-D DDA (Amanatides-Woo in block
 // units) and insert every block whose centre lies in the frustum.
-__global__ __launch_bounds__(256) void alloc_kernel(View V) {
+template <class Src>
+__global__ __launch_bounds__(256) void alloc_kernel(Src src) {
+  const View& V = src.view(blockIdx.z);
   const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15);
   const uint32_t y = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (x >= V.cam.width || y >= V.cam.height) return;
@@ -210,8 +222,11 @@ __global__ __launch_bounds__(256) void alloc_kernel(View V) {
 // probe offsets 1..linked_list_size from the bucket's last slot, skipping
 // multiples of bucket_size, and link the first free slot in as list head.
 // Also clears the worklist counter for the collect pass that follows.
-__global__ void list_insert_kernel(View V) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One workgroup per scene (blockIdx.x), one working lane.
+template <class Src>
+__global__ void list_insert_kernel(Src src) {
+  const View& V = src.view(blockIdx.x);
+  if (threadIdx.x != 0) return;
   const uint32_t n = min(V.ctr->ovf_count, kOverflowQueue);
   for (uint32_t q = 0; q < n; ++q) {
     const uint64_t key = V.ovf[q];
@@ -265,7 +280,13 @@ __global__ void list_insert_kernel(View V) {
 // block pool, not over the whole entry table: the pool is dense up to its
 // high-water mark, which is ~100x smaller than entry_count.  One atomic per
 // wave; the worklist order is not deterministic (neither is the reference's).
-__global__ __launch_bounds__(256) void collect_kernel(View V, int frustum) {
+// Scene = blockIdx.y.  A batch also appends (scene, block) items to the
+// concatenated worklist that the batched block kernels run over.
+template <class Src>
+__global__ __launch_bounds__(256) void collect_kernel(Src src, int frustum, uint32_t* work,
+                                                      uint32_t* work_count) {
+  const uint32_t scene = blockIdx.y;
+  const View& V = src.view(scene);
   const uint32_t n = V.ctr->block_hwm;
   const uint32_t lane = threadIdx.x & 63u;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -279,18 +300,26 @@ __global__ __launch_bounds__(256) void collect_kernel(View V, int frustum) {
     const uint64_t mask = __ballot(take);
     const uint32_t cnt = (uint32_t)__popcll(mask);
     const uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    uint32_t out = 0;
-    if (lane == 0 && cnt) out = atomicAdd(&V.ctr->cand_count, cnt);
+    uint32_t out = 0, gout = 0;
+    if (lane == 0 && cnt) {
+      out = atomicAdd(&V.ctr->cand_count, cnt);
+      if (work) gout = atomicAdd(work_count, cnt);
+    }
     out = __shfl(out, 0);
     if (take) V.cand[out + prefix] = (int32_t)p;
+    if (work) {
+      gout = __shfl(gout, 0);
+      if (take) work[gout + prefix] = (scene << kSceneShift) | p;
+    }
   }
 }
 
 // ------------------------------------------------------------- TSDF fusion
 // UpdateBlocksSimpleKernel (update_simple.cu:13-77) + Voxel::Update
-// (voxel.h:107-115).  One workgroup per worklist block (persistent loop),
-// 2 voxels per lane, coalesced SoA state.  The pixel choice and the skip
-// tests are FP32 decisions; the observed SDF value uses an FP64 camera z.
+// (voxel.h:107-115).  Persistent workgroups over the worklist, a tiling of
+// lanes to blocks chosen by the autotuner, coalesced SoA state.  The pixel
+// choice and the skip tests are FP32 decisions; the observed SDF value uses
+// an FP64 camera z.
 __device__ __forceinline__ uint32_t blend_color(uint32_t c, uint32_t d) {
   uint32_t out = 0;
   for (int k = 0; k < 3; ++k) {
@@ -302,47 +331,95 @@ __device__ __forceinline__ uint32_t blend_color(uint32_t c, uint32_t d) {
   return out;
 }
 
-__global__ __launch_bounds__(256) void integrate_tsdf_kernel(View V) {
-  const uint32_t n = V.ctr->cand_count;
-  const Geo g = V.geo;
-  const Cam c = V.cam;
+// Fuse the observation into voxel i of worklist block ptr; true if the
+// voxel's state changed.
+__device__ __forceinline__ bool integrate_voxel(const View& V, const i3 b, int32_t ptr, uint32_t i) {
+  const Geo& g = V.geo;
+  const Cam& c = V.cam;
+  const i3 o = devectorize(i);
+  const i3 vp = i3{b.x * kBlockSide + o.x, b.y * kBlockSide + o.y, b.z * kBlockSide + o.z};
+  const f3 cp = xform(V.cTw, voxel_to_world(g, vp));
+  const uint32_t u = (uint32_t)f2i(proj_u(c, cp) + 0.5f);
+  const uint32_t v = (uint32_t)f2i(proj_v(c, cp) + 0.5f);
+  if (u >= c.width || v >= c.height) return false;
+  const float depth = V.depth[v * c.width + u];
+  if (depth == -INFINITY || depth == 0.0f || depth >= g.sdf_upper_bound) return false;
+  const float sdf_dec = depth - cp.z;
+  const float wgt = fmaxf(10.0f * g.weight_sample * (1.0f - normalize_depth(depth, c.min_depth, c.max_depth)), 1.0f);
+  const float tr = truncate_distance(g, depth);
+  if (sdf_dec <= -tr) return false;
   const double vsd = (double)g.voxel_size;
-  const float w10 = 10.0f * g.weight_sample;
-  const uint32_t delta_color = 255u << 8;  // (0, 255, 0): no colour stream
-  uint32_t nupd = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&V.ctr->blocks_integrated, (unsigned long long)n);
-  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
-    const int32_t ptr = V.cand[ci];
+  const double X = (double)vp.x * vsd, Y = (double)vp.y * vsd, Z = (double)vp.z * vsd;
+  const double z64 = fma(V.cTw64.m[8], X, fma(V.cTw64.m[9], Y, fma(V.cTw64.m[10], Z, V.cTw64.m[11])));
+  double s = (double)depth - z64;
+  const double trd = (double)tr;
+  s = s > trd ? trd : (s < -trd ? -trd : s);
+  const float sdf = (float)s;
+  const size_t idx = (size_t)ptr * kBlockVoxels + i;
+  const float so = V.sdf[idx];
+  const float wo = V.weight[idx];
+  V.sdf[idx] = (so * wo + sdf * wgt) / (wo + wgt);
+  V.weight[idx] = wo + wgt;
+  V.color[idx] = blend_color(V.color[idx], 255u << 8);  // (0, 255, 0): no colour stream
+  return true;
+}
+
+// One workgroup of NT lanes fuses BPW worklist blocks at a time (NT / BPW
+// lanes per block, a multiple of the wave size, so every wave works on one
+// block of one scene).  Counters are flushed per wave whenever the scene
+// changes, so a batch's per-scene counters equal those of separate runs.
+template <class Src, int NT, int BPW>
+__global__ __launch_bounds__(NT) void integrate_tsdf_kernel(Src src) {
+  constexpr uint32_t kLanes = NT / BPW;
+  static_assert(kLanes % 64 == 0 && kBlockVoxels % kLanes == 0, "tiling");
+  const uint32_t n = src.count();
+  // kLanes is a multiple of the wave size, so sub is uniform per wave
+  const uint32_t sub = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes), lt = threadIdx.x % kLanes;
+  uint32_t nupd = 0, nblk = 0, cur = ~0u;
+  for (uint32_t ci = blockIdx.x * BPW + sub; ci < n; ci += gridDim.x * BPW) {
+    uint32_t scene;
+    const int32_t ptr = src.item(ci, &scene);
+    if (scene != cur) {
+      if (cur != ~0u) {
+        const View& P = src.view(cur);
+        wave_add_u64(&P.ctr->voxel_updates, nupd);
+        wave_add_u64(&P.ctr->blocks_integrated, nblk);
+      }
+      nupd = 0;
+      nblk = 0;
+      cur = scene;
+    }
+    const View& V = src.view(scene);
     const i3 b = unpack_key(V.block_keys[ptr]);
+    nblk += lt == 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kBlockVoxels / kLanes; ++k) nupd += integrate_voxel(V, b, ptr, lt + k * kLanes);
+  }
+  if (cur != ~0u) {
+    const View& P = src.view(cur);
+    wave_add_u64(&P.ctr->voxel_updates, nupd);
+    wave_add_u64(&P.ctr->blocks_integrated, nblk);
+  }
+}
+
+// Save / restore the fused state of the worklist blocks (dense, worklist order).
+__global__ __launch_bounds__(256) void state_copy_kernel(View V, float* sdf, float* weight,
+                                                         uint32_t* color, int restore) {
+  const uint32_t n = V.ctr->cand_count;
+  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
+    const size_t dst = (size_t)V.cand[ci] * kBlockVoxels, buf = (size_t)ci * kBlockVoxels;
     for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x) {
-      const i3 o = devectorize(i);
-      const i3 vp = i3{b.x * kBlockSide + o.x, b.y * kBlockSide + o.y, b.z * kBlockSide + o.z};
-      const f3 cp = xform(V.cTw, voxel_to_world(g, vp));
-      const uint32_t u = (uint32_t)f2i(proj_u(c, cp) + 0.5f);
-      const uint32_t v = (uint32_t)f2i(proj_v(c, cp) + 0.5f);
-      if (u >= c.width || v >= c.height) continue;
-      const float depth = V.depth[v * c.width + u];
-      if (depth == -INFINITY || depth == 0.0f || depth >= g.sdf_upper_bound) continue;
-      const float sdf_dec = depth - cp.z;
-      const float wgt = fmaxf(w10 * (1.0f - normalize_depth(depth, c.min_depth, c.max_depth)), 1.0f);
-      const float tr = truncate_distance(g, depth);
-      if (sdf_dec <= -tr) continue;
-      const double X = (double)vp.x * vsd, Y = (double)vp.y * vsd, Z = (double)vp.z * vsd;
-      const double z64 = fma(V.cTw64.m[8], X, fma(V.cTw64.m[9], Y, fma(V.cTw64.m[10], Z, V.cTw64.m[11])));
-      double s = (double)depth - z64;
-      const double trd = (double)tr;
-      s = s > trd ? trd : (s < -trd ? -trd : s);
-      const float sdf = (float)s;
-      const size_t idx = (size_t)ptr * kBlockVoxels + i;
-      const float so = V.sdf[idx];
-      const float wo = V.weight[idx];
-      V.sdf[idx] = (so * wo + sdf * wgt) / (wo + wgt);
-      V.weight[idx] = wo + wgt;
-      V.color[idx] = blend_color(V.color[idx], delta_color);
-      ++nupd;
+      if (restore) {
+        V.sdf[dst + i] = sdf[buf + i];
+        V.weight[dst + i] = weight[buf + i];
+        V.color[dst + i] = color[buf + i];
+      } else {
+        sdf[buf + i] = V.sdf[dst + i];
+        weight[buf + i] = V.weight[dst + i];
+        color[buf + i] = V.color[dst + i];
+      }
     }
   }
-  wave_add_u64(&V.ctr->voxel_updates, nupd);
 }
 
 // ------------------------------------------------------------- PSDF fusion
@@ -501,13 +578,24 @@ __global__ __launch_bounds__(256) void load_scatter_kernel(View V, const uint64_
 
 void launch_load_insert(const View& V, const uint64_t* keys, uint32_t n, hipStream_t s) {
   load_insert_kernel<<<(n + 255) / 256, 256, 0, s>>>(V, keys, n);
-  list_insert_kernel<<<1, 64, 0, s>>>(V);
+  list_insert_kernel<<<1, 64, 0, s>>>(One{V});
 }
 
 void launch_load_scatter(const View& V, const uint64_t* keys, uint32_t n, const float* sdf,
                          const float* weight, const uint32_t* color, const float* a,
                          const float* b, hipStream_t s) {
   load_scatter_kernel<<<n < 4096u ? n : 4096u, 256, 0, s>>>(V, keys, n, sdf, weight, color, a, b);
+}
+
+void launch_state_save(const View& V, float* sdf, float* weight, uint32_t* color, hipStream_t s,
+                       int grid) {
+  state_copy_kernel<<<grid, 256, 0, s>>>(V, sdf, weight, color, 0);
+}
+
+void launch_state_restore(const View& V, const float* sdf, const float* weight,
+                          const uint32_t* color, hipStream_t s, int grid) {
+  state_copy_kernel<<<grid, 256, 0, s>>>(V, const_cast<float*>(sdf), const_cast<float*>(weight),
+                                         const_cast<uint32_t*>(color), 1);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -526,17 +614,30 @@ void launch_convert_depth(const View& V, const uint16_t* raw, hipStream_t s) {
 
 void launch_alloc(const View& V, hipStream_t s) {
   dim3 grid((V.cam.width + 15) / 16, (V.cam.height + 15) / 16);
-  alloc_kernel<<<grid, 256, 0, s>>>(V);
-  list_insert_kernel<<<1, 64, 0, s>>>(V);
+  alloc_kernel<<<grid, 256, 0, s>>>(One{V});
+  list_insert_kernel<<<1, 64, 0, s>>>(One{V});
 }
 
 void launch_collect(const View& V, int frustum, hipStream_t s, int grid) {
   set_u32_kernel<<<1, 64, 0, s>>>(&V.ctr->cand_count, 0u);
-  collect_kernel<<<grid, 256, 0, s>>>(V, frustum);
+  collect_kernel<<<grid, 256, 0, s>>>(One{V}, frustum, nullptr, nullptr);
 }
 
-void launch_integrate_tsdf(const View& V, hipStream_t s, int grid) {
-  integrate_tsdf_kernel<<<grid, 256, 0, s>>>(V);
+namespace {
+template <class Src, int T = 0>
+void integrate_dispatch(const Src& src, hipStream_t s, int grid, int tiling) {
+  if constexpr (T < kNumIntegrateTilings) {
+    constexpr int NT = kIntegrateTilings[T].nt, BPW = kIntegrateTilings[T].bpw;
+    if (tiling == T)
+      integrate_tsdf_kernel<Src, NT, BPW><<<grid, NT, 0, s>>>(src);
+    else
+      integrate_dispatch<Src, T + 1>(src, s, grid, tiling);
+  }
+}
+}  // namespace
+
+void launch_integrate_tsdf(const View& V, hipStream_t s, int grid, int tiling) {
+  integrate_dispatch(One{V}, s, grid, tiling);
 }
 
 void launch_predict_inlier(const View& V, hipStream_t s, int grid) {
@@ -545,6 +646,35 @@ void launch_predict_inlier(const View& V, hipStream_t s, int grid) {
 
 void launch_integrate_psdf(const View& V, hipStream_t s, int grid) {
   integrate_psdf_kernel<<<grid, 256, 0, s>>>(V);
+}
+
+// ------------------------------------------------------------ batched launchers
+// Per-scene worklist counters, the concatenated worklist's count and the
+// overflow queues are cleared by kernels (a captured graph holds no memsets).
+__global__ void batch_reset_counts_kernel(Many src, uint32_t nscenes, uint32_t* work_count) {
+  for (uint32_t t = threadIdx.x; t < nscenes; t += blockDim.x) src.view(t).ctr->cand_count = 0u;
+  if (threadIdx.x == 0) *work_count = 0u;
+}
+
+void launch_batch_convert_depth(const BatchDev& B, hipStream_t s) {
+  const uint32_t n = B.width * B.height;
+  batch_convert_depth_kernel<<<dim3((n + 255) / 256, B.nscenes), 256, 0, s>>>(B.src(), n);
+}
+
+void launch_batch_alloc(const BatchDev& B, hipStream_t s) {
+  dim3 grid((B.width + 15) / 16, (B.height + 15) / 16, B.nscenes);
+  alloc_kernel<<<grid, 256, 0, s>>>(B.src());
+  list_insert_kernel<<<B.nscenes, 64, 0, s>>>(B.src());
+}
+
+void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid) {
+  batch_reset_counts_kernel<<<1, 256, 0, s>>>(B.src(), B.nscenes, B.work_count);
+  const uint32_t gx = (uint32_t)grid / B.nscenes > 0 ? (uint32_t)grid / B.nscenes : 1u;
+  collect_kernel<<<dim3(gx, B.nscenes), 256, 0, s>>>(B.src(), frustum, B.work, B.work_count);
+}
+
+void launch_batch_integrate_tsdf(const BatchDev& B, hipStream_t s, int grid, int tiling) {
+  integrate_dispatch(B.src(), s, grid, tiling);
 }
 
 }  // namespace mh

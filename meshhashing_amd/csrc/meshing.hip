@@ -274,8 +274,32 @@ __device__ void write_vertex(const Shell<PSDF>& S, const View& V, const McConst&
 // 10^3 halo of cube validities and the meshed flags of the 27 blocks), so
 // no atomics are needed; the other cubes only make sure the vertex slot is
 // allocated.  Any writer would store the same bits (canonical vertices).
-template <bool PSDF>
-__global__ __launch_bounds__(512) void mc_surfel_kernel(View V, McConst mc) {
+__device__ __forceinline__ const McConst& mc_of(const One&, uint32_t, const McConst& mc) { return mc; }
+__device__ __forceinline__ const McConst& mc_of(const Many& m, uint32_t s, const McConst&) {
+  return m.f[s].mc;
+}
+
+// Work counters of one scene, flushed per wave.
+struct McCounts {
+  uint32_t writes = 0, assigned = 0, blocks = 0;
+  int delta = 0;
+  bool oob = false;
+  __device__ void flush(const View& V) {
+    if (__any(oob)) {
+      if ((threadIdx.x & 63u) == 0) flag(V, MH_STATUS_STENCIL_RANGE);
+    }
+    wave_add_u64(&V.ctr->mc_vertex_writes, writes);
+    wave_add_u64(&V.ctr->mc_triangle_writes, assigned);
+    wave_add_u64(&V.ctr->mc_blocks, blocks);
+    for (int off = 32; off > 0; off >>= 1) delta += __shfl_xor(delta, off);
+    if ((threadIdx.x & 63u) == 0 && delta)
+      atomicAdd(&V.ctr->tri_live, (unsigned long long)(long long)delta);
+    *this = McCounts{};
+  }
+};
+
+template <bool PSDF, class Src>
+__global__ __launch_bounds__(512) void mc_surfel_kernel(Src src, McConst mc1) {
   __shared__ Shell<PSDF> S;
   __shared__ int32_t nb[27];
   __shared__ uint8_t meshed[27];
@@ -283,14 +307,19 @@ __global__ __launch_bounds__(512) void mc_surfel_kernel(View V, McConst mc) {
   __shared__ int32_t qptr[kQueue];
   __shared__ uint16_t qkey[kQueue];
   __shared__ uint32_t qn;
-  const uint32_t n = V.ctr->cand_count;
-  const Geo g = V.geo;
-  uint32_t nwrites = 0, tri_assigned = 0;
-  int tri_delta = 0;
-  bool oob = false;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&V.ctr->mc_blocks, (unsigned long long)n);
+  const uint32_t n = src.count();
+  McCounts cnt;
+  uint32_t cur = ~0u;
   for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
-    const int32_t ptr = V.cand[ci];
+    uint32_t scene;
+    const int32_t ptr = src.item(ci, &scene);
+    if (scene != cur) {
+      if (cur != ~0u) cnt.flush(src.view(cur));
+      cur = scene;
+    }
+    const View& V = src.view(scene);
+    const McConst& mc = mc_of(src, scene, mc1);
+    cnt.blocks += threadIdx.x == 0;
     const i3 b = unpack_key(V.block_keys[ptr]);
     stage_shell<PSDF>(V, b, ptr, S, nb);
     if (threadIdx.x < 27) meshed[threadIdx.x] = nb[threadIdx.x] >= 0 && V.block_mark[nb[threadIdx.x]] == mc.pass;
@@ -313,8 +342,8 @@ __global__ __launch_bounds__(512) void mc_surfel_kernel(View V, McConst mc) {
     if (!cube_index(S, o.x, o.y, o.z, mc, &cube)) cube = 0;
     V.cube[idx] = (uint16_t)((uint32_t)cube | (prev << 8));
     // live triangles of a cube = its configuration's table row
-    tri_delta += kMc.ntri[cube] - kMc.ntri[prev];
-    if ((uint32_t)cube != prev) tri_assigned += (uint32_t)kMc.ntri[cube];
+    cnt.delta += kMc.ntri[cube] - kMc.ntri[prev];
+    if ((uint32_t)cube != prev) cnt.assigned += (uint32_t)kMc.ntri[cube];
     const unsigned em = (cube != 0 && cube != 255) ? kMc.edges[cube] : 0u;
     // uniform loop: every lane takes part in the wave-aggregated pops
     for (int e = 0; e < 12; ++e) {
@@ -364,31 +393,30 @@ __global__ __launch_bounds__(512) void mc_surfel_kernel(View V, McConst mc) {
         qptr[q] = cur;
         qkey[q] = (uint16_t)key;
       } else {
-        write_vertex<PSDF>(S, V, mc, base, key, cur, &oob);
-        ++nwrites;
+        write_vertex<PSDF>(S, V, mc, base, key, cur, &cnt.oob);
+        ++cnt.writes;
       }
     }
     __syncthreads();
     const uint32_t nq = qn < kQueue ? qn : kQueue;
     for (uint32_t j = threadIdx.x; j < nq; j += blockDim.x) {
-      write_vertex<PSDF>(S, V, mc, base, qkey[j], qptr[j], &oob);
-      ++nwrites;
+      write_vertex<PSDF>(S, V, mc, base, qkey[j], qptr[j], &cnt.oob);
+      ++cnt.writes;
     }
     __syncthreads();
   }
-  if (oob) flag(V, MH_STATUS_STENCIL_RANGE);
-  wave_add_u64(&V.ctr->mc_vertex_writes, nwrites);
-  wave_add_u64(&V.ctr->mc_triangle_writes, tri_assigned);
-  for (int off = 32; off > 0; off >>= 1) tri_delta += __shfl_xor(tri_delta, off);
-  if ((threadIdx.x & 63u) == 0 && tri_delta)
-    atomicAdd(&V.ctr->tri_live, (unsigned long long)(long long)tri_delta);
+  if (cur != ~0u) cnt.flush(src.view(cur));
 }
 
 // Meshed-in-this-pass marks for the writer rule.
-__global__ void mc_mark_kernel(View V, uint32_t pass) {
-  const uint32_t n = V.ctr->cand_count;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    V.block_mark[V.cand[i]] = pass;
+template <class Src>
+__global__ void mc_mark_kernel(Src src, McConst mc1) {
+  const uint32_t n = src.count();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t scene;
+    const int32_t ptr = src.item(i, &scene);
+    src.view(scene).block_mark[ptr] = mc_of(src, scene, mc1).pass;
+  }
 }
 
 // ---------------------------------------------------------------- pass 2
@@ -409,13 +437,16 @@ __global__ void mc_mark_kernel(View V, uint32_t pass) {
 // before recycling, :187-205).  Cube configurations of the block and of its
 // -x/-y/-z neighbours (voxels -1..7) are staged in LDS.
 constexpr int kCfg = 9;  // local cube coordinates -1..7
-__global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(View V) {
+template <class Src>
+__global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src) {
   __shared__ int32_t nb[8];  // blocks at offsets (dx, dy, dz) in {-1, 0}^3
   __shared__ uint8_t cfg[kCfg * kCfg * kCfg];
   __shared__ int counts[2];
-  const uint32_t n = V.ctr->cand_count;
+  const uint32_t n = src.count();
   for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
-    const int32_t ptr = V.cand[ci];
+    uint32_t scene;
+    const int32_t ptr = src.item(ci, &scene);
+    const View& V = src.view(scene);
     const i3 b = unpack_key(V.block_keys[ptr]);
     if (threadIdx.x < 8) {
       const int t = (int)threadIdx.x;
@@ -473,12 +504,22 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(View V) {
 }
 
 void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int grid) {
-  mc_mark_kernel<<<grid / 4, 256, 0, s>>>(V, p.pass);
+  mc_mark_kernel<<<grid / 4, 256, 0, s>>>(One{V}, p);
   if (V.va)
-    mc_surfel_kernel<true><<<grid, 512, 0, s>>>(V, p);
+    mc_surfel_kernel<true><<<grid, 512, 0, s>>>(One{V}, p);
   else
-    mc_surfel_kernel<false><<<grid, 512, 0, s>>>(V, p);
-  mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(V);
+    mc_surfel_kernel<false><<<grid, 512, 0, s>>>(One{V}, p);
+  mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(One{V});
+}
+
+void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int grid) {
+  const McConst unused{};
+  mc_mark_kernel<<<grid / 4, 256, 0, s>>>(B.src(), unused);
+  if (B.psdf)
+    mc_surfel_kernel<true><<<grid, 512, 0, s>>>(B.src(), unused);
+  else
+    mc_surfel_kernel<false><<<grid, 512, 0, s>>>(B.src(), unused);
+  mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(B.src());
 }
 
 }  // namespace mh

@@ -255,6 +255,7 @@ struct View {
   Cam cam;
   float range_factor;
   float* depth;          // W*H metric depth of the current frame
+  const uint16_t* raw;   // W*H raw depth of the current frame (batched conversion)
   float* inlier;         // W*H PSDF inlier ratio
   M4 wTc, cTw;           // FP32 pose (decisions)
   D34 cTw64;             // FP64 world->camera (values)

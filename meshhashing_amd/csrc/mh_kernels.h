@@ -1,23 +1,11 @@
-// mh_kernels.h -- host launchers of the HIP kernels (one per .hip file).
+// mh_kernels.h -- kernel sources (one volume or a batch of scenes) and the
+// host launchers of the HIP kernels (one group per .hip file).
 #pragma once
 #include "mh_core.h"
 
 namespace mh {
 
-// mapping.hip
-void launch_reset(const View& V, hipStream_t s, int grid);
-void launch_convert_depth(const View& V, const uint16_t* raw, hipStream_t s);
-void launch_alloc(const View& V, hipStream_t s);
-void launch_collect(const View& V, int frustum, hipStream_t s, int grid);
-void launch_integrate_tsdf(const View& V, hipStream_t s, int grid);
-void launch_predict_inlier(const View& V, hipStream_t s, int grid);
-void launch_integrate_psdf(const View& V, hipStream_t s, int grid);
-void launch_load_insert(const View& V, const uint64_t* keys, uint32_t n, hipStream_t s);
-void launch_load_scatter(const View& V, const uint64_t* keys, uint32_t n, const float* sdf,
-                         const float* weight, const uint32_t* color, const float* a,
-                         const float* b, hipStream_t s);
-
-// meshing.hip
+// meshing.hip constants
 struct McConst {
   float sdf_threshold;
   float rho_min;
@@ -26,9 +14,8 @@ struct McConst {
   int flat_normals;
   uint32_t pass;  // 1, 2, ...: per-call stamp for single-writer vertex payloads
 };
-void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int grid);
 
-// raycast.hip
+// raycast.hip constants
 struct RayConst {
   Cam cam;
   float min_depth, max_depth, step;
@@ -37,8 +24,102 @@ struct RayConst {
   float inv_vs;  // 1 / voxel_size (fast empty-space test only)
   M4 cTw, wTc;
 };
+
+// Everything a batched kernel needs about one scene for one frame.
+struct SceneFrame {
+  View V;
+  McConst mc;
+  RayConst ray;
+};
+
+// A batch work item: scene << kSceneShift | block pool index.
+constexpr uint32_t kSceneShift = 24;
+constexpr uint32_t kMaxScenes = 256;
+constexpr uint32_t kMaxBatchCapacity = 1u << kSceneShift;
+
+// Kernel sources.  A kernel templated on its source runs either over one
+// volume (One: the View is a kernel argument, the worklist is the volume's
+// own) or over a batch of independent scenes (Many: per-scene state lives in
+// a device table of SceneFrames; pixel kernels take the scene from
+// blockIdx.z, worklist kernels from the concatenated (scene, block) list).
+// Items of one scene are processed by exactly the per-scene code.
+struct One {
+  View V;
+  __device__ __forceinline__ const View& view(uint32_t) const { return V; }
+  __device__ __forceinline__ uint32_t count() const { return V.ctr->cand_count; }
+  __device__ __forceinline__ int32_t item(uint32_t ci, uint32_t* scene) const {
+    *scene = 0;
+    return V.cand[ci];
+  }
+};
+
+struct Many {
+  const SceneFrame* __restrict__ f;
+  const uint32_t* __restrict__ work;
+  const uint32_t* work_count;
+  __device__ __forceinline__ const View& view(uint32_t s) const { return f[s].V; }
+  __device__ __forceinline__ uint32_t count() const { return *work_count; }
+  __device__ __forceinline__ int32_t item(uint32_t ci, uint32_t* scene) const {
+    const uint32_t w = work[ci];
+    *scene = w >> kSceneShift;
+    return (int32_t)(w & (kMaxBatchCapacity - 1u));
+  }
+};
+
+// Device-side description of a batch, for the batched launchers.
+struct BatchDev {
+  SceneFrame* frames;   // device table, nscenes entries
+  uint32_t* work;       // concatenated worklist
+  uint32_t* work_count;
+  uint32_t nscenes;
+  uint32_t width, height;  // common image size
+  int psdf;             // scenes carry PSDF fields (all or none)
+  Many src() const { return Many{frames, work, work_count}; }
+};
+
+// Integration tilings: workgroup size x blocks per workgroup (each block
+// gets NT / BPW lanes, a multiple of 64, and 512 * BPW / NT voxels per lane).
+struct Tiling {
+  int nt, bpw;
+};
+constexpr Tiling kIntegrateTilings[] = {{64, 1},  {128, 1}, {256, 1}, {512, 1}, {128, 2},
+                                        {256, 2}, {256, 4}, {512, 2}, {512, 4}, {512, 8}};
+constexpr int kNumIntegrateTilings = sizeof(kIntegrateTilings) / sizeof(kIntegrateTilings[0]);
+constexpr int kDefaultIntegrateTiling = 2;  // 256 lanes, one block, 2 voxels per lane
+
+// mapping.hip
+void launch_reset(const View& V, hipStream_t s, int grid);
+void launch_convert_depth(const View& V, const uint16_t* raw, hipStream_t s);
+void launch_alloc(const View& V, hipStream_t s);
+void launch_collect(const View& V, int frustum, hipStream_t s, int grid);
+void launch_integrate_tsdf(const View& V, hipStream_t s, int grid, int tiling);
+void launch_predict_inlier(const View& V, hipStream_t s, int grid);
+void launch_integrate_psdf(const View& V, hipStream_t s, int grid);
+void launch_load_insert(const View& V, const uint64_t* keys, uint32_t n, hipStream_t s);
+void launch_load_scatter(const View& V, const uint64_t* keys, uint32_t n, const float* sdf,
+                         const float* weight, const uint32_t* color, const float* a,
+                         const float* b, hipStream_t s);
+// worklist state save / restore (autotuning runs the stateful integration
+// repeatedly from one starting state)
+void launch_state_save(const View& V, float* sdf, float* weight, uint32_t* color, hipStream_t s,
+                       int grid);
+void launch_state_restore(const View& V, const float* sdf, const float* weight,
+                          const uint32_t* color, hipStream_t s, int grid);
+
+void launch_batch_convert_depth(const BatchDev& B, hipStream_t s);
+void launch_batch_alloc(const BatchDev& B, hipStream_t s);
+void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid);
+void launch_batch_integrate_tsdf(const BatchDev& B, hipStream_t s, int grid, int tiling);
+
+// meshing.hip
+void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int grid);
+void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int grid);
+
+// raycast.hip
 void launch_raycast(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,
                     hipStream_t s);
+void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,
+                          hipStream_t s);
 
 // primal_dual.hip
 void launch_pd_neighbors(const View& V, int32_t* nbr, hipStream_t s, int grid);

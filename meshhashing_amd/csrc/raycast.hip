@@ -186,14 +186,24 @@ __device__ __forceinline__ uint32_t cast_pixel(const View& V, const RayConst& R,
   return nsamp;
 }
 
-__global__ __launch_bounds__(64) void raycast_kernel(View V, RayConst R, float* __restrict__ depth_out,
+__device__ __forceinline__ const RayConst& ray_of(const One&, uint32_t, const RayConst& r) { return r; }
+__device__ __forceinline__ const RayConst& ray_of(const Many& m, uint32_t s, const RayConst&) {
+  return m.f[s].ray;
+}
+
+// Scene = blockIdx.z; a batch's outputs are scene-major (W*H per scene).
+template <class Src>
+__global__ __launch_bounds__(64) void raycast_kernel(Src src, RayConst R1, float* __restrict__ depth_out,
                                                      float* __restrict__ vertex_out,
                                                      float* __restrict__ normal_out) {
+  const uint32_t scene = blockIdx.z;
+  const View& V = src.view(scene);
+  const RayConst& R = ray_of(src, scene, R1);
   const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7);
   const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
   uint32_t ns = 0;
   if (x < R.cam.width && y < R.cam.height) {
-    const uint32_t pix = y * R.cam.width + x;
+    const size_t pix = (size_t)scene * R.cam.width * R.cam.height + y * R.cam.width + x;
     float d;
     f3 vv, nn;
     ns = cast_pixel(V, R, x, y, &d, &vv, &nn);
@@ -215,7 +225,13 @@ __global__ __launch_bounds__(64) void raycast_kernel(View V, RayConst R, float* 
 void launch_raycast(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,
                     hipStream_t s) {
   dim3 grid((p.cam.width + 7) / 8, (p.cam.height + 7) / 8);
-  raycast_kernel<<<grid, 64, 0, s>>>(V, p, depth, vertex, normal);
+  raycast_kernel<<<grid, 64, 0, s>>>(One{V}, p, depth, vertex, normal);
+}
+
+void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,
+                          hipStream_t s) {
+  dim3 grid((B.width + 7) / 8, (B.height + 7) / 8, B.nscenes);
+  raycast_kernel<<<grid, 64, 0, s>>>(B.src(), RayConst{}, depth, vertex, normal);
 }
 
 }  // namespace mh

@@ -41,6 +41,15 @@ def synth_depth_exact(cfg: Config, wTc: np.ndarray) -> np.ndarray:
 class Volume:
     """Hash table + block pool + mesh + frame buffers on the current GPU."""
 
+    owned = True
+
+    @classmethod
+    def wrap(cls, cfg: Config, handle: C.c_void_p) -> "Volume":
+        """A view of a volume owned by someone else (a batch's scene)."""
+        v = cls.__new__(cls)
+        v.cfg, v.lib, v.h, v.owned = cfg, A.load(), handle, False
+        return v
+
     def __init__(self, cfg: Config):
         self.cfg = cfg
         self.lib = A.load()
@@ -51,9 +60,9 @@ class Volume:
                 "mh_volume_create")
 
     def close(self):
-        if self.h:
+        if self.h and self.owned:
             self.lib.mh_volume_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -97,6 +106,23 @@ class Volume:
 
     def integrate_tsdf(self):
         A.check(self.lib.mh_integrate_tsdf(self.h), "mh_integrate_tsdf")
+
+    def tune_integrate(self, reps: int = 3) -> A.TuneResult:
+        r = A.TuneResult()
+        A.check(self.lib.mh_tune_integrate(self.h, reps, C.byref(r)), "mh_tune_integrate")
+        return r
+
+    def set_integrate_tiling(self, tiling: int, workgroups_per_cu: int = 8):
+        A.check(self.lib.mh_set_integrate_tiling(self.h, tiling, workgroups_per_cu),
+                "mh_set_integrate_tiling")
+
+    def pose(self) -> tuple[np.ndarray, np.ndarray]:
+        """(wTc, cTw) of the current frame, FP32, as the kernels use them."""
+        w = np.zeros(16, np.float32)
+        c = np.zeros(16, np.float32)
+        A.check(self.lib.mh_frame_get_pose(self.h, A.ptr(w, C.c_float), A.ptr(c, C.c_float)),
+                "mh_frame_get_pose")
+        return w.reshape(4, 4), c.reshape(4, 4)
 
     def predict_inlier(self):
         A.check(self.lib.mh_predict_inlier(self.h), "mh_predict_inlier")
@@ -258,3 +284,77 @@ class MainEngine:
         self.mapping()
         if mesh:
             self.meshing()
+
+
+class Batch:
+    """Independent scenes of one image size, every stage one launch over all
+    scenes (mh_batch_*).  volumes[i] views scene i for status and dumps."""
+
+    def __init__(self, cfgs: list[Config]):
+        self.cfgs = list(cfgs)
+        self.lib = A.load()
+        n = len(self.cfgs)
+        arr = lambda T, xs: (T * n)(*xs)
+        self._mc = arr(A.McParams, [c.mc_params() for c in self.cfgs])
+        self.h = C.c_void_p()
+        A.check(self.lib.mh_batch_create(
+            n, arr(A.HashParams, [c.hash_params() for c in self.cfgs]),
+            arr(A.VolumeParams, [c.volume_params() for c in self.cfgs]),
+            arr(A.MeshParams, [c.mesh_params() for c in self.cfgs]),
+            arr(A.SensorParams, [c.sensor_params() for c in self.cfgs]), C.byref(self.h)),
+            "mh_batch_create")
+        self.volumes = []
+        for i, c in enumerate(self.cfgs):
+            vh = C.c_void_p()
+            A.check(self.lib.mh_batch_volume(self.h, i, C.byref(vh)), "mh_batch_volume")
+            self.volumes.append(Volume.wrap(c, vh))
+
+    def close(self):
+        if self.h:
+            self.lib.mh_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_ptr: int):
+        A.check(self.lib.mh_batch_set_stream(self.h, C.c_void_p(stream_ptr)), "mh_batch_set_stream")
+
+    def frame(self, depth, wTc: np.ndarray):
+        """depth: (n, H, W) uint16 host array, or an int device pointer to
+        the same layout (kept valid until the run completes)."""
+        w = np.ascontiguousarray(np.asarray(wTc, np.float32).reshape(-1, 16))
+        if isinstance(depth, int):
+            rc = self.lib.mh_batch_frame_u16(self.h, C.c_void_p(depth), 1, A.ptr(w, C.c_float))
+        else:
+            d = np.ascontiguousarray(depth, dtype=np.uint16)
+            rc = self.lib.mh_batch_frame_u16(self.h, d.ctypes.data_as(C.c_void_p), 0,
+                                             A.ptr(w, C.c_float))
+        A.check(rc, "mh_batch_frame_u16")
+
+    def set_raycast(self, depth_ptr: int = 0, vertex_ptr: int = 0, normal_ptr: int = 0):
+        n = len(self.cfgs)
+        rp = (A.RayCasterParams * n)(*[c.raycaster_params() for c in self.cfgs])
+        A.check(self.lib.mh_batch_set_raycast(self.h, rp, C.c_void_p(depth_ptr or None),
+                                              C.c_void_p(vertex_ptr or None),
+                                              C.c_void_p(normal_ptr or None)),
+                "mh_batch_set_raycast")
+
+    def set_integrate_tiling(self, tiling: int, workgroups_per_cu: int = 8):
+        A.check(self.lib.mh_batch_set_integrate_tiling(self.h, tiling, workgroups_per_cu),
+                "mh_batch_set_integrate_tiling")
+
+    def run(self, mesh: bool = True, raycast: bool = False, graph: bool = False,
+            tune: bool = False) -> A.TuneResult | None:
+        flags = ((A.MH_RUN_MESH if mesh else 0) | (A.MH_RUN_RAYCAST if raycast else 0) |
+                 (A.MH_RUN_GRAPH if graph else 0) | (A.MH_RUN_TUNE if tune else 0))
+        r = A.TuneResult()
+        A.check(self.lib.mh_batch_run(self.h, self._mc, flags, C.byref(r) if tune else None),
+                "mh_batch_run")
+        return r if tune else None
+
+    def synchronize(self):
+        A.check(self.lib.mh_batch_synchronize(self.h), "mh_batch_synchronize")

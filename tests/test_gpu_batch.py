@@ -1,0 +1,198 @@
+"""Batched scenes (mh_batch_*) and integration tilings (mh_tune_integrate,
+mh_set_integrate_tiling) through the C ABI.
+
+Bar: bit-identical per-scene results.  A batch runs every stage as one
+launch over all scenes, eagerly or as a replayed HIP graph; each scene must
+end exactly where running it alone through the single-volume entry points
+ends (canonical block dump with SDF / weight / colour bits, canonical hash
+layout, worklist, cube indices, mesh, status counters, raycast images of
+every frame).  Integration tilings and the autotuner must not change a bit.
+"""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from meshhashing_amd import _capi as A
+from meshhashing_amd.configs import C1, c4_scenes
+from meshhashing_amd.engine import Batch, Volume, synth_depth, synth_pose
+
+pytestmark = pytest.mark.gpu
+
+# counters that depend only on the data (vertex writes also count the
+# claimer of a fresh slot, whose identity is a race)
+STATUS_KEYS = ("flags", "dropped_inserts", "dda_cap_hits", "list_inserts", "block_count",
+               "candidate_count", "vertex_count", "triangle_count", "voxel_updates",
+               "blocks_integrated", "mc_triangle_writes", "mc_blocks", "ray_samples")
+
+
+def canon_hash(rec):
+    out = {}
+    for r in rec:
+        out.setdefault(int(r[1]), []).append((tuple(r[3:]), int(r[2])))
+    return {k: (sorted(p for p, _ in v), sum(f for _, f in v)) for k, v in out.items()}
+
+
+def scene_state(vol, mesh=True):
+    b = vol.dump_blocks()
+    st = vol.status()
+    out = {"pos": b["pos"], "sdf": b["sdf"].view(np.uint32), "weight": b["weight"].view(np.uint32),
+           "color": b["color"], "cand": vol.dump_candidates(), "hash": canon_hash(vol.dump_hash()),
+           "status": {k: st[k] for k in STATUS_KEYS}}
+    if mesh:
+        m = vol.dump_mesh()
+        out["cube"] = vol.dump_cube_indices()
+        out["tri"] = m["tri"]
+        out["vkey"] = m["vkey"]
+        out["vval"] = m["vval"].view(np.uint32)
+        out["vref"] = m["vref"]
+    return out
+
+
+def assert_same(a, b, what):
+    assert a.keys() == b.keys()
+    for k in a:
+        if isinstance(a[k], np.ndarray):
+            assert a[k].shape == b[k].shape and np.array_equal(a[k], b[k]), f"{what}: {k} differs"
+        else:
+            assert a[k] == b[k], f"{what}: {k} differs: {a[k]} vs {b[k]}"
+
+
+def sequential(cfg, frames, hip_device, raycast=True):
+    """One scene alone through the single-volume entry points."""
+    import torch
+    vol = Volume(cfg)
+    n = cfg.width * cfg.height
+    images = []
+    for f in frames:
+        depth, pose = synth_depth(cfg, f), synth_pose(cfg, f)
+        vol.upload_depth(depth, pose)
+        vol.alloc()
+        vol.collect_visible()
+        vol.integrate_tsdf()
+        vol.marching_cubes()
+        if raycast:
+            d = torch.empty(n, dtype=torch.float32, device=hip_device)
+            nn = torch.empty(3 * n, dtype=torch.float32, device=hip_device)
+            vol.raycast(vol.pose()[1], d.data_ptr(), 0, nn.data_ptr())
+            vol.synchronize()
+            images.append((d.cpu().numpy().view(np.uint32), nn.cpu().numpy().view(np.uint32)))
+    return vol, images
+
+
+def batched(cfgs, frames, hip_device, graph=False, tune_first=False):
+    import torch
+    b = Batch(cfgs)
+    W, H, S = cfgs[0].width, cfgs[0].height, len(cfgs)
+    d = torch.empty(S * W * H, dtype=torch.float32, device=hip_device)
+    nn = torch.empty(S * W * H * 3, dtype=torch.float32, device=hip_device)
+    b.set_raycast(d.data_ptr(), 0, nn.data_ptr())
+    images = []
+    tune = None
+    for i, f in enumerate(frames):
+        depth = np.stack([synth_depth(c, f) for c in cfgs])
+        pose = np.stack([synth_pose(c, f) for c in cfgs])
+        b.frame(depth, pose)
+        d.fill_(-1.0)  # eager torch work between graph replays
+        r = b.run(mesh=True, raycast=True, graph=graph and not (tune_first and i == 0),
+                  tune=tune_first and i == 0)
+        tune = tune or r
+        b.synchronize()
+        dn, nn_ = d.cpu().numpy().view(np.uint32), nn.cpu().numpy().view(np.uint32)
+        images.append([(dn[s * W * H:(s + 1) * W * H], nn_[3 * s * W * H:3 * (s + 1) * W * H])
+                       for s in range(S)])
+    return b, images, tune
+
+
+def test_batch_matches_sequential(hip_device):
+    cfgs = c4_scenes(n=8, frames=3)
+    sizes = {c.voxel_size for c in cfgs}
+    assert len(sizes) >= 3, sizes
+    frames = range(3)
+    b, bimg, _ = batched(cfgs, frames, hip_device)
+    for s, cfg in enumerate(cfgs):
+        print(f"[batch] scene {s} voxel {cfg.voxel_size}", flush=True)
+        vol, simg = sequential(cfg, frames, hip_device)
+        assert_same(scene_state(b.volumes[s]), scene_state(vol), f"scene {s}")
+        for f in frames:
+            assert np.array_equal(bimg[f][s][0], simg[f][0]), f"scene {s} frame {f}: depth"
+            assert np.array_equal(bimg[f][s][1], simg[f][1]), f"scene {s} frame {f}: normal"
+        st = b.volumes[s].status()
+        assert st["flags"] == 0 and st["block_count"] > 50, st
+        vol.close()
+    b.close()
+
+
+def test_batch_graph_replays_match_eager(hip_device):
+    """The captured frame graph, replayed on every frame (new depth, poses and
+    MC stamps through the device table) with eager work in between, equals
+    the eager batch after every frame."""
+    cfgs = c4_scenes(n=6, frames=5, seed=9)
+    frames = range(5)
+    be, eimg, _ = batched(cfgs, frames, hip_device, graph=False)
+    bg, gimg, _ = batched(cfgs, frames, hip_device, graph=True)
+    for f in frames:
+        for s in range(len(cfgs)):
+            assert np.array_equal(eimg[f][s][0], gimg[f][s][0]), f"frame {f} scene {s}: depth"
+            assert np.array_equal(eimg[f][s][1], gimg[f][s][1]), f"frame {f} scene {s}: normal"
+    for s in range(len(cfgs)):
+        assert_same(scene_state(bg.volumes[s]), scene_state(be.volumes[s]), f"scene {s}")
+    be.close()
+    bg.close()
+
+
+def test_batch_tuning_changes_nothing(hip_device):
+    cfgs = c4_scenes(n=6, frames=3, seed=5)
+    frames = range(3)
+    b0, img0, _ = batched(cfgs, frames, hip_device)
+    b1, img1, tune = batched(cfgs, frames, hip_device, graph=True, tune_first=True)
+    print(f"[tune] batch: tiling {tune.tiling} ({tune.workgroup_size} lanes, "
+          f"{tune.blocks_per_workgroup} blocks/wg, {tune.workgroups_per_cu} wg/CU) "
+          f"{tune.best_ms:.4f} ms vs default {tune.default_ms:.4f} ms")
+    assert tune.candidates == 50 and 0 <= tune.tiling < A.MH_NUM_TILINGS
+    assert tune.best_ms <= tune.default_ms
+    for f in frames:
+        for s in range(len(cfgs)):
+            assert np.array_equal(img0[f][s][0], img1[f][s][0])
+    for s in range(len(cfgs)):
+        assert_same(scene_state(b1.volumes[s]), scene_state(b0.volumes[s]), f"scene {s}")
+    b0.close()
+    b1.close()
+
+
+def run_tsdf(cfg, frames, tiling=None, tune_at=None):
+    vol = Volume(cfg)
+    if tiling is not None:
+        vol.set_integrate_tiling(*tiling)
+    res = None
+    for f in frames:
+        vol.upload_depth(synth_depth(cfg, f), synth_pose(cfg, f))
+        vol.alloc()
+        vol.collect_visible()
+        if f == tune_at:
+            res = vol.tune_integrate(reps=2)
+        vol.integrate_tsdf()
+    return vol, res
+
+
+def test_integrate_tilings_are_output_invariant(hip_device):
+    cfg = replace(C1, frames=6)
+    frames = range(0, 6, 2)
+    ref, _ = run_tsdf(cfg, frames)
+    want = scene_state(ref, mesh=False)
+    for t in range(A.MH_NUM_TILINGS):
+        for g in (2, 32):
+            print(f"[tiling] {t} x {g}", flush=True)
+            vol, _ = run_tsdf(cfg, frames, tiling=(t, g))
+            assert_same(scene_state(vol, mesh=False), want, f"tiling {t} grid {g}")
+            vol.close()
+
+
+def test_tune_integrate_changes_nothing(hip_device):
+    cfg = replace(C1, frames=6)
+    frames = range(0, 6, 2)
+    ref, _ = run_tsdf(cfg, frames)
+    vol, res = run_tsdf(cfg, frames, tune_at=2)
+    print(f"[tune] volume: tiling {res.tiling} {res.best_ms:.4f} ms vs default {res.default_ms:.4f} ms")
+    assert res.candidates == 50
+    assert_same(scene_state(vol, mesh=False), scene_state(ref, mesh=False), "tuned")
