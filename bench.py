@@ -8,10 +8,14 @@ Workload: the full-rate plain pipeline (640x480 uint16 depth, 5 mm voxels,
 All input frames are resident in HBM before the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--no-batched]
 
-Prints one JSON line.  `value` is voxel-updates/s of the whole chain;
-`ms_per_step` is ms/frame.  Per-stage times come from HIP events recorded
-on the volume's stream inside the timed region.
+Prints one JSON line.  `value` is voxel-updates/s of the whole chain and
+`ms_per_step` ms/frame, with each frame's chain replayed as one HIP graph.
+The same frames run eagerly, stage by stage, give per-stage times (HIP
+events on the volume's stream) and the roofline.  `batched` holds the
+batched form: 64 independent 320x240 scenes with voxel sizes 1-4 cm, per-
+scene throughput of the fused batch against one-scene-at-a-time runs.
 """
 from __future__ import annotations
 
@@ -69,13 +73,114 @@ def cpu_baseline(cfg, frames, budget_s=25.0):
             "ms_per_frame": 1e3 * dt / done}
 
 
+def c4_batched(dev, stream, n_scenes=64, n_frames=4):
+    """Batched form: n_scenes independent rooms (voxel sizes 1-4 cm), one frame
+    of all scenes per step, fused per stage and replayed as a HIP graph,
+    against the same scene-frames run one scene after another through the
+    single-volume entry points.  Both timed after a full warm-up pass and a
+    reset of every volume."""
+    import torch
+
+    from meshhashing_amd.configs import c4_scenes
+    from meshhashing_amd.engine import Batch, Volume, synth_depth, synth_pose
+
+    cfgs = c4_scenes(n_scenes, n_frames)
+    W, H = cfgs[0].width, cfgs[0].height
+    depth = np.stack([np.stack([synth_depth(c, f) for c in cfgs]) for f in range(n_frames)])
+    poses = np.stack([np.stack([synth_pose(c, f) for c in cfgs]) for f in range(n_frames)])
+    depth_dev = torch.from_numpy(depth.view(np.int16)).to(dev)
+    fbytes = n_scenes * W * H * 2
+    rd = torch.empty(n_scenes * W * H, dtype=torch.float32, device=dev)
+    rn = torch.empty(3 * n_scenes * W * H, dtype=torch.float32, device=dev)
+
+    def timed(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b)
+
+    # batched
+    bt = Batch(cfgs)
+    bt.set_stream(stream.cuda_stream)
+    bt.set_raycast(rd.data_ptr(), 0, rn.data_ptr())
+    tune = None
+
+    def batch_pass(graph, tune_first=False):
+        nonlocal tune
+        for f in range(n_frames):
+            bt.frame(depth_dev.data_ptr() + f * fbytes, poses[f])
+            r = bt.run(mesh=True, raycast=True, graph=graph and not (tune_first and f == 0),
+                       tune=tune_first and f == 0)
+            tune = r or tune
+
+    batch_pass(True, tune_first=True)  # tunes on frame 0, captures the graph on frame 1
+    torch.cuda.synchronize()
+    for v in bt.volumes:
+        v.reset()
+    c0 = [v.status() for v in bt.volumes]
+    ms_b = timed(lambda: batch_pass(True))
+    st_b = [v.status() for v in bt.volumes]
+    upd_b = sum(b["voxel_updates"] - a["voxel_updates"] for a, b in zip(c0, st_b))
+    blocks = [s["block_count"] for s in st_b]
+    bt.close()
+
+    # sequential single-volume runs of the same scene-frames
+    vols = [Volume(c) for c in cfgs]
+    for v in vols:
+        v.set_stream(stream.cuda_stream)
+    sbytes = W * H * 2
+
+    def seq_pass():
+        for s, v in enumerate(vols):
+            for f in range(n_frames):
+                v.set_depth_device(depth_dev.data_ptr() + f * fbytes + s * sbytes, poses[f][s])
+                v.alloc()
+                v.collect_visible()
+                v.integrate_tsdf()
+                v.marching_cubes()
+                v.raycast(v.pose()[1], rd.data_ptr(), 0, rn.data_ptr())
+
+    seq_pass()
+    torch.cuda.synchronize()
+    for v in vols:
+        v.reset()
+    ms_s = timed(seq_pass)
+    for v in vols:
+        v.close()
+    sf = n_scenes * n_frames
+    vs = [c.voxel_size for c in cfgs]
+    return {
+        "workload": f"{n_scenes} scenes x {n_frames} frames of 320x240, voxel sizes "
+                    f"{ {round(x * 100): vs.count(x) for x in sorted(set(vs))} } cm, "
+                    "alloc->collect->TSDF->marching cubes->raycast per frame",
+        "blocks_per_scene": [min(blocks), max(blocks)],
+        "batched_ms_per_step": ms_b / n_frames,
+        "scene_frames_per_s_batched": sf / (ms_b * 1e-3),
+        "scene_frames_per_s_sequential": sf / (ms_s * 1e-3),
+        "speedup_per_scene": ms_s / ms_b,
+        "voxel_updates_per_s_batched": upd_b / (ms_b * 1e-3),
+        "mode": "per-frame chain replayed as one HIP graph, integration tiling autotuned "
+                "(sequential runs: default tiling, eager launches)",
+        "tuned_tiling": None if tune is None else {
+            "workgroup_size": tune.workgroup_size, "blocks_per_workgroup": tune.blocks_per_workgroup,
+            "workgroups_per_cu": tune.workgroups_per_cu},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=280)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--stage-events", type=int, default=1)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run 5 plain 256 MiB device copies (reference dispatches "
+                         "for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not "
@@ -86,7 +191,7 @@ def main():
     import torch
 
     from meshhashing_amd.configs import C2
-    from meshhashing_amd.engine import Volume, synth_depth, synth_pose
+    from meshhashing_amd.engine import Batch, Volume, synth_depth, synth_pose
 
     cfg = C2
     total = args.warmup + args.steps
@@ -94,24 +199,35 @@ def main():
     t_gen = time.perf_counter()
     frames = np.stack([synth_depth(cfg, f % cfg.frames) for f in range(total)])
     poses = [synth_pose(cfg, f % cfg.frames) for f in range(total)]
-    cTws = [np.linalg.inv(p.astype(np.float64)).astype(np.float32) for p in poses]
     t_gen = time.perf_counter() - t_gen
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    stream = torch.cuda.Stream(device=dev)  # default priority
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     depth_dev = torch.from_numpy(frames.view(np.int16)).to(dev)  # resident in HBM
     out_depth = torch.empty(W * H, dtype=torch.float32, device=dev)
     out_normal = torch.empty(3 * W * H, dtype=torch.float32, device=dev)
-    vol = Volume(cfg)
-    stream = torch.cuda.current_stream()
-    vol.set_stream(stream.cuda_stream)
     frame_bytes = W * H * 2
     base_ptr = depth_dev.data_ptr()
+    K = args.steps
+    tune_frame = args.warmup // 2
 
+    # ---- 1. eager stage-by-stage run (per-stage HIP events, autotuned integration)
+    vol = Volume(cfg)
+    vol.set_stream(stream.cuda_stream)
     stages = ["alloc", "collect", "integrate", "mesh", "raycast"]
     ev = {s: [] for s in stages}
+    tune = None
 
     def run_frame(f, record):
+        nonlocal tune
         e = [torch.cuda.Event(enable_timing=True) for _ in range(6)] if record else None
         if record:
             e[0].record(stream)
@@ -120,6 +236,8 @@ def main():
         if record:
             e[1].record(stream)
         vol.collect_visible()
+        if f == tune_frame:
+            tune = vol.tune_integrate(reps=3)
         if record:
             e[2].record(stream)
         vol.integrate_tsdf()
@@ -128,7 +246,7 @@ def main():
         vol.marching_cubes()
         if record:
             e[4].record(stream)
-        vol.raycast(cTws[f], out_depth.data_ptr(), 0, out_normal.data_ptr())
+        vol.raycast(vol.pose()[1], out_depth.data_ptr(), 0, out_normal.data_ptr())
         if record:
             e[5].record(stream)
             for i, s in enumerate(stages):
@@ -140,22 +258,18 @@ def main():
     c0 = vol.status()
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    wall0 = time.perf_counter()
     t_start.record(stream)
     for f in range(args.warmup, total):
         run_frame(f, bool(args.stage_events))
     t_end.record(stream)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    ms_total = t_start.elapsed_time(t_end)
+    ms_eager = t_start.elapsed_time(t_end)
     c1 = vol.status()
     cnt = {k: c1[k] - c0[k] for k in ("voxel_updates", "blocks_integrated", "mc_vertex_writes",
                                       "mc_triangle_writes", "mc_blocks", "ray_samples")}
     stage_ms = {s: sum(a.elapsed_time(b) for a, b in ev[s]) for s in stages} if args.stage_events else {}
-    K = args.steps
-    ms_per_frame = ms_total / K
-    value = cnt["voxel_updates"] / (ms_total * 1e-3)
+    st = vol.status()
+    vol.close()
 
     roof = None
     if stage_ms:
@@ -171,26 +285,57 @@ def main():
                 "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None,
                 "stages": rows}
 
-    st = vol.status()
+    # ---- 2. the same frames as one HIP graph per frame (a one-scene batch)
+    bt = Batch([cfg])
+    bt.set_stream(stream.cuda_stream)
+    bt.set_raycast(out_depth.data_ptr(), 0, out_normal.data_ptr())
+    gtune = None
+    for f in range(args.warmup):
+        bt.frame(base_ptr + f * frame_bytes, poses[f][None])
+        r = bt.run(mesh=True, raycast=True, graph=f != tune_frame, tune=f == tune_frame)
+        gtune = r or gtune
+    torch.cuda.synchronize()
+    gv = bt.volumes[0]
+    g0 = gv.status()
+    t_start.record(stream)
+    wall0 = time.perf_counter()
+    for f in range(args.warmup, total):
+        bt.frame(base_ptr + f * frame_bytes, poses[f][None])
+        bt.run(mesh=True, raycast=True, graph=True)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_graph = t_start.elapsed_time(t_end)
+    g1 = gv.status()
+    gupd = g1["voxel_updates"] - g0["voxel_updates"]
+    bt.close()
+
     line = {
         "metric": "ms/frame & voxel-updates/s (640x480, 5 mm) plain vs batched; HBM GB/s; grad err",
-        "value": value,
+        "value": gupd / (ms_graph * 1e-3),
         "unit": "voxel-updates/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_frame,
+        "ms_per_step": ms_graph / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "c2 full-rate plain pipeline: 640x480 depth, 5 mm voxels, trunc 2 cm, "
-                               "2^20x10 hash, alloc->collect->TSDF->marching cubes->raycast 640x480",
+                               "2^20x10 hash, alloc->collect->TSDF->marching cubes->raycast 640x480, "
+                               "one HIP graph per frame",
                    "frames": total, "voxel_size": cfg.voxel_size, "truncation": cfg.truncation,
                    "bucket_count": cfg.bucket_count, "parallelism": "single GPU"},
         "roofline": roof,
-        "stage_ms_per_frame": {s: v / K for s, v in stage_ms.items()},
+        "eager": {"ms_per_step": ms_eager / K, "voxel_updates_per_s": cnt["voxel_updates"] / (ms_eager * 1e-3),
+                  "stage_ms_per_frame": {s: v / K for s, v in stage_ms.items()}},
+        "integrate_tiling": None if tune is None else {
+            "workgroup_size": tune.workgroup_size, "blocks_per_workgroup": tune.blocks_per_workgroup,
+            "workgroups_per_cu": tune.workgroups_per_cu, "tuned_ms": tune.best_ms,
+            "default_ms": tune.default_ms},
+        "graph_matches_eager_updates": gupd == cnt["voxel_updates"],
         "counters_per_frame": {k: v / K for k, v in cnt.items()},
         "blocks_allocated": st["block_count"],
         "triangles": st["triangle_count"],
@@ -198,7 +343,8 @@ def main():
         "host_wall_ms_per_frame": 1e3 * wall / K,
         "input_generation_s": t_gen,
     }
-    vol.close()
+    if not args.no_batched:
+        line["batched"] = c4_batched(dev, stream)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, frames=min(total, 3))
     print(json.dumps(line))

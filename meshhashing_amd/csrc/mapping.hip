@@ -84,7 +84,8 @@ __global__ void convert_depth_kernel(const uint16_t* __restrict__ raw, float* __
 }
 
 // The same for every scene of a batch (scene = blockIdx.y).
-__global__ void batch_convert_depth_kernel(Many src, uint32_t n) {
+__global__ void batch_convert_depth_kernel(Many src_, const SceneFrame* __restrict__ tab, uint32_t n) {
+  const Many src = with_table(src_, tab);
   const View& V = src.view(blockIdx.y);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -155,7 +156,8 @@ __device__ void insert_block(const View& V, i3 p) {
 // depth - trunc and depth + trunc with a 3-D DDA (Amanatides-Woo in block
 // units) and insert every block whose centre lies in the frustum.
 template <class Src>
-__global__ __launch_bounds__(256) void alloc_kernel(Src src) {
+__global__ __launch_bounds__(256) void alloc_kernel(Src src_, const SceneFrame* __restrict__ tab) {
+  const Src src = with_table(src_, tab);
   const View& V = src.view(blockIdx.z);
   const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15);
   const uint32_t y = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -224,7 +226,8 @@ __global__ __launch_bounds__(256) void alloc_kernel(Src src) {
 // Also clears the worklist counter for the collect pass that follows.
 // One workgroup per scene (blockIdx.x), one working lane.
 template <class Src>
-__global__ void list_insert_kernel(Src src) {
+__global__ void list_insert_kernel(Src src_, const SceneFrame* __restrict__ tab) {
+  const Src src = with_table(src_, tab);
   const View& V = src.view(blockIdx.x);
   if (threadIdx.x != 0) return;
   const uint32_t n = min(V.ctr->ovf_count, kOverflowQueue);
@@ -283,8 +286,9 @@ __global__ void list_insert_kernel(Src src) {
 // Scene = blockIdx.y.  A batch also appends (scene, block) items to the
 // concatenated worklist that the batched block kernels run over.
 template <class Src>
-__global__ __launch_bounds__(256) void collect_kernel(Src src, int frustum, uint32_t* work,
+__global__ __launch_bounds__(256) void collect_kernel(Src src_, const SceneFrame* __restrict__ tab, int frustum, uint32_t* work,
                                                       uint32_t* work_count) {
+  const Src src = with_table(src_, tab);
   const uint32_t scene = blockIdx.y;
   const View& V = src.view(scene);
   const uint32_t n = V.ctr->block_hwm;
@@ -369,7 +373,8 @@ __device__ __forceinline__ bool integrate_voxel(const View& V, const i3 b, int32
 // block of one scene).  Counters are flushed per wave whenever the scene
 // changes, so a batch's per-scene counters equal those of separate runs.
 template <class Src, int NT, int BPW>
-__global__ __launch_bounds__(NT) void integrate_tsdf_kernel(Src src) {
+__global__ __launch_bounds__(NT) void integrate_tsdf_kernel(Src src_, const SceneFrame* __restrict__ tab) {
+  const Src src = with_table(src_, tab);
   constexpr uint32_t kLanes = NT / BPW;
   static_assert(kLanes % 64 == 0 && kBlockVoxels % kLanes == 0, "tiling");
   const uint32_t n = src.count();
@@ -578,7 +583,7 @@ __global__ __launch_bounds__(256) void load_scatter_kernel(View V, const uint64_
 
 void launch_load_insert(const View& V, const uint64_t* keys, uint32_t n, hipStream_t s) {
   load_insert_kernel<<<(n + 255) / 256, 256, 0, s>>>(V, keys, n);
-  list_insert_kernel<<<1, 64, 0, s>>>(One{V});
+  list_insert_kernel<<<1, 64, 0, s>>>(One{V}, nullptr);
 }
 
 void launch_load_scatter(const View& V, const uint64_t* keys, uint32_t n, const float* sdf,
@@ -614,13 +619,13 @@ void launch_convert_depth(const View& V, const uint16_t* raw, hipStream_t s) {
 
 void launch_alloc(const View& V, hipStream_t s) {
   dim3 grid((V.cam.width + 15) / 16, (V.cam.height + 15) / 16);
-  alloc_kernel<<<grid, 256, 0, s>>>(One{V});
-  list_insert_kernel<<<1, 64, 0, s>>>(One{V});
+  alloc_kernel<<<grid, 256, 0, s>>>(One{V}, nullptr);
+  list_insert_kernel<<<1, 64, 0, s>>>(One{V}, nullptr);
 }
 
 void launch_collect(const View& V, int frustum, hipStream_t s, int grid) {
   set_u32_kernel<<<1, 64, 0, s>>>(&V.ctr->cand_count, 0u);
-  collect_kernel<<<grid, 256, 0, s>>>(One{V}, frustum, nullptr, nullptr);
+  collect_kernel<<<grid, 256, 0, s>>>(One{V}, nullptr, frustum, nullptr, nullptr);
 }
 
 namespace {
@@ -629,7 +634,7 @@ void integrate_dispatch(const Src& src, hipStream_t s, int grid, int tiling) {
   if constexpr (T < kNumIntegrateTilings) {
     constexpr int NT = kIntegrateTilings[T].nt, BPW = kIntegrateTilings[T].bpw;
     if (tiling == T)
-      integrate_tsdf_kernel<Src, NT, BPW><<<grid, NT, 0, s>>>(src);
+      integrate_tsdf_kernel<Src, NT, BPW><<<grid, NT, 0, s>>>(src, table_of(src));
     else
       integrate_dispatch<Src, T + 1>(src, s, grid, tiling);
   }
@@ -651,26 +656,27 @@ void launch_integrate_psdf(const View& V, hipStream_t s, int grid) {
 // ------------------------------------------------------------ batched launchers
 // Per-scene worklist counters, the concatenated worklist's count and the
 // overflow queues are cleared by kernels (a captured graph holds no memsets).
-__global__ void batch_reset_counts_kernel(Many src, uint32_t nscenes, uint32_t* work_count) {
+__global__ void batch_reset_counts_kernel(Many src_, const SceneFrame* __restrict__ tab, uint32_t nscenes, uint32_t* work_count) {
+  const Many src = with_table(src_, tab);
   for (uint32_t t = threadIdx.x; t < nscenes; t += blockDim.x) src.view(t).ctr->cand_count = 0u;
   if (threadIdx.x == 0) *work_count = 0u;
 }
 
 void launch_batch_convert_depth(const BatchDev& B, hipStream_t s) {
   const uint32_t n = B.width * B.height;
-  batch_convert_depth_kernel<<<dim3((n + 255) / 256, B.nscenes), 256, 0, s>>>(B.src(), n);
+  batch_convert_depth_kernel<<<dim3((n + 255) / 256, B.nscenes), 256, 0, s>>>(B.src(), B.frames, n);
 }
 
 void launch_batch_alloc(const BatchDev& B, hipStream_t s) {
   dim3 grid((B.width + 15) / 16, (B.height + 15) / 16, B.nscenes);
-  alloc_kernel<<<grid, 256, 0, s>>>(B.src());
-  list_insert_kernel<<<B.nscenes, 64, 0, s>>>(B.src());
+  alloc_kernel<<<grid, 256, 0, s>>>(B.src(), B.frames);
+  list_insert_kernel<<<B.nscenes, 64, 0, s>>>(B.src(), B.frames);
 }
 
 void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid) {
-  batch_reset_counts_kernel<<<1, 256, 0, s>>>(B.src(), B.nscenes, B.work_count);
+  batch_reset_counts_kernel<<<1, 256, 0, s>>>(B.src(), B.frames, B.nscenes, B.work_count);
   const uint32_t gx = (uint32_t)grid / B.nscenes > 0 ? (uint32_t)grid / B.nscenes : 1u;
-  collect_kernel<<<dim3(gx, B.nscenes), 256, 0, s>>>(B.src(), frustum, B.work, B.work_count);
+  collect_kernel<<<dim3(gx, B.nscenes), 256, 0, s>>>(B.src(), B.frames, frustum, B.work, B.work_count);
 }
 
 void launch_batch_integrate_tsdf(const BatchDev& B, hipStream_t s, int grid, int tiling) {

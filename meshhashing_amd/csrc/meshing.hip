@@ -299,7 +299,8 @@ struct McCounts {
 };
 
 template <bool PSDF, class Src>
-__global__ __launch_bounds__(512) void mc_surfel_kernel(Src src, McConst mc1) {
+__global__ __launch_bounds__(512) void mc_surfel_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1) {
+  const Src src = with_table(src_, tab);
   __shared__ Shell<PSDF> S;
   __shared__ int32_t nb[27];
   __shared__ uint8_t meshed[27];
@@ -410,11 +411,12 @@ __global__ __launch_bounds__(512) void mc_surfel_kernel(Src src, McConst mc1) {
 
 // Meshed-in-this-pass marks for the writer rule.
 template <class Src>
-__global__ void mc_mark_kernel(Src src, McConst mc1) {
+__global__ void mc_mark_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1) {
+  const Src src = with_table(src_, tab);
   const uint32_t n = src.count();
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     uint32_t scene;
-    const int32_t ptr = src.item(i, &scene);
+    const int32_t ptr = src.lane_item(i, &scene);
     src.view(scene).block_mark[ptr] = mc_of(src, scene, mc1).pass;
   }
 }
@@ -438,7 +440,8 @@ __global__ void mc_mark_kernel(Src src, McConst mc1) {
 // -x/-y/-z neighbours (voxels -1..7) are staged in LDS.
 constexpr int kCfg = 9;  // local cube coordinates -1..7
 template <class Src>
-__global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src) {
+__global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, const SceneFrame* __restrict__ tab) {
+  const Src src = with_table(src_, tab);
   __shared__ int32_t nb[8];  // blocks at offsets (dx, dy, dz) in {-1, 0}^3
   __shared__ uint8_t cfg[kCfg * kCfg * kCfg];
   __shared__ int counts[2];
@@ -504,22 +507,22 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src) {
 }
 
 void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int grid) {
-  mc_mark_kernel<<<grid / 4, 256, 0, s>>>(One{V}, p);
+  mc_mark_kernel<<<grid / 4, 256, 0, s>>>(One{V}, nullptr, p);
   if (V.va)
-    mc_surfel_kernel<true><<<grid, 512, 0, s>>>(One{V}, p);
+    mc_surfel_kernel<true><<<grid, 512, 0, s>>>(One{V}, nullptr, p);
   else
-    mc_surfel_kernel<false><<<grid, 512, 0, s>>>(One{V}, p);
-  mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(One{V});
+    mc_surfel_kernel<false><<<grid, 512, 0, s>>>(One{V}, nullptr, p);
+  mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(One{V}, nullptr);
 }
 
 void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int grid) {
   const McConst unused{};
-  mc_mark_kernel<<<grid / 4, 256, 0, s>>>(B.src(), unused);
+  mc_mark_kernel<<<grid / 4, 256, 0, s>>>(B.src(), B.frames, unused);
   if (B.psdf)
-    mc_surfel_kernel<true><<<grid, 512, 0, s>>>(B.src(), unused);
+    mc_surfel_kernel<true><<<grid, 512, 0, s>>>(B.src(), B.frames, unused);
   else
-    mc_surfel_kernel<false><<<grid, 512, 0, s>>>(B.src(), unused);
-  mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(B.src());
+    mc_surfel_kernel<false><<<grid, 512, 0, s>>>(B.src(), B.frames, unused);
+  mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(B.src(), B.frames);
 }
 
 }  // namespace mh

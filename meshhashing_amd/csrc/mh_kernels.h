@@ -43,6 +43,7 @@ constexpr uint32_t kMaxBatchCapacity = 1u << kSceneShift;
 // a device table of SceneFrames; pixel kernels take the scene from
 // blockIdx.z, worklist kernels from the concatenated (scene, block) list).
 // Items of one scene are processed by exactly the per-scene code.
+// item(ci) needs a wave-uniform ci; lane_item(ci) takes any ci.
 struct One {
   View V;
   __device__ __forceinline__ const View& view(uint32_t) const { return V; }
@@ -50,6 +51,9 @@ struct One {
   __device__ __forceinline__ int32_t item(uint32_t ci, uint32_t* scene) const {
     *scene = 0;
     return V.cand[ci];
+  }
+  __device__ __forceinline__ int32_t lane_item(uint32_t ci, uint32_t* scene) const {
+    return item(ci, scene);
   }
 };
 
@@ -60,11 +64,26 @@ struct Many {
   __device__ __forceinline__ const View& view(uint32_t s) const { return f[s].V; }
   __device__ __forceinline__ uint32_t count() const { return *work_count; }
   __device__ __forceinline__ int32_t item(uint32_t ci, uint32_t* scene) const {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(work[ci]);  // ci is wave-uniform
+    *scene = w >> kSceneShift;
+    return (int32_t)(w & (kMaxBatchCapacity - 1u));
+  }
+  __device__ __forceinline__ int32_t lane_item(uint32_t ci, uint32_t* scene) const {
     const uint32_t w = work[ci];
     *scene = w >> kSceneShift;
     return (int32_t)(w & (kMaxBatchCapacity - 1u));
   }
 };
+
+// Batched kernels also take the scene table as a __restrict__ kernel
+// argument: stores through the per-scene pointers then provably leave the
+// table unchanged, so per-scene constants load into scalar registers.
+__device__ __forceinline__ One with_table(const One& o, const SceneFrame*) { return o; }
+__device__ __forceinline__ Many with_table(const Many& m, const SceneFrame* t) {
+  return Many{t, m.work, m.work_count};
+}
+inline const SceneFrame* table_of(const One&) { return nullptr; }
+inline const SceneFrame* table_of(const Many& m) { return m.f; }
 
 // Device-side description of a batch, for the batched launchers.
 struct BatchDev {

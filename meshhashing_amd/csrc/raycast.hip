@@ -193,9 +193,10 @@ __device__ __forceinline__ const RayConst& ray_of(const Many& m, uint32_t s, con
 
 // Scene = blockIdx.z; a batch's outputs are scene-major (W*H per scene).
 template <class Src>
-__global__ __launch_bounds__(64) void raycast_kernel(Src src, RayConst R1, float* __restrict__ depth_out,
+__global__ __launch_bounds__(64) void raycast_kernel(Src src_, const SceneFrame* __restrict__ tab, RayConst R1, float* __restrict__ depth_out,
                                                      float* __restrict__ vertex_out,
                                                      float* __restrict__ normal_out) {
+  const Src src = with_table(src_, tab);
   const uint32_t scene = blockIdx.z;
   const View& V = src.view(scene);
   const RayConst& R = ray_of(src, scene, R1);
@@ -225,13 +226,13 @@ __global__ __launch_bounds__(64) void raycast_kernel(Src src, RayConst R1, float
 void launch_raycast(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,
                     hipStream_t s) {
   dim3 grid((p.cam.width + 7) / 8, (p.cam.height + 7) / 8);
-  raycast_kernel<<<grid, 64, 0, s>>>(One{V}, p, depth, vertex, normal);
+  raycast_kernel<<<grid, 64, 0, s>>>(One{V}, nullptr, p, depth, vertex, normal);
 }
 
 void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,
                           hipStream_t s) {
   dim3 grid((B.width + 7) / 8, (B.height + 7) / 8, B.nscenes);
-  raycast_kernel<<<grid, 64, 0, s>>>(B.src(), RayConst{}, depth, vertex, normal);
+  raycast_kernel<<<grid, 64, 0, s>>>(B.src(), B.frames, RayConst{}, depth, vertex, normal);
 }
 
 }  // namespace mh

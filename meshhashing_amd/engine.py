@@ -89,6 +89,9 @@ class Volume:
     def set_stream(self, stream_ptr: int):
         A.check(self.lib.mh_volume_set_stream(self.h, C.c_void_p(stream_ptr)), "set_stream")
 
+    def reset(self):
+        A.check(self.lib.mh_volume_reset(self.h), "mh_volume_reset")
+
     # -- stages --------------------------------------------------------------
     def alloc(self):
         A.check(self.lib.mh_alloc_blocks(self.h), "mh_alloc_blocks")
