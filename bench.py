@@ -44,6 +44,28 @@ def algorithmic_bytes(stage, c, frames, w, h):
     return None
 
 
+STAGE_KERNEL = {"raycast": "raycast_kernel<One>", "mesh": "mc_surfel_kernel<false, One>"}
+
+
+def measured_traffic(stage, tiling=None):
+    """HBM bytes per launch of the stage's dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the counters' reading
+    on a copy of known size); None if the summary does not cover it."""
+    path = ROOT / "profiles" / "c2_640x480_5mm_traffic.json"
+    if not path.exists():
+        return None
+    prof = json.loads(path.read_text())
+    name = STAGE_KERNEL.get(stage)
+    if stage == "integrate" and tiling:
+        name = f"integrate_tsdf_kernel<One, {tiling[0]}, {tiling[1]}>"
+    k = prof["kernels"].get(name) if name else None
+    if not k or k.get("hbm_bytes") is None:
+        return None
+    return {"bytes_per_launch": k["hbm_bytes"], "kernel": name,
+            "fetch_factor": prof["fetch_factor"], "write_factor": prof["write_factor"],
+            "source": "profiles/c2_640x480_5mm_kernels.md"}
+
+
 def cpu_baseline(cfg, frames, budget_s=25.0):
     """The FP64 oracle (1 thread) on the first frames of the same sequence,
     full chain; stops before a frame that would end past budget_s (at the
@@ -281,8 +303,12 @@ def main():
                        "frac": b / (stage_ms[s] * 1e-3) / HBM_PEAK}
         b = algorithmic_bytes(dom, cnt, K, W, H)
         ach = b / (stage_ms[dom] * 1e-3)
+        tr = measured_traffic(dom, None if tune is None else (tune.workgroup_size,
+                                                              tune.blocks_per_workgroup))
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None,
+                "unit": "GB/s", "frac": ach / HBM_PEAK,
+                "traffic": None if tr is None else tr["bytes_per_launch"],
+                "traffic_source": tr, "algorithmic_bytes_per_launch": b / K,
                 "stages": rows}
 
     # ---- 2. the same frames as one HIP graph per frame (a one-scene batch)
