@@ -265,6 +265,43 @@ typedef struct mh_tune_result {
 int mh_tune_integrate(mh_volume* v, uint32_t reps, mh_tune_result* out);
 int mh_set_integrate_tiling(mh_volume* v, int tiling, int workgroups_per_cu);
 
+/* ---- differentiable raycast (no counterpart in the reference) ------------
+ * Loss L = 1/2 sum over pixels of (depth - obs)^2, over the pixels where
+ * mh_raycast (same params and cTw) hits and obs_depth_dev (W*H metres,
+ * device) holds a finite positive value.  The backward pass recomputes the
+ * march and differentiates the discrete algorithm: the bracketing samples
+ * and the three regula-falsi steps, through the trilinear interpolant.
+ * Pose: xi = (omega, upsilon) perturbs the camera pose on the left,
+ * wTc <- exp(xi^) wTc, with wTc = inverse(cTw) (FP32).  grad_sdf_dev (may be
+ * NULL): capacity*512 floats laid out as the volume's SDF (pool index major),
+ * zeroed by the caller; dL/dsdf is added to it (atomically, so sums of many
+ * pixels agree with FP64 to float rounding).  Per-pixel values are FP32;
+ * the sums below are FP64 in a fixed order.                                */
+typedef struct mh_raycast_grad {
+  double loss;          /* L                                                 */
+  double grad[6];       /* dL/dxi                                            */
+  double hessian[36];   /* sum J^T J (Gauss-Newton), J = d depth / d xi      */
+  uint32_t pixels;      /* pixels in the sum                                 */
+  uint32_t outliers;    /* hit + observation, left out by a residual gate   */
+} mh_raycast_grad;
+int mh_raycast_backward(mh_volume* v, const mh_raycaster_params* p,
+                        const float cTw[16], const float* obs_depth_dev,
+                        float* grad_sdf_dev, mh_raycast_grad* out);
+/* Pose refinement: Levenberg-damped Gauss-Newton over xi, iterations steps
+ * from wTc_init, on a truncated quadratic: pixels with |depth - obs| above
+ * a gate cost gate^2 / 2 each (no gate for the first 40 % of the steps,
+ * then 0.10 m, then 0.05 m for the last 30 %); a step is kept only if it
+ * lowers the cost per pixel.  cost_trace (may be NULL) receives the cost
+ * per pixel before the first and after every step (iterations + 1 values,
+ * each at that step's gate).                                               */
+int mh_refine_pose(mh_volume* v, const mh_raycaster_params* p,
+                   const float wTc_init[16], const float* obs_depth_dev,
+                   uint32_t iterations, float wTc_out[16], double* cost_trace);
+/* A caller's per-voxel device field (capacity*512 floats, pool-index major,
+ * e.g. grad_sdf_dev) in mh_dump_blocks order.                             */
+int mh_dump_voxel_field(mh_volume* v, const float* field_dev, uint32_t capacity,
+                        float* out_host, uint32_t* count_host);
+
 /* ---- batched scenes (no counterpart in the reference) --------------------
  * Many independent scenes (uneven block counts and voxel sizes, one common
  * image size) fused per stage: every stage of a frame is one launch over

@@ -78,6 +78,11 @@ class Status(C.Structure):
                 ("mc_blocks", C.c_uint64), ("ray_samples", C.c_uint64)]
 
 
+class RaycastGrad(C.Structure):
+    _fields_ = [("loss", C.c_double), ("grad", C.c_double * 6), ("hessian", C.c_double * 36),
+                ("pixels", C.c_uint32), ("outliers", C.c_uint32)]
+
+
 class TuneResult(C.Structure):
     _fields_ = [("tiling", C.c_int32), ("workgroup_size", C.c_int32),
                 ("blocks_per_workgroup", C.c_int32), ("voxels_per_lane", C.c_int32),
@@ -145,6 +150,11 @@ SIGNATURES = {
     "mh_dump_cube_indices": (C.c_int, [VP, C.c_uint32, P_U16, P_U32]),
     "mh_dump_primal_dual": (C.c_int, [VP, C.c_uint32, P_F32, P_U8, P_U32]),
     "mh_frame_get_pose": (C.c_int, [VP, P_F32, P_F32]),
+    "mh_raycast_backward": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, VP, VP,
+                                      C.POINTER(RaycastGrad)]),
+    "mh_refine_pose": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, VP, C.c_uint32, P_F32,
+                                 P_F64]),
+    "mh_dump_voxel_field": (C.c_int, [VP, VP, C.c_uint32, P_F32, P_U32]),
     "mh_tune_integrate": (C.c_int, [VP, C.c_uint32, C.POINTER(TuneResult)]),
     "mh_set_integrate_tiling": (C.c_int, [VP, C.c_int, C.c_int]),
     "mh_batch_create": (C.c_int, [C.c_uint32, C.POINTER(HashParams), C.POINTER(VolumeParams),

@@ -93,6 +93,9 @@ struct mh_volume {
   double* pd_energy = nullptr;
   uint32_t pd_npartials = 0;
   uint32_t mc_pass = 0;
+  double* rgrad_partials = nullptr;     // raycast adjoint: per-workgroup sums
+  uint32_t rgrad_cap = 0;
+  double* rgrad_out = nullptr;
   int tiling = kDefaultIntegrateTiling;  // integration tiling (kIntegrateTilings)
   int grid_per_cu = 8;                   // integration workgroups per CU
   std::vector<void*> owned;
@@ -322,6 +325,8 @@ int mh_volume_destroy(mh_volume* v) {
   if (!v) return MH_OK;
   (void)hipStreamSynchronize(v->stream);
   for (void* p : v->owned) (void)hipFree(p);
+  if (v->rgrad_partials) (void)hipFree(v->rgrad_partials);
+  if (v->rgrad_out) (void)hipFree(v->rgrad_out);
   delete v;
   return MH_OK;
 }
@@ -1265,6 +1270,211 @@ int mh_batch_synchronize(mh_batch* b) {
   if (!b) return fail(MH_ERR_ARG, "null batch");
   HIP_TRY(hipStreamSynchronize(b->stream));
   return MH_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ raycast adjoint
+namespace {
+
+int raycast_grad(mh_volume* v, const mh_raycaster_params* p, const float cTw[16], const float* obs,
+                 float* grad_sdf, mh_raycast_grad* out, float max_residual = INFINITY) {
+  RayConst R;
+  const int rc = build_ray(v, p, cTw, &R);
+  if (rc) return rc;
+  const uint32_t n = raycast_grad_partials(R);
+  if (n > v->rgrad_cap) {
+    if (v->rgrad_partials) (void)hipFree(v->rgrad_partials);
+    v->rgrad_partials = nullptr;
+    v->rgrad_cap = 0;
+    HIP_TRY(dalloc(&v->rgrad_partials, (size_t)n * 30));
+    v->rgrad_cap = n;
+  }
+  if (!v->rgrad_out) HIP_TRY(dalloc(&v->rgrad_out, 30));
+  launch_raycast_backward(v->V, R, obs, grad_sdf, v->rgrad_partials, v->rgrad_out, max_residual,
+                          v->stream);
+  HIP_TRY(hipGetLastError());
+  double h[30];
+  HIP_TRY(hipMemcpyAsync(h, v->rgrad_out, sizeof(h), hipMemcpyDeviceToHost, v->stream));
+  HIP_TRY(hipStreamSynchronize(v->stream));
+  out->loss = h[0];
+  for (int k = 0; k < 6; ++k) out->grad[k] = h[1 + k];
+  int t = 7;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) {
+      out->hessian[6 * i + j] = h[t];
+      out->hessian[6 * j + i] = h[t];
+      ++t;
+    }
+  out->pixels = (uint32_t)h[28];
+  out->outliers = (uint32_t)h[29];
+  return MH_OK;
+}
+
+// exp of xi = (omega, upsilon) in se(3) as a row-major 4x4 (Rodrigues, and
+// V = I + (1 - cos)/th^2 W + (th - sin)/th^3 W^2 for the translation).
+void se3_exp(const double xi[6], double T[16]) {
+  const double wx = xi[0], wy = xi[1], wz = xi[2];
+  const double th2 = wx * wx + wy * wy + wz * wz, th = std::sqrt(th2);
+  double A, B, C;
+  if (th < 1e-6) {
+    A = 1.0 - th2 / 6.0;
+    B = 0.5 - th2 / 24.0;
+    C = 1.0 / 6.0 - th2 / 120.0;
+  } else {
+    A = std::sin(th) / th;
+    B = (1.0 - std::cos(th)) / th2;
+    C = (th - std::sin(th)) / (th2 * th);
+  }
+  const double W[9] = {0, -wz, wy, wz, 0, -wx, -wy, wx, 0};
+  double W2[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double a = 0;
+      for (int k = 0; k < 3; ++k) a += W[3 * i + k] * W[3 * k + j];
+      W2[3 * i + j] = a;
+    }
+  for (int i = 0; i < 3; ++i) {
+    double t = 0;
+    for (int j = 0; j < 3; ++j) {
+      const double I = i == j ? 1.0 : 0.0;
+      T[4 * i + j] = I + A * W[3 * i + j] + B * W2[3 * i + j];
+      t += (I + B * W[3 * i + j] + C * W2[3 * i + j]) * xi[3 + j];
+    }
+    T[4 * i + 3] = t;
+  }
+  T[12] = T[13] = T[14] = 0.0;
+  T[15] = 1.0;
+}
+
+void matmul4(const double* a, const double* b, double* c) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      double s = 0;
+      for (int k = 0; k < 4; ++k) s += a[4 * i + k] * b[4 * k + j];
+      c[4 * i + j] = s;
+    }
+}
+
+// Solves (H + lambda diag(H)) x = -g by Gaussian elimination with partial
+// pivoting; false if singular.
+bool solve6(const double* H, const double* g, double lambda, double* x) {
+  double A[6][7];
+  for (int i = 0; i < 6; ++i) {
+    for (int j = 0; j < 6; ++j) A[i][j] = H[6 * i + j] + (i == j ? lambda * H[6 * i + j] : 0.0);
+    A[i][6] = -g[i];
+  }
+  for (int c = 0; c < 6; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < 6; ++r)
+      if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+    if (!(std::fabs(A[piv][c]) > 0.0)) return false;
+    for (int j = 0; j < 7; ++j) std::swap(A[c][j], A[piv][j]);
+    for (int r = c + 1; r < 6; ++r) {
+      const double f = A[r][c] / A[c][c];
+      for (int j = c; j < 7; ++j) A[r][j] -= f * A[c][j];
+    }
+  }
+  for (int i = 5; i >= 0; --i) {
+    double s = A[i][6];
+    for (int j = i + 1; j < 6; ++j) s -= A[i][j] * x[j];
+    x[i] = s / A[i][i];
+  }
+  return true;
+}
+
+int eval_pose(mh_volume* v, const mh_raycaster_params* p, const double T[16], const float* obs,
+              float gate, mh_raycast_grad* out) {
+  double inv[16];
+  invert4<double>(T, inv);
+  float cTw[16];
+  for (int i = 0; i < 16; ++i) cTw[i] = (float)inv[i];
+  return raycast_grad(v, p, cTw, obs, nullptr, out, gate);
+}
+
+// Residual gate of the pose refinement: pixels with |r| above it (occlusion
+// boundaries, surfaces the volume does not hold) leave the least-squares
+// problem and cost gate^2 / 2 each (a truncated quadratic), so leaving
+// pixels out is never free.  The first 40 % of the steps are ungated (the
+// start may be centimetres off), then 0.10 m for 30 %, then 0.05 m.
+float gate_at(uint32_t step, uint32_t steps) {
+  if (10 * step < 4 * steps) return INFINITY;
+  if (10 * step < 7 * steps) return 0.10f;
+  return 0.05f;
+}
+
+double truncated_cost(const mh_raycast_grad& g, float gate) {
+  const double n = (double)g.pixels + (double)g.outliers;
+  if (n == 0.0) return INFINITY;
+  const double out = g.outliers ? 0.5 * (double)gate * (double)gate * g.outliers : 0.0;
+  return (g.loss + out) / n;
+}
+
+
+}  // namespace
+
+extern "C" {
+
+int mh_raycast_backward(mh_volume* v, const mh_raycaster_params* p, const float cTw[16],
+                        const float* obs_depth_dev, float* grad_sdf_dev, mh_raycast_grad* out) {
+  if (check_vol(v) || !p || !cTw || !obs_depth_dev || !out) return fail(MH_ERR_ARG, "null argument");
+  return raycast_grad(v, p, cTw, obs_depth_dev, grad_sdf_dev, out);
+}
+
+int mh_refine_pose(mh_volume* v, const mh_raycaster_params* p, const float wTc_init[16],
+                   const float* obs_depth_dev, uint32_t iterations, float wTc_out[16],
+                   double* cost_trace) {
+  if (check_vol(v) || !p || !wTc_init || !obs_depth_dev || !wTc_out)
+    return fail(MH_ERR_ARG, "null argument");
+  double T[16];
+  for (int i = 0; i < 16; ++i) T[i] = wTc_init[i];
+  float gate = gate_at(0, iterations);
+  mh_raycast_grad cur;
+  int rc = eval_pose(v, p, T, obs_depth_dev, gate, &cur);
+  if (rc) return rc;
+  if (cost_trace) cost_trace[0] = truncated_cost(cur, gate);
+  double lambda = 1e-4;
+  for (uint32_t it = 0; it < iterations; ++it) {
+    double xi[6];
+    bool improved = false;
+    if (cur.pixels >= 6 && solve6(cur.hessian, cur.grad, lambda, xi)) {
+      double E[16], Tn[16];
+      se3_exp(xi, E);
+      matmul4(E, T, Tn);
+      mh_raycast_grad nxt;
+      rc = eval_pose(v, p, Tn, obs_depth_dev, gate, &nxt);
+      if (rc) return rc;
+      if (truncated_cost(nxt, gate) < truncated_cost(cur, gate)) {
+        std::memcpy(T, Tn, sizeof(T));
+        cur = nxt;
+        lambda = std::max(lambda * 0.1, 1e-9);
+        improved = true;
+      }
+    }
+    if (!improved) lambda *= 10.0;
+    if (cost_trace) cost_trace[it + 1] = truncated_cost(cur, gate);
+    const float g2 = gate_at(it + 1, iterations);
+    if (g2 != gate && it + 1 < iterations) {  // re-linearise at the new gate
+      gate = g2;
+      rc = eval_pose(v, p, T, obs_depth_dev, gate, &cur);
+      if (rc) return rc;
+    }
+  }
+  for (int i = 0; i < 16; ++i) wTc_out[i] = (float)T[i];
+  return MH_OK;
+}
+
+int mh_dump_voxel_field(mh_volume* v, const float* field_dev, uint32_t capacity, float* out_host,
+                        uint32_t* count_host) {
+  if (check_vol(v) || !field_dev) return fail(MH_ERR_ARG, "null argument");
+  HIP_TRY(hipStreamSynchronize(v->stream));
+  std::vector<BlockRec> blocks;
+  const int rc = sorted_blocks(v, blocks);
+  if (rc) return rc;
+  if (count_host) *count_host = (uint32_t)blocks.size();
+  if (!out_host) return MH_OK;
+  if (capacity < blocks.size()) return fail(MH_ERR_CAPACITY, "buffer too small");
+  return gather_field(v, field_dev, blocks, read_counters(v).block_hwm, 1, out_host);
 }
 
 }  // extern "C"

@@ -139,6 +139,10 @@ void launch_raycast(const View& V, const RayConst& p, float* depth, float* verte
                     hipStream_t s);
 void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,
                           hipStream_t s);
+// adjoint: 29 FP64 sums (loss, dL/dxi[6], J^T J upper triangle[21], pixels)
+uint32_t raycast_grad_partials(const RayConst& p);
+void launch_raycast_backward(const View& V, const RayConst& p, const float* obs, float* grad_sdf,
+                             double* partials, double* out, float max_residual, hipStream_t s);
 
 // primal_dual.hip
 void launch_pd_neighbors(const View& V, int32_t* nbr, hipStream_t s, int grid);

@@ -107,28 +107,25 @@ __device__ __forceinline__ bool ray_sdf(const View& V, BlockCache& cache, f3 pos
   return true;
 }
 
-// March one pixel; returns the number of valid samples taken.
-__device__ __forceinline__ uint32_t cast_pixel(const View& V, const RayConst& R, uint32_t x, uint32_t y,
-                               float* res_d, f3* res_v, f3* res_n) {
-  uint32_t nsamp = 0;
-  float out_d = -INFINITY;
-  f3 out_v = mk3(-INFINITY, -INFINITY, -INFINITY);
-  f3 out_n = mk3(-INFINITY, -INFINITY, -INFINITY);
+// The accepted crossing of one ray: the bracketing valid samples (t_p, s_p)
+// and (t_k, s_k), and the last regula-falsi point m.
+struct Hit {
+  float tp, sp, tk, sk, m;
+};
 
-  const f3 dir_c = normalize(reproject(R.cam, x, y, 1.0f));
-  const float t_min = R.min_depth / dir_c.z;
-  const float t_max = R.max_depth / dir_c.z;
-  const f3 o = xform(R.wTc, mk3(0.0f, 0.0f, 0.0f));
-  const f3 dir = normalize(xform_dir(R.wTc, dir_c));
-
-  BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr};
+// The march of CastKernel (ray_caster.cu:56-92): fixed FP32 steps, a
+// crossing between the previous valid sample and the current one, three
+// regula-falsi steps (isosurface_intersection.h:24-55), then the acceptance
+// test.  A failed refinement or test continues the march.
+__device__ __forceinline__ bool find_surface(const View& V, const RayConst& R, f3 o, f3 dir, float t_min,
+                                             float t_max, BlockCache& cache, Hit* hit, uint32_t* nsamp) {
   float prev_sdf = 0.0f, prev_t = 0.0f;
   bool prev_ok = false;
   for (float t = t_min; t < t_max; t += R.step) {
     const f3 p = mk3(o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z);
     float s;
     if (!ray_sdf(V, cache, p, R.inv_vs, &s)) continue;
-    ++nsamp;
+    ++*nsamp;
     if (prev_ok && prev_sdf > 0.0f && s < 0.0f) {
       // BisectionIntersection: three regula-falsi steps.
       float l = prev_t, r = t, m = (l + r) * 0.5f;
@@ -151,39 +148,259 @@ __device__ __forceinline__ uint32_t cast_pixel(const View& V, const RayConst& R,
         }
       }
       if (found && fabsf(prev_sdf - s) < R.sample_sdf_threshold && fabsf(s) < R.sdf_threshold) {
-        const float d = m * dir_c.z;
-        out_d = d;
-        out_v = reproject(R.cam, x, y, d);
-        if (R.enable_gradients) {
-          const f3 iso = mk3(o.x + m * dir.x, o.y + m * dir.y, o.z + m * dir.z);
-          const float h = 0.5f * V.geo.voxel_size;
-          bool valid = true;
-          float sp[3] = {0, 0, 0}, sn[3] = {0, 0, 0};
-#pragma unroll
-          for (int k = 0; k < 3 && valid; ++k) {
-            const f3 dp = mk3(k == 0 ? h : 0.0f, k == 1 ? h : 0.0f, k == 2 ? h : 0.0f);
-            valid = ray_sdf(V, cache, iso - dp, R.inv_vs, &sn[k]);
-            if (!valid) break;
-            valid = ray_sdf(V, cache, iso + dp, R.inv_vs, &sp[k]);
-          }
-          const float vs = V.geo.voxel_size;
-          const f3 grad = mk3((sp[0] - sn[0]) / vs, (sp[1] - sn[1]) / vs, (sp[2] - sn[2]) / vs);
-          const float l2 = length(grad);
-          f3 nw = (l2 > 0.0f && valid) ? grad / l2 : mk3(0.0f, 0.0f, 0.0f);
-          nw = -nw;
-          out_n = xform_dir(R.cTw, nw);
-        }
-        break;
+        *hit = Hit{prev_t, prev_sdf, t, s, m};
+        return true;
       }
     }
     prev_sdf = s;
     prev_t = t;
     prev_ok = true;
   }
+  return false;
+}
+
+// March one pixel; returns the number of valid samples taken.
+__device__ __forceinline__ uint32_t cast_pixel(const View& V, const RayConst& R, uint32_t x, uint32_t y,
+                               float* res_d, f3* res_v, f3* res_n) {
+  uint32_t nsamp = 0;
+  float out_d = -INFINITY;
+  f3 out_v = mk3(-INFINITY, -INFINITY, -INFINITY);
+  f3 out_n = mk3(-INFINITY, -INFINITY, -INFINITY);
+
+  const f3 dir_c = normalize(reproject(R.cam, x, y, 1.0f));
+  const float t_min = R.min_depth / dir_c.z;
+  const float t_max = R.max_depth / dir_c.z;
+  const f3 o = xform(R.wTc, mk3(0.0f, 0.0f, 0.0f));
+  const f3 dir = normalize(xform_dir(R.wTc, dir_c));
+
+  BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr};
+  Hit hit;
+  if (find_surface(V, R, o, dir, t_min, t_max, cache, &hit, &nsamp)) {
+    const float m = hit.m;
+    const float d = m * dir_c.z;
+    out_d = d;
+    out_v = reproject(R.cam, x, y, d);
+    if (R.enable_gradients) {
+      const f3 iso = mk3(o.x + m * dir.x, o.y + m * dir.y, o.z + m * dir.z);
+      const float h = 0.5f * V.geo.voxel_size;
+      bool valid = true;
+      float sp[3] = {0, 0, 0}, sn[3] = {0, 0, 0};
+#pragma unroll
+      for (int k = 0; k < 3 && valid; ++k) {
+        const f3 dp = mk3(k == 0 ? h : 0.0f, k == 1 ? h : 0.0f, k == 2 ? h : 0.0f);
+        valid = ray_sdf(V, cache, iso - dp, R.inv_vs, &sn[k]);
+        if (!valid) break;
+        valid = ray_sdf(V, cache, iso + dp, R.inv_vs, &sp[k]);
+      }
+      const float vs = V.geo.voxel_size;
+      const f3 grad = mk3((sp[0] - sn[0]) / vs, (sp[1] - sn[1]) / vs, (sp[2] - sn[2]) / vs);
+      const float l2 = length(grad);
+      f3 nw = (l2 > 0.0f && valid) ? grad / l2 : mk3(0.0f, 0.0f, 0.0f);
+      nw = -nw;
+      out_n = xform_dir(R.cTw, nw);
+    }
+  }
   *res_d = out_d;
   *res_v = out_v;
   *res_n = out_n;
   return nsamp;
+}
+
+// ------------------------------------------------------------ adjoint
+// GetSpatialValue's SDF (same corners, weights and summation order as
+// ray_sdf, so the value is bit-identical) with the spatial gradient of the
+// trilinear interpolant: each ratio frac(pos / vs) moves at 1 / vs.
+__device__ __forceinline__ bool ray_sdf_grad(const View& V, BlockCache& cache, f3 pos, float* val,
+                                             f3* grad) {
+  const float vs = V.geo.voxel_size;
+  const float hv = 0.5f * vs;
+  const float pcx = pos.x - hv, pcy = pos.y - hv, pcz = pos.z - hv;
+  const float qx = pos.x / vs, qy = pos.y / vs, qz = pos.z / vs;
+  const float rx = qx - floorf(qx), ry = qy - floorf(qy), rz = qz - floorf(qz);
+  const int cx[2] = {vox1(pcx + 0.0f, vs), vox1(pcx + vs, vs)};
+  const int cy[2] = {vox1(pcy + 0.0f, vs), vox1(pcy + vs, vs)};
+  const int cz[2] = {vox1(pcz + 0.0f, vs), vox1(pcz + vs, vs)};
+  float sdf = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int mx = (i >> 2) & 1, my = (i >> 1) & 1, mz = i & 1;
+    const i3 b = i3{voxel_to_block1(cx[mx]), voxel_to_block1(cy[my]), voxel_to_block1(cz[mz])};
+    const int32_t p = cached_find(V, cache, b);
+    if (p < 0) return false;
+    const uint32_t li = (uint32_t)((cx[mx] - b.x * 8) + 8 * (cy[my] - b.y * 8) + 64 * (cz[mz] - b.z * 8));
+    const float c = V.sdf[(size_t)p * kBlockVoxels + li];
+    const float wx = mx ? rx : 1.0f - rx, wy = my ? ry : 1.0f - ry, wz = mz ? rz : 1.0f - rz;
+    sdf += wx * wy * wz * c;
+    gx += (mx ? c : -c) * wy * wz;
+    gy += (my ? c : -c) * wx * wz;
+    gz += (mz ? c : -c) * wx * wy;
+  }
+  *val = sdf;
+  *grad = mk3(gx / vs, gy / vs, gz / vs);
+  return true;
+}
+
+// Adds coef * (trilinear weight of each corner) to the corners' gradient.
+__device__ __forceinline__ void scatter_sdf(const View& V, BlockCache& cache, f3 pos, float coef,
+                                            float* __restrict__ grad_sdf) {
+  const float vs = V.geo.voxel_size;
+  const float hv = 0.5f * vs;
+  const float pcx = pos.x - hv, pcy = pos.y - hv, pcz = pos.z - hv;
+  const float qx = pos.x / vs, qy = pos.y / vs, qz = pos.z / vs;
+  const float rx = qx - floorf(qx), ry = qy - floorf(qy), rz = qz - floorf(qz);
+  const int cx[2] = {vox1(pcx + 0.0f, vs), vox1(pcx + vs, vs)};
+  const int cy[2] = {vox1(pcy + 0.0f, vs), vox1(pcy + vs, vs)};
+  const int cz[2] = {vox1(pcz + 0.0f, vs), vox1(pcz + vs, vs)};
+  for (int i = 0; i < 8; ++i) {
+    const int mx = (i >> 2) & 1, my = (i >> 1) & 1, mz = i & 1;
+    const i3 b = i3{voxel_to_block1(cx[mx]), voxel_to_block1(cy[my]), voxel_to_block1(cz[mz])};
+    const int32_t p = cached_find(V, cache, b);
+    if (p < 0) continue;  // cannot happen for a sample the forward pass accepted
+    const uint32_t li = (uint32_t)((cx[mx] - b.x * 8) + 8 * (cy[my] - b.y * 8) + 64 * (cz[mz] - b.z * 8));
+    const float w = (mx ? rx : 1.0f - rx) * (my ? ry : 1.0f - ry) * (mz ? rz : 1.0f - rz);
+    atomicAdd(&grad_sdf[(size_t)p * kBlockVoxels + li], coef * w);
+  }
+}
+
+// Tangents: d/d(omega_xyz, upsilon_xyz) of a pose perturbation
+// wTc <- exp(xi^) wTc (a world point x moves by omega x x + upsilon), then
+// d/d(eps_0..eps_4), a perturbation of the value of each of the five SDF
+// evaluations the depth depends on (the two bracketing samples and the
+// three regula-falsi points).
+constexpr int kTan = 11;
+constexpr int kGradSlots = 30;  // loss, g[6], H upper triangle [21], pixels, gated out
+
+__device__ __forceinline__ void sample_tangent(f3 x, f3 g, int slot, float* t) {
+  const f3 c = mk3(x.y * g.z - x.z * g.y, x.z * g.x - x.x * g.z, x.x * g.y - x.y * g.x);  // x cross g
+  t[0] = c.x; t[1] = c.y; t[2] = c.z;
+  t[3] = g.x; t[4] = g.y; t[5] = g.z;
+#pragma unroll
+  for (int k = 6; k < kTan; ++k) t[k] = k == slot ? 1.0f : 0.0f;
+}
+
+// Differentiates the rendered depth of one pixel: recomputes the march
+// (same decisions as the forward pass), then carries tangents through the
+// three regula-falsi steps.  Returns false when the pixel has no hit.
+__device__ __forceinline__ bool depth_tangent(const View& V, const RayConst& R, uint32_t x, uint32_t y,
+                                              BlockCache& cache, float* depth, float* dd, f3* X) {
+  const f3 dir_c = normalize(reproject(R.cam, x, y, 1.0f));
+  const float t_min = R.min_depth / dir_c.z;
+  const float t_max = R.max_depth / dir_c.z;
+  const f3 o = xform(R.wTc, mk3(0.0f, 0.0f, 0.0f));
+  const f3 dir = normalize(xform_dir(R.wTc, dir_c));
+  Hit hit;
+  uint32_t ns = 0;
+  if (!find_surface(V, R, o, dir, t_min, t_max, cache, &hit, &ns)) return false;
+  float l = hit.tp, r = hit.tk, ls, rs;
+  float dl[kTan], dr[kTan], dls[kTan], drs[kTan], dm[kTan];
+  f3 g;
+  X[0] = mk3(o.x + l * dir.x, o.y + l * dir.y, o.z + l * dir.z);
+  X[1] = mk3(o.x + r * dir.x, o.y + r * dir.y, o.z + r * dir.z);
+  ray_sdf_grad(V, cache, X[0], &ls, &g);
+  sample_tangent(X[0], g, 6, dls);
+  ray_sdf_grad(V, cache, X[1], &rs, &g);
+  sample_tangent(X[1], g, 7, drs);
+#pragma unroll
+  for (int k = 0; k < kTan; ++k) dl[k] = dr[k] = 0.0f;
+  float m = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const float D = ls - rs;
+    const float q = ls / D;
+    m = l + (ls / (ls - rs)) * (r - l);
+    const float inv_d2 = 1.0f / (D * D);
+#pragma unroll
+    for (int k = 0; k < kTan; ++k) {
+      const float dq = (ls * drs[k] - rs * dls[k]) * inv_d2;
+      dm[k] = dl[k] + dq * (r - l) + q * (dr[k] - dl[k]);
+    }
+    X[2 + j] = mk3(o.x + m * dir.x, o.y + m * dir.y, o.z + m * dir.z);
+    float ms;
+    ray_sdf_grad(V, cache, X[2 + j], &ms, &g);
+    float dms[kTan];
+    sample_tangent(X[2 + j], g, 8 + j, dms);
+    const float gd = g.x * dir.x + g.y * dir.y + g.z * dir.z;
+#pragma unroll
+    for (int k = 0; k < kTan; ++k) dms[k] += gd * dm[k];
+    if (ls * ms > 0.0f) {
+      l = m;
+      ls = ms;
+#pragma unroll
+      for (int k = 0; k < kTan; ++k) { dl[k] = dm[k]; dls[k] = dms[k]; }
+    } else {
+      r = m;
+      rs = ms;
+#pragma unroll
+      for (int k = 0; k < kTan; ++k) { dr[k] = dm[k]; drs[k] = dms[k]; }
+    }
+  }
+  *depth = m * dir_c.z;
+#pragma unroll
+  for (int k = 0; k < kTan; ++k) dd[k] = dm[k] * dir_c.z;
+  return true;
+}
+
+// Per pixel: residual r = depth - observed depth (pixels with a hit, a
+// finite positive observation and |r| <= max_residual; the others with a
+// hit and an observation are counted as gated out), J = d depth / d xi.  Per workgroup (one
+// wave): FP64 sums of r^2 / 2, r J, J J^T (upper triangle) and the pixel
+// count, reduced in a fixed order by raycast_grad_reduce_kernel.  With
+// grad_sdf, r * d depth / d sdf is scattered onto the voxels.
+__global__ __launch_bounds__(64) void raycast_backward_kernel(View V, RayConst R,
+                                                              const float* __restrict__ obs,
+                                                              float* __restrict__ grad_sdf,
+                                                              double* __restrict__ partials,
+                                                              float max_residual) {
+  const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7);
+  const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
+  double acc[kGradSlots];
+#pragma unroll
+  for (int k = 0; k < kGradSlots; ++k) acc[k] = 0.0;
+  if (x < R.cam.width && y < R.cam.height) {
+    const float od = obs[y * R.cam.width + x];
+    BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr};
+    float d, dd[kTan];
+    f3 X[5];
+    if (od > 0.0f && od < INFINITY && depth_tangent(V, R, x, y, cache, &d, dd, X)) {
+      const float res = d - od;
+      if (!(fabsf(res) <= max_residual)) {
+        acc[29] = 1.0;
+      } else {
+        acc[0] = 0.5 * (double)res * (double)res;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[1 + k] = (double)res * (double)dd[k];
+        int t = 7;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+          for (int j = i; j < 6; ++j) acc[t++] = (double)dd[i] * (double)dd[j];
+        acc[28] = 1.0;
+        if (grad_sdf)
+          for (int e = 0; e < 5; ++e) scatter_sdf(V, cache, X[e], res * dd[6 + e], grad_sdf);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kGradSlots; ++k) {
+    double v = acc[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    acc[k] = v;
+  }
+  if (threadIdx.x == 0) {
+    const size_t wg = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+#pragma unroll
+    for (int k = 0; k < kGradSlots; ++k) partials[wg * kGradSlots + k] = acc[k];
+  }
+}
+
+// Sums the per-workgroup partials in workgroup order (deterministic).
+__global__ void raycast_grad_reduce_kernel(const double* __restrict__ partials, uint32_t n,
+                                           double* __restrict__ out) {
+  const uint32_t k = threadIdx.x;
+  if (k >= (uint32_t)kGradSlots) return;
+  double s = 0.0;
+  for (uint32_t i = 0; i < n; ++i) s += partials[(size_t)i * kGradSlots + k];
+  out[k] = s;
 }
 
 __device__ __forceinline__ const RayConst& ray_of(const One&, uint32_t, const RayConst& r) { return r; }
@@ -227,6 +444,17 @@ void launch_raycast(const View& V, const RayConst& p, float* depth, float* verte
                     hipStream_t s) {
   dim3 grid((p.cam.width + 7) / 8, (p.cam.height + 7) / 8);
   raycast_kernel<<<grid, 64, 0, s>>>(One{V}, nullptr, p, depth, vertex, normal);
+}
+
+uint32_t raycast_grad_partials(const RayConst& p) {
+  return ((p.cam.width + 7) / 8) * ((p.cam.height + 7) / 8);
+}
+
+void launch_raycast_backward(const View& V, const RayConst& p, const float* obs, float* grad_sdf,
+                             double* partials, double* out, float max_residual, hipStream_t s) {
+  dim3 grid((p.cam.width + 7) / 8, (p.cam.height + 7) / 8);
+  raycast_backward_kernel<<<grid, 64, 0, s>>>(V, p, obs, grad_sdf, partials, max_residual);
+  raycast_grad_reduce_kernel<<<1, 64, 0, s>>>(partials, grid.x * grid.y, out);
 }
 
 void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,

@@ -144,6 +144,39 @@ class Volume:
                                     C.c_void_p(vertex_ptr or None), C.c_void_p(normal_ptr or None)),
                 "mh_raycast")
 
+    def raycast_backward(self, cTw: np.ndarray, obs_ptr: int, grad_sdf_ptr: int = 0,
+                         params: A.RayCasterParams | None = None) -> dict:
+        """Adjoint of the raycast depth (mh_raycast_backward): loss, dL/dxi,
+        Gauss-Newton J^T J and pixel count; dL/dsdf added to grad_sdf_ptr."""
+        p = params if params is not None else self.cfg.raycaster_params()
+        g = A.RaycastGrad()
+        A.check(self.lib.mh_raycast_backward(self.h, C.byref(p), A.f32x16(cTw), C.c_void_p(obs_ptr),
+                                             C.c_void_p(grad_sdf_ptr or None), C.byref(g)),
+                "mh_raycast_backward")
+        return {"loss": g.loss, "grad": np.array(g.grad[:]),
+                "hessian": np.array(g.hessian[:]).reshape(6, 6), "pixels": g.pixels,
+                "outliers": g.outliers}
+
+    def refine_pose(self, wTc: np.ndarray, obs_ptr: int, iterations: int = 10,
+                    params: A.RayCasterParams | None = None):
+        """Gauss-Newton pose refinement (mh_refine_pose): (wTc, cost trace)."""
+        p = params if params is not None else self.cfg.raycaster_params()
+        out = (C.c_float * 16)()
+        trace = np.zeros(iterations + 1, np.float64)
+        A.check(self.lib.mh_refine_pose(self.h, C.byref(p), A.f32x16(wTc), C.c_void_p(obs_ptr),
+                                        iterations, out, A.ptr(trace, C.c_double)),
+                "mh_refine_pose")
+        return np.array(out[:], np.float32).reshape(4, 4), trace
+
+    def dump_voxel_field(self, field_ptr: int) -> np.ndarray:
+        n = C.c_uint32(0)
+        A.check(self.lib.mh_dump_voxel_field(self.h, C.c_void_p(field_ptr), 0, None, C.byref(n)),
+                "count")
+        out = np.zeros((n.value, 512), np.float32)
+        A.check(self.lib.mh_dump_voxel_field(self.h, C.c_void_p(field_ptr), n.value,
+                                             A.ptr(out, C.c_float), C.byref(n)), "mh_dump_voxel_field")
+        return out
+
     def pd_init(self):
         A.check(self.lib.mh_primal_dual_init(self.h), "mh_primal_dual_init")
 

@@ -945,6 +945,253 @@ int orc_raycast(orc_volume* v, const mh_raycaster_params* rp, const float* cTw, 
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------- raycast adjoint
+// The backward pass of the raycaster (no counterpart in the reference): the
+// depth of a pixel as a function of a left pose perturbation xi = (omega,
+// upsilon), wTc <- exp(xi^) wTc, and of the voxel SDFs, differentiated
+// through the bracketing samples and the three regula-falsi steps of the
+// march (visualization/ray_caster.cu:56-92, isosurface_intersection.h:24-55)
+// and the trilinear interpolant (spatial_query.h:23-66).
+//   T = float : the march exactly as orc_raycast (FP32 positions, values and
+//               decisions); derivatives in FP64.  The GPU's parity target.
+//   T = double: the whole march in FP64.  The finite-difference target.
+template <typename T>
+struct V3 {
+  T x, y, z;
+};
+template <typename T>
+V3<T> nrm(V3<T> v) {
+  const T s = T(1) / std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+  return V3<T>{v.x * s, v.y * s, v.z * s};
+}
+template <typename T>
+int w2v1T(T w, T vs) {
+  const T p = w / vs;
+  const T q = p + (p < T(0) ? T(-1) : T(1)) * T(0.5);
+  if (!(q == q)) return 0;
+  if (q >= T(2147483647.0)) return 2147483647;
+  if (q <= T(-2147483648.0)) return -2147483647 - 1;
+  return (int)q;
+}
+
+template <typename T>
+struct TriG {
+  T val;
+  double g[3];
+  I3 lo;  // lowest corner voxel (cell id for path checks)
+  int bi[8], li[8];
+  double w[8];
+};
+
+template <typename T>
+bool tri_eval(const orc_volume* v, V3<T> pos, TriG<T>* o) {
+  const T vs = (T)v->g.vs;
+  const T hv = T(0.5) * vs;
+  const T pc[3] = {pos.x - hv, pos.y - hv, pos.z - hv};
+  const T pp[3] = {pos.x, pos.y, pos.z};
+  int cv[3][2];
+  T r[3];
+  for (int k = 0; k < 3; ++k) {
+    cv[k][0] = w2v1T<T>(pc[k] + T(0), vs);
+    cv[k][1] = w2v1T<T>(pc[k] + vs, vs);
+    const T q = pp[k] / vs;
+    r[k] = q - std::floor(q);
+  }
+  T val = T(0);
+  double g[3] = {0, 0, 0};
+  for (int i = 0; i < 8; ++i) {
+    const int m[3] = {(i >> 2) & 1, (i >> 1) & 1, i & 1};
+    int bi, li;
+    if (!voxel_at(v, I3{cv[0][m[0]], cv[1][m[1]], cv[2][m[2]]}, &bi, &li)) return false;
+    const T c = (T)v->blocks[bi].sdf[li];
+    const T wx = m[0] ? r[0] : T(1) - r[0], wy = m[1] ? r[1] : T(1) - r[1], wz = m[2] ? r[2] : T(1) - r[2];
+    const T w = wx * wy * wz;
+    val += w * c;
+    const double cd = (double)c;
+    g[0] += (m[0] ? cd : -cd) * (double)wy * (double)wz;
+    g[1] += (m[1] ? cd : -cd) * (double)wx * (double)wz;
+    g[2] += (m[2] ? cd : -cd) * (double)wx * (double)wy;
+    o->bi[i] = bi;
+    o->li[i] = li;
+    o->w[i] = (double)w;
+  }
+  o->val = val;
+  for (int k = 0; k < 3; ++k) o->g[k] = g[k] / (double)vs;
+  o->lo = I3{cv[0][0], cv[1][0], cv[2][0]};
+  return true;
+}
+
+constexpr int kTanN = 11;  // omega xyz, upsilon xyz, eps_0..eps_4
+
+struct PixAdj {
+  double depth;
+  double dd[kTanN];
+  int eval_b[5][8], eval_l[5][8];
+  double eval_w[5][8];
+  int64_t path;
+};
+
+template <typename T>
+void sample_tan(const V3<T>& x, const double* g, int slot, double* t) {
+  const double X[3] = {(double)x.x, (double)x.y, (double)x.z};
+  t[0] = X[1] * g[2] - X[2] * g[1];
+  t[1] = X[2] * g[0] - X[0] * g[2];
+  t[2] = X[0] * g[1] - X[1] * g[0];
+  t[3] = g[0];
+  t[4] = g[1];
+  t[5] = g[2];
+  for (int k = 6; k < kTanN; ++k) t[k] = k == slot ? 1.0 : 0.0;
+}
+
+uint64_t mix64(uint64_t h, uint64_t x) {
+  h ^= x + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  return h * 0xBF58476D1CE4E5B9ull;
+}
+
+template <typename T>
+bool adj_pixel(const orc_volume* v, const mh_raycaster_params* rp, const T* wTc, uint32_t x,
+               uint32_t y, PixAdj* out) {
+  const T fx = rp->fx, fy = rp->fy, cx = rp->cx, cy = rp->cy;
+  V3<T> dc{((T)x - cx) / fx, ((T)y - cy) / fy, T(1)};
+  dc = V3<T>{T(1) * dc.x, T(1) * dc.y, T(1)};
+  dc = nrm(dc);
+  const T t_min = (T)rp->min_raycast_depth / dc.z;
+  const T t_max = (T)rp->max_raycast_depth / dc.z;
+  const T step = (T)rp->raycast_step;
+  const V3<T> o{wTc[0] * T(0) + wTc[1] * T(0) + wTc[2] * T(0) + wTc[3] * T(1),
+                wTc[4] * T(0) + wTc[5] * T(0) + wTc[6] * T(0) + wTc[7] * T(1),
+                wTc[8] * T(0) + wTc[9] * T(0) + wTc[10] * T(0) + wTc[11] * T(1)};
+  const V3<T> dir = nrm(V3<T>{wTc[0] * dc.x + wTc[1] * dc.y + wTc[2] * dc.z + wTc[3] * T(0),
+                              wTc[4] * dc.x + wTc[5] * dc.y + wTc[6] * dc.z + wTc[7] * T(0),
+                              wTc[8] * dc.x + wTc[9] * dc.y + wTc[10] * dc.z + wTc[11] * T(0)});
+  auto at = [&](T t) { return V3<T>{o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z}; };
+  T ps = T(0), pt = T(0);
+  bool pok = false;
+  int64_t k = 0;
+  for (T t = t_min; t < t_max; t += step, ++k) {
+    TriG<T> s;
+    if (!tri_eval(v, at(t), &s)) continue;
+    if (pok && ps > T(0) && s.val < T(0)) {
+      T l = pt, r = t, m = (l + r) * T(0.5), ls = ps, rs = s.val;
+      bool found = true;
+      for (int j = 0; j < 3; ++j) {
+        m = l + (ls / (ls - rs)) * (r - l);
+        TriG<T> q;
+        if (!tri_eval(v, at(m), &q)) {
+          found = false;
+          break;
+        }
+        if (ls * q.val > T(0)) { l = m; ls = q.val; }
+        else { r = m; rs = q.val; }
+      }
+      if (found && std::fabs(ps - s.val) < (T)rp->sample_sdf_threshold &&
+          std::fabs(s.val) < (T)rp->sdf_threshold) {
+        // accepted: replay the refinement with tangents
+        TriG<T> e[5];
+        V3<T> X[5] = {at(pt), at(t)};
+        tri_eval(v, X[0], &e[0]);
+        tri_eval(v, X[1], &e[1]);
+        double dl[kTanN] = {0}, dr[kTanN] = {0}, dls[kTanN], drs[kTanN], dm[kTanN] = {0};
+        sample_tan(X[0], e[0].g, 6, dls);
+        sample_tan(X[1], e[1].g, 7, drs);
+        T L = pt, Rr = t, LS = e[0].val, RS = e[1].val, M = T(0);
+        uint64_t path = mix64(0, (uint64_t)k);
+        for (int j = 0; j < 3; ++j) {
+          const double D = (double)LS - (double)RS, q = (double)LS / D;
+          M = L + (LS / (LS - RS)) * (Rr - L);
+          for (int c = 0; c < kTanN; ++c) {
+            const double dq = ((double)LS * drs[c] - (double)RS * dls[c]) / (D * D);
+            dm[c] = dl[c] + dq * ((double)Rr - (double)L) + q * (dr[c] - dl[c]);
+          }
+          X[2 + j] = at(M);
+          tri_eval(v, X[2 + j], &e[2 + j]);
+          double dms[kTanN];
+          sample_tan(X[2 + j], e[2 + j].g, 8 + j, dms);
+          const double gd = e[2 + j].g[0] * (double)dir.x + e[2 + j].g[1] * (double)dir.y +
+                            e[2 + j].g[2] * (double)dir.z;
+          for (int c = 0; c < kTanN; ++c) dms[c] += gd * dm[c];
+          const bool left = LS * e[2 + j].val > T(0);
+          path = mix64(path, left ? 1u : 2u);
+          if (left) {
+            L = M;
+            LS = e[2 + j].val;
+            std::memcpy(dl, dm, sizeof(dl));
+            std::memcpy(dls, dms, sizeof(dls));
+          } else {
+            Rr = M;
+            RS = e[2 + j].val;
+            std::memcpy(dr, dm, sizeof(dr));
+            std::memcpy(drs, dms, sizeof(drs));
+          }
+        }
+        for (int q = 0; q < 5; ++q) {
+          path = mix64(path, (uint64_t)(uint32_t)e[q].lo.x);
+          path = mix64(path, (uint64_t)(uint32_t)e[q].lo.y);
+          path = mix64(path, (uint64_t)(uint32_t)e[q].lo.z);
+          for (int c = 0; c < 8; ++c) {
+            out->eval_b[q][c] = e[q].bi[c];
+            out->eval_l[q][c] = e[q].li[c];
+            out->eval_w[q][c] = e[q].w[c];
+          }
+        }
+        out->depth = (double)(M * dc.z);
+        for (int c = 0; c < kTanN; ++c) out->dd[c] = dm[c] * (double)dc.z;
+        out->path = (int64_t)(path >> 1);
+        return true;
+      }
+    }
+    ps = s.val;
+    pt = t;
+    pok = true;
+  }
+  return false;
+}
+
+template <typename T>
+void adj_image(const orc_volume* v, const mh_raycaster_params* rp, const T* wTc, const float* obs,
+               double max_residual, double* depth_out, double* jac_out, int64_t* path_out,
+               std::vector<double>* gsdf, double* summary) {
+  double acc[30] = {0};
+  const uint32_t W = rp->width, H = rp->height;
+  for (uint32_t y = 0; y < H; ++y)
+    for (uint32_t x = 0; x < W; ++x) {
+      const uint32_t pix = y * W + x;
+      PixAdj a;
+      const bool hit = adj_pixel<T>(v, rp, wTc, x, y, &a);
+      if (depth_out) depth_out[pix] = hit ? a.depth : -INFINITY;
+      if (path_out) path_out[pix] = hit ? a.path : -1;
+      if (jac_out)
+        for (int c = 0; c < kTanN; ++c) jac_out[(size_t)pix * kTanN + c] = hit ? a.dd[c] : 0.0;
+      if (!hit || !obs) continue;
+      const float od = obs[pix];
+      if (!(od > 0.0f && od < INFINITY)) continue;
+      const double r = a.depth - (double)od;
+      if (!(std::fabs((float)(a.depth - (double)od)) <= (float)max_residual)) {
+        acc[29] += 1.0;  // valid but gated out
+        continue;
+      }
+      acc[0] += 0.5 * r * r;
+      for (int c = 0; c < 6; ++c) acc[1 + c] += r * a.dd[c];
+      int t = 7;
+      for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 6; ++j) acc[t++] += a.dd[i] * a.dd[j];
+      acc[28] += 1.0;
+      if (gsdf)
+        for (int q = 0; q < 5; ++q)
+          for (int c = 0; c < 8; ++c)
+            (*gsdf)[(size_t)a.eval_b[q][c] * 512 + a.eval_l[q][c]] += r * a.dd[6 + q] * a.eval_w[q][c];
+    }
+  if (summary) std::memcpy(summary, acc, sizeof(acc));
+}
+
+}  // namespace
+
+extern "C" {
+
 // ---------------------------------------------------------------- primal-dual
 // optimize/primal_dual.cu:10-223, geometry/primal_dual_query.h:15-234.
 static bool below_eps(double w) { return (double)(float)w < 1e-6; }
@@ -1200,6 +1447,43 @@ uint64_t orc_voxel_updates(const orc_volume* v) { return v->voxel_updates; }
 uint32_t orc_list_inserts(const orc_volume* v) { return v->list_inserts; }
 
 // MC table access for the table tests.
+// Raycast adjoint (see adj_image).  mode 0: FP32 march from cTw32 (wTc =
+// FP32 inverse, as orc_raycast); mode 1: FP64 march from wTc64.  Outputs per
+// pixel (optional): depth, 11 tangents (d depth / d(omega, upsilon, eps0..4)),
+// path id (-1: no hit); grad_sdf: dL/dsdf per voxel in orc_dump_blocks
+// order; summary: loss, dL/dxi[6], J^T J upper triangle[21], pixels, and
+// the valid pixels left out by |residual| > max_residual.
+int orc_raycast_adjoint(const orc_volume* v, const mh_raycaster_params* rp, int mode,
+                        const float* cTw32, const double* wTc64, const float* obs,
+                        double max_residual, double* depth_out, double* jac_out,
+                        int64_t* path_out, double* grad_sdf, double* summary) {
+  std::vector<double> g;
+  if (grad_sdf) g.assign(v->blocks.size() * 512, 0.0);
+  if (mode == 0) {
+    float wTc[16];
+    inv4<float>(cTw32, wTc);
+    adj_image<float>(v, rp, wTc, obs, max_residual, depth_out, jac_out, path_out,
+                     grad_sdf ? &g : nullptr, summary);
+  } else {
+    adj_image<double>(v, rp, wTc64, obs, max_residual, depth_out, jac_out, path_out,
+                      grad_sdf ? &g : nullptr, summary);
+  }
+  if (grad_sdf) {
+    const std::vector<int> order = sorted_ptrs(v);
+    for (size_t i = 0; i < order.size(); ++i)
+      std::memcpy(grad_sdf + i * 512, g.data() + (size_t)order[i] * 512, 512 * sizeof(double));
+  }
+  return 0;
+}
+
+// Adds delta to one voxel's SDF (block i of orc_dump_blocks order, voxel li).
+int orc_perturb_sdf(orc_volume* v, uint32_t block, uint32_t li, double delta) {
+  const std::vector<int> order = sorted_ptrs(v);
+  if (block >= order.size() || li >= 512) return -1;
+  v->blocks[order[block]].sdf[li] += delta;
+  return 0;
+}
+
 int orc_mc_ntri(int cube) { return ntri(cube); }
 int orc_mc_tri_edge(int cube, int k) { return tri_edge(cube, k); }
 uint32_t orc_mc_edge_mask(int cube) { return edge_mask(cube); }

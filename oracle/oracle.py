@@ -74,6 +74,11 @@ def lib() -> C.CDLL:
             "orc_mc_corner": (C.c_int, [C.c_int, C.c_int]),
             "orc_mc_edge_a": (C.c_int, [C.c_int]),
             "orc_mc_edge_b": (C.c_int, [C.c_int]),
+            "orc_raycast_adjoint": (C.c_int, [vp, vp, C.c_int, P(C.c_float), P(C.c_double),
+                                              P(C.c_float), C.c_double, P(C.c_double),
+                                              P(C.c_double), P(C.c_int64), P(C.c_double),
+                                              P(C.c_double)]),
+            "orc_perturb_sdf": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_double]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -250,6 +255,51 @@ class Oracle:
 
     def voxel_updates(self) -> int:
         return self.L.orc_voxel_updates(self.h)
+
+    def raycast_adjoint(self, fp64: bool, pose, obs=None, params=None, jac=False, grad_sdf=False,
+                        max_residual=float("inf")):
+        """Raycast adjoint.  fp64=False: FP32 march from pose = cTw (float32
+        4x4, as the GPU); fp64=True: FP64 march from pose = wTc (float64).
+        Returns depth, path (int64, -1: no hit), optional per-pixel tangents
+        (H, W, 11), optional dL/dsdf (blocks, 512) and the summary: loss,
+        grad[6], hessian (6x6), pixels."""
+        p = params if params is not None else self.cfg.raycaster_params()
+        n = p.width * p.height
+        depth = np.zeros(n, np.float64)
+        path = np.zeros(n, np.int64)
+        J = np.zeros((n, 11), np.float64) if jac else None
+        nb = self.L.orc_block_count(self.h)
+        G = np.zeros((nb, 512), np.float64) if grad_sdf else None
+        summ = np.zeros(30, np.float64)
+        o = None if obs is None else np.ascontiguousarray(obs, np.float32).reshape(-1)
+        if fp64:
+            w = np.ascontiguousarray(np.asarray(pose, np.float64).reshape(16))
+            c32 = None
+        else:
+            w = None
+            c32 = np.ascontiguousarray(np.asarray(pose, np.float32).reshape(16))
+        self.L.orc_raycast_adjoint(self.h, C.byref(p), 1 if fp64 else 0, _p(c32, C.c_float),
+                                   _p(w, C.c_double), _p(o, C.c_float), max_residual,
+                                   _p(depth, C.c_double),
+                                   _p(J, C.c_double), _p(path, C.c_int64), _p(G, C.c_double),
+                                   _p(summ, C.c_double))
+        H = np.zeros((6, 6))
+        t = 7
+        for i in range(6):
+            for j in range(i, 6):
+                H[i, j] = H[j, i] = summ[t]
+                t += 1
+        out = {"depth": depth.reshape(p.height, p.width), "path": path.reshape(p.height, p.width),
+               "loss": summ[0], "grad": summ[1:7].copy(), "hessian": H, "pixels": int(summ[28]),
+               "outliers": int(summ[29])}
+        if jac:
+            out["jac"] = J.reshape(p.height, p.width, 11)
+        if grad_sdf:
+            out["grad_sdf"] = G
+        return out
+
+    def perturb_sdf(self, block: int, voxel: int, delta: float):
+        assert self.L.orc_perturb_sdf(self.h, block, voxel, delta) == 0
 
 
 def hash_bucket(x: int, y: int, z: int, n: int) -> int:
