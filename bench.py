@@ -8,7 +8,7 @@ Workload: the full-rate plain pipeline (640x480 uint16 depth, 5 mm voxels,
 All input frames are resident in HBM before the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
-                    [--no-batched]
+                    [--no-batched] [--no-pose]
 
 Prints one JSON line.  `value` is voxel-updates/s of the whole chain and
 `ms_per_step` ms/frame, with each frame's chain replayed as one HIP graph.
@@ -16,6 +16,9 @@ The same frames run eagerly, stage by stage, give per-stage times (HIP
 events on the volume's stream) and the roofline.  `batched` holds the
 batched form: 64 independent 320x240 scenes with voxel sizes 1-4 cm, per-
 scene throughput of the fused batch against one-scene-at-a-time runs.
+`pose_refinement` holds the differentiable raycast: Gauss-Newton pose
+refinement of 32 perturbed 640x480 views and the adjoint's error against
+the CPU FP64 adjoint.
 """
 from __future__ import annotations
 
@@ -192,6 +195,88 @@ def c4_batched(dev, stream, n_scenes=64, n_frames=4):
     }
 
 
+def se3_exp(xi):
+    """exp of xi = (omega, upsilon) in se(3) as a 4x4."""
+    w, u = np.asarray(xi[:3], float), np.asarray(xi[3:], float)
+    th = np.linalg.norm(w)
+    W = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+    if th < 1e-9:
+        a, b, c = 1.0, 0.5, 1.0 / 6.0
+    else:
+        a, b, c = np.sin(th) / th, (1 - np.cos(th)) / th ** 2, (th - np.sin(th)) / th ** 3
+    T = np.eye(4)
+    T[:3, :3] = np.eye(3) + a * W + b * W @ W
+    T[:3, 3] = (np.eye(3) + b * W + c * W @ W) @ u
+    return T
+
+
+def c5_pose_refinement(dev, stream, n_poses=32, iterations=10, grad_check=True):
+    """Differentiable raycast: the room of the 320x240 sequence fused at 5 mm
+    from 640x480 frames; n_poses poses around the room perturbed by a
+    rotation of N(0, 2 deg) about a random axis and a translation of
+    N(0, (2 cm)^2 / 3) per axis; iterations Gauss-Newton steps each on the
+    observed 640x480 depth.  grad_check: the GPU adjoint against the
+    oracle's (same FP32 march, FP64 tangents) at the first query pose."""
+    import torch
+    from dataclasses import replace
+
+    from meshhashing_amd.configs import C5
+    from meshhashing_amd.engine import Volume, synth_depth, synth_pose
+
+    vol = Volume(C5)
+    vol.set_stream(stream.cuda_stream)
+    for f in range(C5.frames):
+        vol.upload_depth(synth_depth(C5, f), synth_pose(C5, f))
+        vol.alloc()
+        vol.collect_visible()
+        vol.integrate_tsdf()
+    q = replace(C5, frames=n_poses)
+    rng = np.random.default_rng(5)
+    err0, err1, ms = [], [], []
+    first = None
+    for i in range(n_poses):
+        gt = synth_pose(q, i).astype(np.float64)
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        xi = np.concatenate([axis * np.radians(rng.normal(0.0, 2.0)), rng.normal(size=3) * 0.02 / np.sqrt(3)])
+        start = se3_exp(xi) @ gt
+        obs_np = (synth_depth(q, i) * np.float32(0.001)).astype(np.float32)
+        obs = torch.from_numpy(obs_np).to(dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out, trace = vol.refine_pose(start.astype(np.float32), obs.data_ptr(), iterations)
+        ms.append(1e3 * (time.perf_counter() - t0))
+        for lst, T in ((err0, start), (err1, out)):
+            D = np.linalg.inv(gt) @ T.astype(np.float64)
+            ang = np.degrees(np.arccos(np.clip((np.trace(D[:3, :3]) - 1) / 2, -1, 1)))
+            lst.append((ang, 1e3 * np.linalg.norm(D[:3, 3])))
+        if first is None:
+            first = (np.linalg.inv(start).astype(np.float32), obs_np, obs)
+    res = {
+        "workload": f"{n_poses} poses, 640x480 depth, 5 mm volume of the 320x240 room "
+                    f"({vol.status()['block_count']} blocks), {iterations} Gauss-Newton steps each",
+        "initial_error_deg_mm_median": [float(np.median([e[0] for e in err0])), float(np.median([e[1] for e in err0]))],
+        "final_error_deg_mm_median": [float(np.median([e[0] for e in err1])), float(np.median([e[1] for e in err1]))],
+        "final_error_deg_mm_max": [float(max(e[0] for e in err1)), float(max(e[1] for e in err1))],
+        "ms_per_pose": float(np.median(ms)),
+    }
+    if grad_check:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as orc
+
+        cTw, obs_np, obs = first
+        b = vol.dump_blocks()
+        o = orc.Oracle(C5)
+        o.load_blocks(b["pos"], b["sdf"], b["weight"], b["color"])
+        g = vol.raycast_backward(cTw, obs.data_ptr())
+        ref = o.raycast_adjoint(False, cTw, obs_np)
+        res["grad_rel_err_vs_cpu_fp64_adjoint"] = float(np.abs(g["grad"] - ref["grad"]).max() /
+                                                        np.abs(ref["grad"]).max())
+        res["grad_check_pixels"] = [g["pixels"], ref["pixels"]]
+    vol.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +284,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-pose", action="store_true")
     ap.add_argument("--stage-events", type=int, default=1)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run 5 plain 256 MiB device copies (reference dispatches "
@@ -371,6 +457,8 @@ def main():
     }
     if not args.no_batched:
         line["batched"] = c4_batched(dev, stream)
+    if not args.no_pose:
+        line["pose_refinement"] = c5_pose_refinement(dev, stream, grad_check=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, frames=min(total, 3))
     print(json.dumps(line))

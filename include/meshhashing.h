@@ -291,7 +291,8 @@ int mh_raycast_backward(mh_volume* v, const mh_raycaster_params* p,
  * from wTc_init, on a truncated quadratic: pixels with |depth - obs| above
  * a gate cost gate^2 / 2 each (no gate for the first 40 % of the steps,
  * then 0.10 m, then 0.05 m for the last 30 %); a step is kept only if it
- * lowers the cost per pixel.  cost_trace (may be NULL) receives the cost
+ * lowers the cost per pixel.  Rays close to grazing the surface (depth
+ * moving more than 3x a camera translation along some axis) are skipped.  cost_trace (may be NULL) receives the cost
  * per pixel before the first and after every step (iterations + 1 values,
  * each at that step's gate).                                               */
 int mh_refine_pose(mh_volume* v, const mh_raycaster_params* p,

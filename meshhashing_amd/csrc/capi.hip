@@ -1278,7 +1278,8 @@ int mh_batch_synchronize(mh_batch* b) {
 namespace {
 
 int raycast_grad(mh_volume* v, const mh_raycaster_params* p, const float cTw[16], const float* obs,
-                 float* grad_sdf, mh_raycast_grad* out, float max_residual = INFINITY) {
+                 float* grad_sdf, mh_raycast_grad* out, float max_residual = INFINITY,
+                 float max_slope = INFINITY) {
   RayConst R;
   const int rc = build_ray(v, p, cTw, &R);
   if (rc) return rc;
@@ -1292,7 +1293,7 @@ int raycast_grad(mh_volume* v, const mh_raycaster_params* p, const float cTw[16]
   }
   if (!v->rgrad_out) HIP_TRY(dalloc(&v->rgrad_out, 30));
   launch_raycast_backward(v->V, R, obs, grad_sdf, v->rgrad_partials, v->rgrad_out, max_residual,
-                          v->stream);
+                          max_slope, v->stream);
   HIP_TRY(hipGetLastError());
   double h[30];
   HIP_TRY(hipMemcpyAsync(h, v->rgrad_out, sizeof(h), hipMemcpyDeviceToHost, v->stream));
@@ -1383,13 +1384,17 @@ bool solve6(const double* H, const double* g, double lambda, double* x) {
   return true;
 }
 
+// Rays that nearly graze the surface (depth moving more than 3x a camera
+// translation) carry unreliable Jacobians on a noisy SDF; refinement skips them.
+constexpr float kMaxSlope = 3.0f;
+
 int eval_pose(mh_volume* v, const mh_raycaster_params* p, const double T[16], const float* obs,
               float gate, mh_raycast_grad* out) {
   double inv[16];
   invert4<double>(T, inv);
   float cTw[16];
   for (int i = 0; i < 16; ++i) cTw[i] = (float)inv[i];
-  return raycast_grad(v, p, cTw, obs, nullptr, out, gate);
+  return raycast_grad(v, p, cTw, obs, nullptr, out, gate, kMaxSlope);
 }
 
 // Residual gate of the pose refinement: pixels with |r| above it (occlusion

@@ -142,7 +142,8 @@ void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float*
 // adjoint: 29 FP64 sums (loss, dL/dxi[6], J^T J upper triangle[21], pixels)
 uint32_t raycast_grad_partials(const RayConst& p);
 void launch_raycast_backward(const View& V, const RayConst& p, const float* obs, float* grad_sdf,
-                             double* partials, double* out, float max_residual, hipStream_t s);
+                             double* partials, double* out, float max_residual, float max_slope,
+                             hipStream_t s);
 
 // primal_dual.hip
 void launch_pd_neighbors(const View& V, int32_t* nbr, hipStream_t s, int grid);

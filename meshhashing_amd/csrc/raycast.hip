@@ -342,7 +342,10 @@ __device__ __forceinline__ bool depth_tangent(const View& V, const RayConst& R, 
 
 // Per pixel: residual r = depth - observed depth (pixels with a hit, a
 // finite positive observation and |r| <= max_residual; the others with a
-// hit and an observation are counted as gated out), J = d depth / d xi.  Per workgroup (one
+// hit and an observation are counted as gated out), J = d depth / d xi.
+// Pixels whose depth moves faster than max_slope times a camera translation
+// (|d depth / d upsilon| on some axis; rays close to grazing the surface)
+// are left out altogether.  Per workgroup (one
 // wave): FP64 sums of r^2 / 2, r J, J J^T (upper triangle) and the pixel
 // count, reduced in a fixed order by raycast_grad_reduce_kernel.  With
 // grad_sdf, r * d depth / d sdf is scattered onto the voxels.
@@ -350,7 +353,7 @@ __global__ __launch_bounds__(64) void raycast_backward_kernel(View V, RayConst R
                                                               const float* __restrict__ obs,
                                                               float* __restrict__ grad_sdf,
                                                               double* __restrict__ partials,
-                                                              float max_residual) {
+                                                              float max_residual, float max_slope) {
   const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7);
   const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
   double acc[kGradSlots];
@@ -361,7 +364,8 @@ __global__ __launch_bounds__(64) void raycast_backward_kernel(View V, RayConst R
     BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr};
     float d, dd[kTan];
     f3 X[5];
-    if (od > 0.0f && od < INFINITY && depth_tangent(V, R, x, y, cache, &d, dd, X)) {
+    if (od > 0.0f && od < INFINITY && depth_tangent(V, R, x, y, cache, &d, dd, X) &&
+        fmaxf(fabsf(dd[3]), fmaxf(fabsf(dd[4]), fabsf(dd[5]))) <= max_slope) {
       const float res = d - od;
       if (!(fabsf(res) <= max_residual)) {
         acc[29] = 1.0;
@@ -451,9 +455,11 @@ uint32_t raycast_grad_partials(const RayConst& p) {
 }
 
 void launch_raycast_backward(const View& V, const RayConst& p, const float* obs, float* grad_sdf,
-                             double* partials, double* out, float max_residual, hipStream_t s) {
+                             double* partials, double* out, float max_residual, float max_slope,
+                             hipStream_t s) {
   dim3 grid((p.cam.width + 7) / 8, (p.cam.height + 7) / 8);
-  raycast_backward_kernel<<<grid, 64, 0, s>>>(V, p, obs, grad_sdf, partials, max_residual);
+  raycast_backward_kernel<<<grid, 64, 0, s>>>(V, p, obs, grad_sdf, partials, max_residual,
+                                              max_slope);
   raycast_grad_reduce_kernel<<<1, 64, 0, s>>>(partials, grid.x * grid.y, out);
 }
 
