@@ -155,6 +155,15 @@ int mh_frame_upload_depth_u16(mh_volume* v, const uint16_t* depth_host,
 int mh_frame_depth_u16_device(mh_volume* v, const uint16_t* depth_dev,
                               const float wTc[16]);
 int mh_frame_set_pose(mh_volume* v, const float wTc[16]);
+/* Colour of the current frame (Sensor::Process's ConvertColorFormat,
+ * sensor/preprocess.cu:44-58): 8-bit BGRA, W*H*4 bytes.  A pixel is valid
+ * iff its B, G and R are all non-zero; fusion then uses
+ * uchar(255 * (c / 255.0f)) per channel (update_simple.cu:68-70), an invalid
+ * pixel contributes (0, 0, 0).  Applies to the frame uploaded last; a depth
+ * upload clears it, and without colour TSDF fusion uses (0, 255, 0)
+ * (update_simple.cu:71-73).                                                */
+int mh_frame_upload_color_bgra8(mh_volume* v, const uint8_t* bgra_host);
+int mh_frame_color_bgra8_device(mh_volume* v, const uint8_t* bgra_dev);
 /* Converted metric depth of the current frame (device, W*H floats). */
 int mh_frame_depth_device_ptr(mh_volume* v, const float** depth_dev);
 
@@ -367,6 +376,9 @@ int mh_synth_depth(const mh_synth_spec* s, uint32_t frame, uint16_t* depth_host)
 /* Exact (noise-free) z-depth in metres, for tests (+inf: no hit).          */
 int mh_synth_depth_exact(const mh_synth_spec* s, uint32_t frame,
                          const float wTc[16], double* depth_host);
+/* Colour of frame f: 8-bit BGRA per pixel (W*H*4 bytes), a procedural
+ * texture of the surface point, 0 where there is no surface.               */
+int mh_synth_color(const mh_synth_spec* s, uint32_t frame, uint8_t* bgra_host);
 
 #ifdef __cplusplus
 }

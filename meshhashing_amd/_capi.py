@@ -128,6 +128,8 @@ SIGNATURES = {
     "mh_frame_upload_depth_u16": (C.c_int, [VP, P_U16, P_F32]),
     "mh_frame_depth_u16_device": (C.c_int, [VP, VP, P_F32]),
     "mh_frame_set_pose": (C.c_int, [VP, P_F32]),
+    "mh_frame_upload_color_bgra8": (C.c_int, [VP, P_U8]),
+    "mh_frame_color_bgra8_device": (C.c_int, [VP, VP]),
     "mh_frame_depth_device_ptr": (C.c_int, [VP, C.POINTER(VP)]),
     "mh_alloc_blocks": (C.c_int, [VP]),
     "mh_collect_visible": (C.c_int, [VP, P_U32]),
@@ -171,6 +173,7 @@ SIGNATURES = {
     "mh_synth_pose": (C.c_int, [C.POINTER(SynthSpec), C.c_uint32, P_F32]),
     "mh_synth_depth": (C.c_int, [C.POINTER(SynthSpec), C.c_uint32, P_U16]),
     "mh_synth_depth_exact": (C.c_int, [C.POINTER(SynthSpec), C.c_uint32, P_F32, P_F64]),
+    "mh_synth_color": (C.c_int, [C.POINTER(SynthSpec), C.c_uint32, P_U8]),
 }
 
 _lib = None

@@ -86,6 +86,7 @@ struct mh_volume {
   int device = 0;
   int cus = 256;
   uint16_t* raw = nullptr;  // staging for host uploads
+  uint8_t* raw_color = nullptr;  // staging for host colour uploads (BGRA)
   bool have_frame = false;
   int32_t* pd_nbr = nullptr;
   uint32_t pd_nbr_cap = 0;
@@ -291,7 +292,9 @@ int mh_volume_create(const mh_hash_params* hash, const mh_volume_params* vol,
   A(&V.cand, cap);
   A(&V.depth, npix);
   A(&V.inlier, npix);
+  A(&V.color_in, npix);
   A(&v->raw, npix);
+  A(&v->raw_color, npix * 4);
   if (!ok) {
     for (void* p : v->owned) (void)hipFree(p);
     delete v;
@@ -385,7 +388,25 @@ int mh_frame_depth_u16_device(mh_volume* v, const uint16_t* depth_dev, const flo
   launch_convert_depth(v->V, depth_dev, v->stream);
   HIP_TRY(hipGetLastError());
   v->have_frame = true;
+  v->V.has_color = 0;
   return MH_OK;
+}
+
+int mh_frame_color_bgra8_device(mh_volume* v, const uint8_t* bgra_dev) {
+  if (check_vol(v) || !bgra_dev) return fail(MH_ERR_ARG, "null argument");
+  if ((uintptr_t)bgra_dev % 4) return fail(MH_ERR_ARG, "colour buffer must be 4-byte aligned");
+  if (!v->have_frame) return fail(MH_ERR_STATE, "upload the frame's depth first");
+  launch_convert_color(v->V, bgra_dev, v->stream);
+  HIP_TRY(hipGetLastError());
+  v->V.has_color = 1;
+  return MH_OK;
+}
+
+int mh_frame_upload_color_bgra8(mh_volume* v, const uint8_t* bgra_host) {
+  if (check_vol(v) || !bgra_host) return fail(MH_ERR_ARG, "null argument");
+  const size_t n = (size_t)v->sp.width * v->sp.height * 4;
+  HIP_TRY(hipMemcpyAsync(v->raw_color, bgra_host, n, hipMemcpyHostToDevice, v->stream));
+  return mh_frame_color_bgra8_device(v, v->raw_color);
 }
 
 int mh_frame_upload_depth_u16(mh_volume* v, const uint16_t* depth_host, const float wTc[16]) {
@@ -1170,7 +1191,10 @@ int mh_batch_frame_u16(mh_batch* b, const uint16_t* depth, int on_device, const 
     HIP_TRY(hipMemcpyAsync(b->d_raw, depth, npix * sizeof(uint16_t), hipMemcpyHostToDevice, b->stream));
     b->raw = b->d_raw;
   }
-  for (uint32_t s = 0; s < b->n; ++s) set_pose(b->vols[s], wTc + 16 * s);
+  for (uint32_t s = 0; s < b->n; ++s) {
+    set_pose(b->vols[s], wTc + 16 * s);
+    b->vols[s]->V.has_color = 0;
+  }
   b->have_frame = true;
   return MH_OK;
 }

@@ -83,6 +83,26 @@ __global__ void convert_depth_kernel(const uint16_t* __restrict__ raw, float* __
   if (inlier) inlier[i] = 0.1f;
 }
 
+// ConvertColorFormatKernel (sensor/preprocess.cu:44-58) followed by the
+// float -> uchar conversion of UpdateBlocksSimpleKernel (update_simple.cu:
+// 68-70): valid iff B, G, R are all non-zero; channel = uchar(255 * (c /
+// 255.0f)), a truncation that clamps to [0, 255] (an invalid pixel's -inf
+// becomes 0).
+__device__ __forceinline__ uint32_t fuse_channel(uint32_t c) {
+  const float f = 255.0f * ((float)c / 255.0f);
+  const int i = f2i(f);
+  return (uint32_t)(i < 0 ? 0 : (i > 255 ? 255 : i));
+}
+
+__global__ void convert_color_kernel(const uint32_t* __restrict__ bgra, uint32_t* __restrict__ out,
+                                     uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = bgra[i];
+  const uint32_t b = c & 255u, g = (c >> 8) & 255u, r = (c >> 16) & 255u;
+  out[i] = (b && g && r) ? (fuse_channel(r) | (fuse_channel(g) << 8) | (fuse_channel(b) << 16)) : 0u;
+}
+
 // The same for every scene of a batch (scene = blockIdx.y).
 __global__ void batch_convert_depth_kernel(Many src_, const SceneFrame* __restrict__ tab, uint32_t n) {
   const Many src = with_table(src_, tab);
@@ -364,7 +384,8 @@ __device__ __forceinline__ bool integrate_voxel(const View& V, const i3 b, int32
   const float wo = V.weight[idx];
   V.sdf[idx] = (so * wo + sdf * wgt) / (wo + wgt);
   V.weight[idx] = wo + wgt;
-  V.color[idx] = blend_color(V.color[idx], 255u << 8);  // (0, 255, 0): no colour stream
+  const uint32_t dc = V.has_color ? V.color_in[v * c.width + u] : (255u << 8);  // (0, 255, 0)
+  V.color[idx] = blend_color(V.color[idx], dc);
   return true;
 }
 
@@ -615,6 +636,12 @@ void launch_convert_depth(const View& V, const uint16_t* raw, hipStream_t s) {
   const uint32_t n = V.cam.width * V.cam.height;
   convert_depth_kernel<<<(n + 255) / 256, 256, 0, s>>>(raw, V.depth, V.inlier, n, V.range_factor,
                                                        V.cam.min_depth, V.cam.max_depth);
+}
+
+void launch_convert_color(const View& V, const uint8_t* bgra, hipStream_t s) {
+  const uint32_t n = V.cam.width * V.cam.height;
+  convert_color_kernel<<<(n + 255) / 256, 256, 0, s>>>(reinterpret_cast<const uint32_t*>(bgra),
+                                                       V.color_in, n);
 }
 
 void launch_alloc(const View& V, hipStream_t s) {

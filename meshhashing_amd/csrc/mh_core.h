@@ -256,6 +256,8 @@ struct View {
   float range_factor;
   float* depth;          // W*H metric depth of the current frame
   const uint16_t* raw;   // W*H raw depth of the current frame (batched conversion)
+  uint32_t* color_in;    // W*H fusion colour of the current frame, R | G << 8 | B << 16
+  int has_color;         // the current frame has a colour image
   float* inlier;         // W*H PSDF inlier ratio
   M4 wTc, cTw;           // FP32 pose (decisions)
   D34 cTw64;             // FP64 world->camera (values)

@@ -109,6 +109,7 @@ constexpr int kDefaultIntegrateTiling = 2;  // 256 lanes, one block, 2 voxels pe
 // mapping.hip
 void launch_reset(const View& V, hipStream_t s, int grid);
 void launch_convert_depth(const View& V, const uint16_t* raw, hipStream_t s);
+void launch_convert_color(const View& V, const uint8_t* bgra, hipStream_t s);
 void launch_alloc(const View& V, hipStream_t s);
 void launch_collect(const View& V, int frustum, hipStream_t s, int grid);
 void launch_integrate_tsdf(const View& V, hipStream_t s, int grid, int tiling);

@@ -355,4 +355,34 @@ int mh_synth_depth(const mh_synth_spec* s, uint32_t frame, uint16_t* depth_host)
   return MH_OK;
 }
 
+// Colour image of frame f: a smooth procedural texture of the world hit
+// point, 8-bit BGRA (alpha 255), every channel in [28, 228]; 0 where the ray
+// leaves the room (no surface).
+int mh_synth_color(const mh_synth_spec* s, uint32_t frame, uint8_t* bgra_host) {
+  if (!spec_ok(s) || !bgra_host) return MH_ERR_ARG;
+  const Scene sc = make_scene(*s);
+  const Pose p = frame_pose(*s, frame);
+  parallel_rows(s->height, [&](uint32_t y) {
+    for (uint32_t x = 0; x < s->width; ++x) {
+      const V3 dc{((double)x - s->cx) / s->fx, ((double)y - s->cy) / s->fy, 1.0};
+      const V3 dw{p.R[0] * dc.x + p.R[1] * dc.y + p.R[2] * dc.z,
+                  p.R[3] * dc.x + p.R[4] * dc.y + p.R[5] * dc.z,
+                  p.R[6] * dc.x + p.R[7] * dc.y + p.R[8] * dc.z};
+      const double t = trace(sc, p.t, dw);
+      uint8_t* o = bgra_host + 4 * ((size_t)y * s->width + x);
+      if (!std::isfinite(t)) {
+        o[0] = o[1] = o[2] = o[3] = 0;
+        continue;
+      }
+      const V3 X{p.t.x + t * dw.x, p.t.y + t * dw.y, p.t.z + t * dw.z};
+      auto ch = [](double v) { return (uint8_t)std::floor(128.0 + 100.0 * std::sin(v) + 0.5); };
+      o[0] = ch(4.0 * X.z + 2.0);  // B
+      o[1] = ch(4.0 * X.y + 1.0);  // G
+      o[2] = ch(4.0 * X.x);        // R
+      o[3] = 255;
+    }
+  });
+  return MH_OK;
+}
+
 }  // extern "C"

@@ -30,6 +30,15 @@ def synth_depth(cfg: Config, frame: int) -> np.ndarray:
     return d
 
 
+def synth_color(cfg: Config, frame: int) -> np.ndarray:
+    """(H, W, 4) uint8 BGRA colour of a frame (0 where there is no surface)."""
+    lib = A.load()
+    c = np.zeros((cfg.height, cfg.width, 4), dtype=np.uint8)
+    A.check(lib.mh_synth_color(C.byref(cfg.synth_spec()), frame, A.ptr(c, C.c_uint8)),
+            "mh_synth_color")
+    return c
+
+
 def synth_depth_exact(cfg: Config, wTc: np.ndarray) -> np.ndarray:
     lib = A.load()
     d = np.zeros((cfg.height, cfg.width), dtype=np.float64)
@@ -81,6 +90,11 @@ class Volume:
         d = np.ascontiguousarray(depth_u16, dtype=np.uint16)
         A.check(self.lib.mh_frame_upload_depth_u16(self.h, A.ptr(d, C.c_uint16), A.f32x16(wTc)),
                 "mh_frame_upload_depth_u16")
+
+    def upload_color(self, bgra: np.ndarray):
+        c = np.ascontiguousarray(bgra, dtype=np.uint8)
+        A.check(self.lib.mh_frame_upload_color_bgra8(self.h, A.ptr(c, C.c_uint8)),
+                "mh_frame_upload_color_bgra8")
 
     def set_depth_device(self, dev_ptr: int, wTc: np.ndarray):
         A.check(self.lib.mh_frame_depth_u16_device(self.h, C.c_void_p(dev_ptr), A.f32x16(wTc)),

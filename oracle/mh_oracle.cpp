@@ -243,6 +243,8 @@ struct orc_volume {
   // frame
   std::vector<float> depth;
   std::vector<double> inlier;
+  std::vector<uint32_t> color_in;  // fusion colour per pixel, R | G << 8 | B << 16
+  bool has_color = false;
   float wTc[16], cTw[16];
   double cTw64[16];
   // mesh
@@ -504,11 +506,31 @@ void orc_invert4d(const double* m, double* out) { inv4<double>(m, out); }
 // (sensor/preprocess.cu:25-42) and ResetInlierRatioKernel (:10-23).
 int orc_set_frame(orc_volume* v, const uint16_t* raw, const float* wTc) {
   set_pose(v, wTc);
+  v->has_color = false;
   for (size_t i = 0; i < v->depth.size(); ++i) {
     const float d = v->range_factor * (float)(int16_t)raw[i];
     v->depth[i] = (d >= v->cam.dmin && d <= v->cam.dmax) ? d : -INFINITY;
     v->inlier[i] = 0.1;
   }
+  return 0;
+}
+
+// ConvertColorFormatKernel (sensor/preprocess.cu:44-58): BGRA, valid iff B,
+// G and R are non-zero, channel c -> c / 255.0f (FP32); then the fusion
+// colour of UpdateBlocksSimpleKernel (update_simple.cu:68-70),
+// uchar(255 * x) with truncation clamped to [0, 255] (-inf -> 0).
+int orc_set_color(orc_volume* v, const uint8_t* bgra) {
+  v->color_in.assign(v->depth.size(), 0u);
+  auto conv = [](uint8_t c) -> uint32_t {
+    const float x = (float)c / 255.0f;
+    const int i = f2i(255.0f * x);
+    return (uint32_t)std::max(0, std::min(255, i));
+  };
+  for (size_t i = 0; i < v->depth.size(); ++i) {
+    const uint8_t b = bgra[4 * i], g = bgra[4 * i + 1], r = bgra[4 * i + 2];
+    v->color_in[i] = (b && g && r) ? (conv(r) | (conv(g) << 8) | (conv(b) << 16)) : 0u;
+  }
+  v->has_color = true;
   return 0;
 }
 
@@ -581,7 +603,6 @@ uint32_t orc_collect(orc_volume* v, int frustum) {
 int orc_integrate_tsdf(orc_volume* v) {
   const Geo& g = v->g;
   const Cam& c = v->cam;
-  const uint8_t dcol[3] = {0, 255, 0};
   for (int ptr : v->cand) {
     Block& B = v->blocks[ptr];
     for (int i = 0; i < 512; ++i) {
@@ -603,6 +624,13 @@ int orc_integrate_tsdf(orc_volume* v) {
       B.sdf[i] = (B.sdf[i] * B.w[i] + s * w) / (B.w[i] + w);
       B.w[i] += w;
       ++v->voxel_updates;
+      uint8_t dcol[3] = {0, 255, 0};  // no colour image (update_simple.cu:71-73)
+      if (v->has_color) {
+        const uint32_t cin = v->color_in[vv * c.w + u];
+        dcol[0] = (uint8_t)(cin & 255u);
+        dcol[1] = (uint8_t)((cin >> 8) & 255u);
+        dcol[2] = (uint8_t)((cin >> 16) & 255u);
+      }
       for (int k = 0; k < 3; ++k) {
         const float cc = 0.5f * (float)B.col[i][k] + 0.5f * (float)dcol[k];
         B.col[i][k] = (uint8_t)(cc + 0.5f);

@@ -79,6 +79,7 @@ def lib() -> C.CDLL:
                                               P(C.c_double), P(C.c_int64), P(C.c_double),
                                               P(C.c_double)]),
             "orc_perturb_sdf": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_double]),
+            "orc_set_color": (C.c_int, [vp, P(C.c_uint8)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -123,6 +124,10 @@ class Oracle:
     def set_frame(self, depth_u16, wTc):
         d = np.ascontiguousarray(depth_u16, np.uint16)
         self.L.orc_set_frame(self.h, _p(d, C.c_uint16), _m16(wTc))
+
+    def set_color(self, bgra):
+        c = np.ascontiguousarray(bgra, np.uint8)
+        self.L.orc_set_color(self.h, _p(c, C.c_uint8))
 
     def alloc(self):
         self.L.orc_alloc(self.h)

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from meshhashing_amd.configs import C1, C3
-from meshhashing_amd.engine import Volume, synth_depth, synth_pose
+from meshhashing_amd.engine import Volume, synth_color, synth_depth, synth_pose
 
 pytestmark = pytest.mark.gpu
 
@@ -28,14 +28,23 @@ def rel_err(gpu, ref, scale):
     return np.abs(gpu.astype(np.float64) - ref) / np.maximum(np.abs(ref), scale)
 
 
-def run_frames(cfg, frames, vol, orc, mesh=False):
+def run_frames(cfg, frames, vol, orc, mesh=False, color=False):
     for f in frames:
         depth = synth_depth(cfg, f)
         pose = synth_pose(cfg, f)
         vol.upload_depth(depth, pose)
+        if color:
+            bgra = synth_color(cfg, f)
+            vol.upload_color(bgra)
+            orc.set_frame(depth, pose)
+            orc.set_color(bgra)
+            orc.alloc()
+            orc.collect_visible()
+            orc.integrate_tsdf()
+        else:
+            orc.frame(depth, pose, mesh=False)
         vol.alloc()
         vol.collect_visible()
-        orc.frame(depth, pose, mesh=False)
         if cfg.psdf:
             vol.predict_inlier()
             vol.integrate_psdf()
@@ -108,11 +117,12 @@ def test_alloc_with_linked_lists(hip_device, oracle_mod):
     assert st["list_inserts"] > 0
 
 
-def test_tsdf_fusion_end_to_end(hip_device, oracle_mod):
+@pytest.mark.parametrize("color", [False, True])
+def test_tsdf_fusion_end_to_end(hip_device, oracle_mod, color):
     cfg = C1
     vol = Volume(cfg)
     orc = oracle_mod.Oracle(cfg)
-    run_frames(cfg, range(0, cfg.frames, 2), vol, orc)
+    run_frames(cfg, range(0, cfg.frames, 2), vol, orc, color=color)
     g = vol.dump_blocks()
     o = orc.dump_blocks()
     assert np.array_equal(g["pos"], o["pos"])
@@ -124,6 +134,8 @@ def test_tsdf_fusion_end_to_end(hip_device, oracle_mod):
     assert e_sdf.max() < 1e-5
     assert e_w.max() < 1e-5
     assert np.array_equal(g["color"], o["color"])
+    if color:  # the colour image was used: more than (0, 255, 0) blends
+        assert len(np.unique(g["color"][obs])) > 1000
 
 
 def test_mc_lockstep_given_identical_sdf(hip_device, oracle_mod):

@@ -202,6 +202,53 @@ def test_tsdf_fronto_parallel_plane_closed_form(oracle_mod):
     assert (vz[upd] <= float(Dm) + 0.08 + 1e-6).all()
 
 
+def test_tsdf_colour_closed_form(oracle_mod):
+    """Colour fusion (preprocess.cu:44-58, update_simple.cu:68-73,
+    voxel.h:107-115) on a plane: every updated voxel after two frames of a
+    constant BGRA image holds c2 = uchar(0.5 c1 + 0.5 C + 0.5) with
+    c1 = uchar(0.5 * 0 + 0.5 C + 0.5) and C = uchar(255 * (c / 255)) per
+    channel, all in float32; a pixel with a zero channel contributes 0 and,
+    with no colour image at all, (0, 255, 0)."""
+    cfg = plane_cfg()
+    depth = np.full((cfg.height, cfg.width), 1234, np.uint16)
+    bgra = np.zeros((cfg.height, cfg.width, 4), np.uint8)
+    bgra[...] = (37, 200, 254, 255)  # B, G, R, A
+    bgra[:, : cfg.width // 2, 2] = 0  # left half invalid (R == 0)
+    f32 = np.float32
+
+    def fuse(c):
+        return np.trunc(f32(255) * (f32(c) / f32(255))).astype(np.int64)
+
+    def blend(old, new):
+        return np.floor(f32(0.5) * f32(old) + f32(0.5) * f32(new) + f32(0.5)).astype(np.int64)
+
+    C = np.array([fuse(254), fuse(200), fuse(37)])  # stored R, G, B
+    o = oracle_mod.Oracle(cfg)
+    for _ in range(2):
+        o.set_frame(depth, np.eye(4))
+        o.set_color(bgra)
+        o.alloc()
+        o.collect_visible()
+        o.integrate_tsdf()
+    d = o.dump_blocks()
+    upd = d["weight"] > 0
+    col = d["color"][upd]
+    got = np.stack([col & 255, (col >> 8) & 255, (col >> 16) & 255], -1)
+    valid_want = blend(blend(0, C), C)
+    is_valid = (got == valid_want).all(-1)
+    is_invalid = (got == 0).all(-1)
+    assert (is_valid | is_invalid).all()
+    assert is_valid.sum() > 500 and is_invalid.sum() > 500
+    # no colour image: (0, 255, 0)
+    o2 = oracle_mod.Oracle(cfg)
+    o2.set_frame(depth, np.eye(4))
+    o2.alloc()
+    o2.collect_visible()
+    o2.integrate_tsdf()
+    c2 = o2.dump_blocks()["color"][o2.dump_blocks()["weight"] > 0]
+    assert ((c2 & 255) == 0).all() and (((c2 >> 8) & 255) == blend(0, 255)).all()
+
+
 def test_alloc_covers_dense_samples(oracle_mod):
     """Brute force: every block that a dense sampling of [d - tr, d + tr]
     along each pixel ray touches (and whose centre is in the frustum) is
