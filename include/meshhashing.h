@@ -149,7 +149,10 @@ const char* mh_last_error(void);
  * conversion: d = range_factor * (int16)raw, valid iff min <= d <= max, else
  * -inf), sensor/rgbd_sensor.h:51-54 (cTw = inverse(wTc), FP32).
  * wTc is camera->world, row-major m11..m44.  The PSDF inlier ratio is reset to
- * 0.1 (sensor/preprocess.cu:10-23) by every upload.                        */
+ * 0.1 (sensor/preprocess.cu:10-23) for every frame.  The conversion is fused
+ * into the next mh_alloc_blocks (one pass over the image); a stage that
+ * reads the frame before any allocation converts it first.  A device depth
+ * buffer must therefore stay valid until that stage has run.              */
 int mh_frame_upload_depth_u16(mh_volume* v, const uint16_t* depth_host,
                               const float wTc[16]);
 int mh_frame_depth_u16_device(mh_volume* v, const uint16_t* depth_dev,

@@ -87,6 +87,11 @@ struct mh_volume {
   int cus = 256;
   uint16_t* raw = nullptr;  // staging for host uploads
   uint8_t* raw_color = nullptr;  // staging for host colour uploads (BGRA)
+  // Frame input not converted yet: mh_alloc_blocks converts it inside the
+  // allocation kernel (one pass over the image); any other stage that reads
+  // the frame first runs the standalone conversion (flush_frame).
+  const uint16_t* pending_raw = nullptr;
+  const uint8_t* pending_color = nullptr;
   bool have_frame = false;
   int32_t* pd_nbr = nullptr;
   uint32_t pd_nbr_cap = 0;
@@ -110,6 +115,13 @@ namespace {
 int check_vol(mh_volume* v) {
   if (!v) return fail(MH_ERR_ARG, "null volume");
   return MH_OK;
+}
+
+void flush_frame(mh_volume* v) {
+  if (v->pending_raw) launch_convert_depth(v->V, v->pending_raw, v->stream);
+  if (v->pending_color) launch_convert_color(v->V, v->pending_color, v->stream);
+  v->pending_raw = nullptr;
+  v->pending_color = nullptr;
 }
 
 Counters read_counters(mh_volume* v) {
@@ -385,8 +397,8 @@ int mh_frame_set_pose(mh_volume* v, const float wTc[16]) {
 int mh_frame_depth_u16_device(mh_volume* v, const uint16_t* depth_dev, const float wTc[16]) {
   if (check_vol(v) || !depth_dev || !wTc) return fail(MH_ERR_ARG, "null argument");
   set_pose(v, wTc);
-  launch_convert_depth(v->V, depth_dev, v->stream);
-  HIP_TRY(hipGetLastError());
+  v->pending_raw = depth_dev;
+  v->pending_color = nullptr;
   v->have_frame = true;
   v->V.has_color = 0;
   return MH_OK;
@@ -396,8 +408,12 @@ int mh_frame_color_bgra8_device(mh_volume* v, const uint8_t* bgra_dev) {
   if (check_vol(v) || !bgra_dev) return fail(MH_ERR_ARG, "null argument");
   if ((uintptr_t)bgra_dev % 4) return fail(MH_ERR_ARG, "colour buffer must be 4-byte aligned");
   if (!v->have_frame) return fail(MH_ERR_STATE, "upload the frame's depth first");
-  launch_convert_color(v->V, bgra_dev, v->stream);
-  HIP_TRY(hipGetLastError());
+  if (v->pending_raw) {
+    v->pending_color = bgra_dev;  // converted with the depth
+  } else {
+    launch_convert_color(v->V, bgra_dev, v->stream);
+    HIP_TRY(hipGetLastError());
+  }
   v->V.has_color = 1;
   return MH_OK;
 }
@@ -419,12 +435,16 @@ int mh_frame_upload_depth_u16(mh_volume* v, const uint16_t* depth_host, const fl
 
 int mh_frame_depth_device_ptr(mh_volume* v, const float** depth_dev) {
   if (check_vol(v) || !depth_dev) return fail(MH_ERR_ARG, "null argument");
+  flush_frame(v);
+  HIP_TRY(hipGetLastError());
   *depth_dev = v->V.depth;
   return MH_OK;
 }
 
 int mh_inlier_ratio_device_ptr(mh_volume* v, float** inlier_dev) {
   if (check_vol(v) || !inlier_dev) return fail(MH_ERR_ARG, "null argument");
+  flush_frame(v);
+  HIP_TRY(hipGetLastError());
   *inlier_dev = v->V.inlier;
   return MH_OK;
 }
@@ -432,7 +452,12 @@ int mh_inlier_ratio_device_ptr(mh_volume* v, float** inlier_dev) {
 int mh_alloc_blocks(mh_volume* v) {
   if (check_vol(v)) return MH_ERR_ARG;
   if (!v->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
-  launch_alloc(v->V, v->stream);
+  View Va = v->V;  // the frame's conversion runs inside the allocation kernel
+  Va.raw = v->pending_raw;
+  Va.raw_color = v->pending_color;
+  v->pending_raw = nullptr;
+  v->pending_color = nullptr;
+  launch_alloc(Va, v->stream);
   HIP_TRY(hipGetLastError());
   return MH_OK;
 }
@@ -457,6 +482,7 @@ int mh_collect_all(mh_volume* v, uint32_t* count_host) {
 int mh_integrate_tsdf(mh_volume* v) {
   if (check_vol(v)) return MH_ERR_ARG;
   if (!v->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
+  flush_frame(v);
   launch_integrate_tsdf(v->V, v->stream, v->cus * v->grid_per_cu, v->tiling);
   HIP_TRY(hipGetLastError());
   return MH_OK;
@@ -466,6 +492,7 @@ int mh_predict_inlier(mh_volume* v) {
   if (check_vol(v)) return MH_ERR_ARG;
   if (!v->psdf) return fail(MH_ERR_STATE, "volume created without PSDF fields");
   if (!v->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
+  flush_frame(v);
   launch_predict_inlier(v->V, v->stream, v->grid_wg256());
   HIP_TRY(hipGetLastError());
   return MH_OK;
@@ -475,6 +502,7 @@ int mh_integrate_psdf(mh_volume* v) {
   if (check_vol(v)) return MH_ERR_ARG;
   if (!v->psdf) return fail(MH_ERR_STATE, "volume created without PSDF fields");
   if (!v->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
+  flush_frame(v);
   launch_integrate_psdf(v->V, v->stream, v->grid_wg256());
   HIP_TRY(hipGetLastError());
   return MH_OK;
@@ -1008,8 +1036,7 @@ void batch_free(mh_batch* b) {
 // [marching cubes] -> [raycast], one launch per stage over all scenes.
 void batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   const BatchDev B = b->dev();
-  launch_batch_convert_depth(B, b->stream);
-  launch_batch_alloc(B, b->stream);
+  launch_batch_alloc(B, b->stream);  // converts each scene's raw depth (View.raw) as it goes
   launch_batch_collect(B, 1, b->stream, b->cus * 4);
   if (!integrate) return;
   launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
@@ -1072,6 +1099,7 @@ int mh_set_integrate_tiling(mh_volume* v, int tiling, int grid_per_cu) {
 int mh_tune_integrate(mh_volume* v, uint32_t reps, mh_tune_result* out) {
   if (check_vol(v)) return MH_ERR_ARG;
   if (!v->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
+  flush_frame(v);
   if (reps == 0) reps = 3;
   const uint32_t n = read_counters(v).cand_count;
   if (n == 0) return fail(MH_ERR_EMPTY, "empty worklist");

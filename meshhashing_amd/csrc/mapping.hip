@@ -93,25 +93,21 @@ __device__ __forceinline__ uint32_t fuse_channel(uint32_t c) {
   const int i = f2i(f);
   return (uint32_t)(i < 0 ? 0 : (i > 255 ? 255 : i));
 }
+__device__ __forceinline__ uint32_t fuse_color(uint32_t c) {
+  const uint32_t b = c & 255u, g = (c >> 8) & 255u, r = (c >> 16) & 255u;
+  return (b && g && r) ? (fuse_channel(r) | (fuse_channel(g) << 8) | (fuse_channel(b) << 16)) : 0u;
+}
+// ConvertDepthFormatKernel (sensor/preprocess.cu:25-42).
+__device__ __forceinline__ float convert_depth(uint16_t raw, float range_factor, float dmin, float dmax) {
+  const float d = range_factor * (float)(int16_t)raw;
+  return (d >= dmin && d <= dmax) ? d : -INFINITY;
+}
 
 __global__ void convert_color_kernel(const uint32_t* __restrict__ bgra, uint32_t* __restrict__ out,
                                      uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t c = bgra[i];
-  const uint32_t b = c & 255u, g = (c >> 8) & 255u, r = (c >> 16) & 255u;
-  out[i] = (b && g && r) ? (fuse_channel(r) | (fuse_channel(g) << 8) | (fuse_channel(b) << 16)) : 0u;
-}
-
-// The same for every scene of a batch (scene = blockIdx.y).
-__global__ void batch_convert_depth_kernel(Many src_, const SceneFrame* __restrict__ tab, uint32_t n) {
-  const Many src = with_table(src_, tab);
-  const View& V = src.view(blockIdx.y);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float d = V.range_factor * (float)(int16_t)V.raw[i];
-  V.depth[i] = (d >= V.cam.min_depth && d <= V.cam.max_depth) ? d : -INFINITY;
-  if (V.inlier) V.inlier[i] = 0.1f;
+  out[i] = fuse_color(bgra[i]);
 }
 
 // ------------------------------------------------------------- allocation
@@ -183,7 +179,16 @@ __global__ __launch_bounds__(256) void alloc_kernel(Src src_, const SceneFrame* 
   const uint32_t y = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (x >= V.cam.width || y >= V.cam.height) return;
   const Geo& g = V.geo;
-  const float depth = V.depth[y * V.cam.width + x];
+  const uint32_t pix = y * V.cam.width + x;
+  float depth;
+  if (V.raw) {  // sensor preprocessing fused in: this pixel's conversion
+    depth = convert_depth(V.raw[pix], V.range_factor, V.cam.min_depth, V.cam.max_depth);
+    V.depth[pix] = depth;
+    if (V.inlier) V.inlier[pix] = 0.1f;
+    if (V.raw_color) V.color_in[pix] = fuse_color(reinterpret_cast<const uint32_t*>(V.raw_color)[pix]);
+  } else {
+    depth = V.depth[pix];
+  }
   if (depth == -INFINITY || depth == 0.0f || depth >= g.sdf_upper_bound) return;
 
   const float tr = truncate_distance(g, depth);
@@ -687,11 +692,6 @@ __global__ void batch_reset_counts_kernel(Many src_, const SceneFrame* __restric
   const Many src = with_table(src_, tab);
   for (uint32_t t = threadIdx.x; t < nscenes; t += blockDim.x) src.view(t).ctr->cand_count = 0u;
   if (threadIdx.x == 0) *work_count = 0u;
-}
-
-void launch_batch_convert_depth(const BatchDev& B, hipStream_t s) {
-  const uint32_t n = B.width * B.height;
-  batch_convert_depth_kernel<<<dim3((n + 255) / 256, B.nscenes), 256, 0, s>>>(B.src(), B.frames, n);
 }
 
 void launch_batch_alloc(const BatchDev& B, hipStream_t s) {

@@ -255,7 +255,8 @@ struct View {
   Cam cam;
   float range_factor;
   float* depth;          // W*H metric depth of the current frame
-  const uint16_t* raw;   // W*H raw depth of the current frame (batched conversion)
+  const uint16_t* raw;   // W*H raw depth not yet converted (alloc converts it), or null
+  const uint8_t* raw_color;  // W*H BGRA colour not yet converted (alloc converts it), or null
   uint32_t* color_in;    // W*H fusion colour of the current frame, R | G << 8 | B << 16
   int has_color;         // the current frame has a colour image
   float* inlier;         // W*H PSDF inlier ratio

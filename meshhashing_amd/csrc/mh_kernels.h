@@ -126,7 +126,6 @@ void launch_state_save(const View& V, float* sdf, float* weight, uint32_t* color
 void launch_state_restore(const View& V, const float* sdf, const float* weight,
                           const uint32_t* color, hipStream_t s, int grid);
 
-void launch_batch_convert_depth(const BatchDev& B, hipStream_t s);
 void launch_batch_alloc(const BatchDev& B, hipStream_t s);
 void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid);
 void launch_batch_integrate_tsdf(const BatchDev& B, hipStream_t s, int grid, int tiling);

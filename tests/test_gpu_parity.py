@@ -138,6 +138,35 @@ def test_tsdf_fusion_end_to_end(hip_device, oracle_mod, color):
         assert len(np.unique(g["color"][obs])) > 1000
 
 
+def test_frame_conversion_without_allocation(hip_device, oracle_mod):
+    """The frame's depth / colour conversion normally runs inside the
+    allocation kernel; a frame fused without an allocation pass (worklist
+    of the blocks already allocated) is converted by the standalone path and
+    must give the same result as the oracle."""
+    cfg = C1
+    vol = Volume(cfg)
+    orc = oracle_mod.Oracle(cfg)
+    run_frames(cfg, [0], vol, orc, color=True)
+    for f in (2, 4):
+        depth, pose, bgra = synth_depth(cfg, f), synth_pose(cfg, f), synth_color(cfg, f)
+        vol.upload_depth(depth, pose)
+        vol.upload_color(bgra)
+        vol.collect_visible()
+        vol.integrate_tsdf()
+        orc.set_frame(depth, pose)
+        orc.set_color(bgra)
+        orc.collect_visible()
+        orc.integrate_tsdf()
+    g, o = vol.dump_blocks(), orc.dump_blocks()
+    assert np.array_equal(g["pos"], o["pos"])
+    obs = o["weight"] > 0
+    assert np.array_equal(g["weight"] > 0, obs)
+    assert rel_err(g["sdf"][obs], o["sdf"][obs], cfg.truncation).max() < 1e-5
+    assert np.array_equal(g["color"], o["color"])
+    depth_dev = vol.depth_ptr()  # converted depth of the last frame
+    assert depth_dev
+
+
 def test_mc_lockstep_given_identical_sdf(hip_device, oracle_mod):
     """Per frame, the oracle takes the GPU's fused SDF, then both mesh the
     same worklist incrementally; topology and vertex positions must agree
