@@ -277,6 +277,23 @@ typedef struct mh_tune_result {
 int mh_tune_integrate(mh_volume* v, uint32_t reps, mh_tune_result* out);
 int mh_set_integrate_tiling(mh_volume* v, int tiling, int workgroups_per_cu);
 
+/* ---- mesh compaction and PLY output -------------------------------------
+ * CompressMesh (visualization/compress_mesh.cu:137-172) over the current
+ * worklist (mh_collect_all + mh_marching_cubes first for the whole mesh, as
+ * MainEngine::FinalLog, engine/main_engine.cc:299-311): every live triangle
+ * whose three vertex slots are allocated, with the vertices they use
+ * renumbered densely.  Triangles come in worklist x voxel x table-slot order,
+ * vertices in vertex-pool order.  Synchronises the volume's stream.       */
+int mh_compress_mesh(mh_volume* v, uint32_t* vertex_count, uint32_t* triangle_count);
+/* The compacted mesh: 3 floats per vertex (position, normal, colour) and 3
+ * int32 vertex indices per triangle; any buffer may be NULL.              */
+int mh_compact_mesh_get(mh_volume* v, uint32_t vertex_capacity, float* pos_host,
+                        float* normal_host, float* color_host,
+                        uint32_t triangle_capacity, int32_t* tri_host);
+/* SavePly (io/mesh_writer.cc:75-150): ASCII PLY of the compacted mesh.
+ * Colour channels are int(255 c), with a NaN channel written as 0.        */
+int mh_save_ply(mh_volume* v, const char* path);
+
 /* ---- differentiable raycast (no counterpart in the reference) ------------
  * Loss L = 1/2 sum over pixels of (depth - obs)^2, over the pixels where
  * mh_raycast (same params and cTw) hits and obs_depth_dev (W*H metres,

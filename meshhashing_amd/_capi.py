@@ -152,6 +152,9 @@ SIGNATURES = {
     "mh_dump_cube_indices": (C.c_int, [VP, C.c_uint32, P_U16, P_U32]),
     "mh_dump_primal_dual": (C.c_int, [VP, C.c_uint32, P_F32, P_U8, P_U32]),
     "mh_frame_get_pose": (C.c_int, [VP, P_F32, P_F32]),
+    "mh_compress_mesh": (C.c_int, [VP, P_U32, P_U32]),
+    "mh_compact_mesh_get": (C.c_int, [VP, C.c_uint32, P_F32, P_F32, P_F32, C.c_uint32, P_I32]),
+    "mh_save_ply": (C.c_int, [VP, C.c_char_p]),
     "mh_raycast_backward": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, VP, VP,
                                       C.POINTER(RaycastGrad)]),
     "mh_refine_pose": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, VP, C.c_uint32, P_F32,

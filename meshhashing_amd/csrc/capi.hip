@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,18 @@ struct mh_volume {
   double* pd_energy = nullptr;
   uint32_t pd_npartials = 0;
   uint32_t mc_pass = 0;
+  // mesh compaction (allocated on first use)
+  uint32_t* cm_used = nullptr;    // per vertex slot: used by a live triangle
+  uint32_t* cm_remap = nullptr;   // per vertex slot: compact index
+  uint32_t* cm_blk = nullptr;     // per worklist block: triangle count, then offset
+  uint32_t* cm_blk_off = nullptr;
+  uint32_t* cm_sums = nullptr;    // scan chunk sums
+  uint32_t* cm_total = nullptr;   // 2 totals: triangles, vertices
+  float* cm_pos = nullptr;
+  float* cm_nrm = nullptr;
+  float* cm_col = nullptr;
+  int32_t* cm_tri = nullptr;
+  uint32_t cm_vertices = 0, cm_triangles = 0;
   double* rgrad_partials = nullptr;     // raycast adjoint: per-workgroup sums
   uint32_t rgrad_cap = 0;
   double* rgrad_out = nullptr;
@@ -1532,6 +1545,125 @@ int mh_dump_voxel_field(mh_volume* v, const float* field_dev, uint32_t capacity,
   if (!out_host) return MH_OK;
   if (capacity < blocks.size()) return fail(MH_ERR_CAPACITY, "buffer too small");
   return gather_field(v, field_dev, blocks, read_counters(v).block_hwm, 1, out_host);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ mesh compaction
+namespace {
+
+int ensure_compact(mh_volume* v) {
+  if (v->cm_used) return MH_OK;
+  const size_t nv = v->mp.max_vertex_count, cap = v->hp.value_capacity;
+  const size_t nt = v->mp.max_triangle_count;
+  const size_t chunks = std::max((nv + 1023) / 1024, (cap + 1023) / 1024) + 1;
+  auto A = [&](auto** p, size_t n) -> int {
+    HIP_TRY(dalloc(p, n));
+    v->owned.push_back((void*)*p);
+    return MH_OK;
+  };
+  int rc = 0;
+  if ((rc = A(&v->cm_used, nv)) || (rc = A(&v->cm_remap, nv)) || (rc = A(&v->cm_blk, cap)) ||
+      (rc = A(&v->cm_blk_off, cap)) || (rc = A(&v->cm_sums, chunks)) || (rc = A(&v->cm_total, 2)) ||
+      (rc = A(&v->cm_pos, 3 * nv)) || (rc = A(&v->cm_nrm, 3 * nv)) || (rc = A(&v->cm_col, 3 * nv)) ||
+      (rc = A(&v->cm_tri, 3 * nt)))
+    return rc;
+  return MH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mh_compress_mesh(mh_volume* v, uint32_t* vertex_count, uint32_t* triangle_count) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  int rc = ensure_compact(v);
+  if (rc) return rc;
+  const uint32_t n = read_counters(v).cand_count;
+  const uint32_t nv = v->mp.max_vertex_count;
+  HIP_TRY(hipMemsetAsync(v->cm_used, 0, (size_t)nv * sizeof(uint32_t), v->stream));
+  const int grid = v->grid_wg512();
+  launch_compact_count(v->V, n, v->cm_used, v->cm_blk, v->stream, grid);
+  launch_scan(v->cm_blk, v->cm_blk_off, n, v->cm_sums, v->cm_total + 0, v->stream);
+  launch_scan(v->cm_used, v->cm_remap, nv, v->cm_sums, v->cm_total + 1, v->stream);
+  HIP_TRY(hipGetLastError());
+  uint32_t tot[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(tot, v->cm_total, sizeof(tot), hipMemcpyDeviceToHost, v->stream));
+  HIP_TRY(hipStreamSynchronize(v->stream));
+  if (tot[0] > v->mp.max_triangle_count)
+    return fail(MH_ERR_CAPACITY, "more live triangles than max_triangle_count");
+  launch_compact_emit(v->V, n, v->cm_blk_off, v->cm_remap, v->cm_tri, v->cm_used, v->cm_pos,
+                      v->cm_nrm, v->cm_col, v->stream, grid);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(v->stream));
+  v->cm_triangles = tot[0];
+  v->cm_vertices = tot[1];
+  if (vertex_count) *vertex_count = tot[1];
+  if (triangle_count) *triangle_count = tot[0];
+  return MH_OK;
+}
+
+int mh_compact_mesh_get(mh_volume* v, uint32_t vertex_capacity, float* pos_host, float* normal_host,
+                        float* color_host, uint32_t triangle_capacity, int32_t* tri_host) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  if (!v->cm_used) return fail(MH_ERR_STATE, "mh_compress_mesh first");
+  if ((pos_host || normal_host || color_host) && vertex_capacity < v->cm_vertices)
+    return fail(MH_ERR_CAPACITY, "vertex buffers too small");
+  if (tri_host && triangle_capacity < v->cm_triangles)
+    return fail(MH_ERR_CAPACITY, "triangle buffer too small");
+  const size_t vb = (size_t)v->cm_vertices * 3 * sizeof(float);
+  if (pos_host && vb) HIP_TRY(hipMemcpy(pos_host, v->cm_pos, vb, hipMemcpyDeviceToHost));
+  if (normal_host && vb) HIP_TRY(hipMemcpy(normal_host, v->cm_nrm, vb, hipMemcpyDeviceToHost));
+  if (color_host && vb) HIP_TRY(hipMemcpy(color_host, v->cm_col, vb, hipMemcpyDeviceToHost));
+  const size_t tb = (size_t)v->cm_triangles * 3 * sizeof(int32_t);
+  if (tri_host && tb) HIP_TRY(hipMemcpy(tri_host, v->cm_tri, tb, hipMemcpyDeviceToHost));
+  return MH_OK;
+}
+
+// SavePly (io/mesh_writer.cc:75-150): ASCII PLY, vertex x y z nx ny nz and
+// colour int(255 * c) per channel, faces "3 i j k"; numbers as the default
+// C++ stream formatting.  int() of a NaN colour (plain-TSDF vertices carry
+// ValToRGB of 0/0) is undefined in C++; here it is 0, other values saturate.
+static int ply_channel(float c) {
+  const float x = 255.0f * c;
+  if (!(x == x)) return 0;
+  if (x >= 2147483647.0f) return 2147483647;
+  if (x <= -2147483648.0f) return -2147483647 - 1;
+  return (int)x;
+}
+int mh_save_ply(mh_volume* v, const char* path) {
+  if (check_vol(v) || !path) return fail(MH_ERR_ARG, "null argument");
+  if (!v->cm_used) return fail(MH_ERR_STATE, "mh_compress_mesh first");
+  const uint32_t nv = v->cm_vertices, nt = v->cm_triangles;
+  std::vector<float> pos((size_t)3 * nv), nrm((size_t)3 * nv), col((size_t)3 * nv);
+  std::vector<int32_t> tri((size_t)3 * nt);
+  const int rc = mh_compact_mesh_get(v, nv, pos.data(), nrm.data(), col.data(), nt, tri.data());
+  if (rc) return rc;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(MH_ERR_ARG, std::string("cannot open ") + path);
+  std::ostringstream ss;
+  ss << "ply\nformat ascii 1.0\n";
+  ss << "element vertex " << nv << "\n";
+  ss << "property float x\nproperty float y\nproperty float z\n"
+        "property float nx\nproperty float ny\nproperty float nz\n"
+        "property uchar red\nproperty uchar green\nproperty uchar blue\n";
+  ss << "element face " << nt << "\n";
+  ss << "property list uchar int vertex_index\nend_header\n";
+  for (uint32_t i = 0; i < nv; ++i) {
+    const float* p = &pos[3 * (size_t)i];
+    const float* n = &nrm[3 * (size_t)i];
+    const float* c = &col[3 * (size_t)i];
+    ss << p[0] << " " << p[1] << " " << p[2] << " " << n[0] << " " << n[1] << " " << n[2] << " "
+       << ply_channel(c[0]) << " " << ply_channel(c[1]) << " " << ply_channel(c[2]) << "\n";
+  }
+  for (uint32_t i = 0; i < nt; ++i)
+    ss << "3 " << tri[3 * (size_t)i] << " " << tri[3 * (size_t)i + 1] << " " << tri[3 * (size_t)i + 2]
+       << "\n";
+  const std::string out = ss.str();
+  const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+  std::fclose(f);
+  if (!ok) return fail(MH_ERR_ARG, std::string("write failed: ") + path);
+  return MH_OK;
 }
 
 }  // extern "C"

@@ -506,6 +506,202 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
   }
 }
 
+// ---------------------------------------------------------------- compaction
+// CompressMesh (visualization/compress_mesh.cu:10-172) over the implicit
+// mesh: the live triangles of a worklist block are its cubes' table rows;
+// a triangle is kept when all three vertex slots are allocated.  Pass 1
+// counts triangles per block and marks the vertex slots they use; after
+// exclusive scans (triangles over the worklist, used slots over the vertex
+// pool) pass 2 writes triangles in worklist x voxel x slot order with the
+// vertices renumbered in pool order, and the used vertices are gathered.
+
+// Vertex pool index of edge e of the cube at local voxel o, -1 if its slot is
+// not allocated.  nb holds the blocks at offsets {0, 1}^3 (x + 2y + 4z).
+__device__ __forceinline__ int32_t cube_edge_vertex(const View& V, const int32_t* nb, i3 o, int e) {
+  const int ox = o.x + edge_owner_off(e, 0), oy = o.y + edge_owner_off(e, 1), oz = o.z + edge_owner_off(e, 2);
+  const int dx = ox >> 3, dy = oy >> 3, dz = oz >> 3;
+  const int32_t bp = nb[dx + 2 * dy + 4 * dz];
+  if (bp < 0) return -1;
+  const uint32_t li = (uint32_t)((ox - 8 * dx) + 8 * (oy - 8 * dy) + 64 * (oz - 8 * dz));
+  return V.vptr[((size_t)bp * kBlockVoxels + li) * 3 + edge_axis(e)];
+}
+
+// Exclusive scan of one value per lane over a 512-lane workgroup.
+__device__ __forceinline__ uint32_t block_excl_scan512(uint32_t v, uint32_t* wave_sums, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  if (lane == 63) wave_sums[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t w = 0; w < 8; ++w) {
+    const uint32_t ws = wave_sums[w];
+    if (w < wave) before += ws;
+    all += ws;
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+template <bool EMIT>
+__global__ __launch_bounds__(512) void compact_tris_kernel(View V, uint32_t n, uint32_t* __restrict__ used,
+                                                           uint32_t* __restrict__ blk_tris,
+                                                           const uint32_t* __restrict__ blk_off,
+                                                           const uint32_t* __restrict__ remap,
+                                                           int32_t* __restrict__ tri_out) {
+  __shared__ int32_t nb[8];
+  __shared__ uint32_t wave_sums[8];
+  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
+    const int32_t ptr = V.cand[ci];
+    const i3 b = unpack_key(V.block_keys[ptr]);
+    if (threadIdx.x < 8) {
+      const int t = (int)threadIdx.x;
+      nb[t] = t == 0 ? ptr : hash_find(V, i3{b.x + (t & 1), b.y + ((t >> 1) & 1), b.z + ((t >> 2) & 1)});
+    }
+    __syncthreads();
+    const uint32_t i = threadIdx.x;
+    const i3 o = devectorize(i);
+    const int c = V.cube[(size_t)ptr * kBlockVoxels + i] & 255;
+    const int nt = (c == 0 || c == 255) ? 0 : kMc.ntri[c];
+    uint32_t keep = 0;
+    for (int t = 0; t < nt; ++t) {
+      int32_t vp[3];
+      bool ok = true;
+      for (int k = 0; k < 3; ++k) {
+        vp[k] = cube_edge_vertex(V, nb, o, kMc.tri[c][3 * t + k]);
+        ok = ok && vp[k] >= 0;
+      }
+      if (!ok) continue;
+      if (!EMIT) {
+        for (int k = 0; k < 3; ++k) used[vp[k]] = 1u;
+      } else {
+        // slot within the block filled below after the scan
+      }
+      ++keep;
+    }
+    uint32_t total;
+    const uint32_t excl = block_excl_scan512(keep, wave_sums, &total);
+    if (!EMIT) {
+      if (threadIdx.x == 0) blk_tris[ci] = total;
+    } else {
+      uint32_t at = blk_off[ci] + excl;
+      for (int t = 0; t < nt; ++t) {
+        int32_t vp[3];
+        bool ok = true;
+        for (int k = 0; k < 3; ++k) {
+          vp[k] = cube_edge_vertex(V, nb, o, kMc.tri[c][3 * t + k]);
+          ok = ok && vp[k] >= 0;
+        }
+        if (!ok) continue;
+        for (int k = 0; k < 3; ++k) tri_out[3 * (size_t)at + k] = (int32_t)remap[vp[k]];
+        ++at;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void compact_vertices_kernel(View V, const uint32_t* __restrict__ used,
+                                                               const uint32_t* __restrict__ remap,
+                                                               float* __restrict__ pos, float* __restrict__ nrm,
+                                                               float* __restrict__ col, uint32_t nv) {
+  for (uint32_t vp = blockIdx.x * blockDim.x + threadIdx.x; vp < nv; vp += gridDim.x * blockDim.x) {
+    if (!used[vp]) continue;
+    const size_t j = remap[vp];
+    const Vertex& x = V.vertices[vp];
+    pos[3 * j + 0] = x.px; pos[3 * j + 1] = x.py; pos[3 * j + 2] = x.pz;
+    nrm[3 * j + 0] = x.nx; nrm[3 * j + 1] = x.ny; nrm[3 * j + 2] = x.nz;
+    col[3 * j + 0] = x.cr; col[3 * j + 1] = x.cg; col[3 * j + 2] = x.cb;
+  }
+}
+
+// Exclusive scan of n uint32 (three launches: 1024-element chunks, the chunk
+// sums in one workgroup, then the chunk offsets added back).
+constexpr uint32_t kScanChunk = 1024;
+__global__ __launch_bounds__(1024) void scan_chunks_kernel(const uint32_t* __restrict__ in,
+                                                           uint32_t* __restrict__ out, uint32_t n,
+                                                           uint32_t* __restrict__ chunk_sums) {
+  __shared__ uint32_t ws[16];
+  const uint32_t i = blockIdx.x * kScanChunk + threadIdx.x;
+  const uint32_t v = i < n ? in[i] : 0u;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  if (lane == 63) ws[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t w = 0; w < 16; ++w) {
+    if (w < wave) before += ws[w];
+    all += ws[w];
+  }
+  if (i < n) out[i] = before + x - v;
+  if (threadIdx.x == 0) chunk_sums[blockIdx.x] = all;
+}
+
+__global__ __launch_bounds__(1024) void scan_sums_kernel(uint32_t* __restrict__ sums, uint32_t nchunks,
+                                                         uint32_t* __restrict__ total) {
+  __shared__ uint32_t ws[16];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  for (uint32_t base = 0; base < nchunks; base += kScanChunk) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < nchunks ? sums[i] : 0u;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off);
+      if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63) ws[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t w = 0; w < 16; ++w) {
+      if (w < wave) before += ws[w];
+      all += ws[w];
+    }
+    const uint32_t c = carry;
+    if (i < nchunks) sums[i] = c + before + x - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry = c + all;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(1024) void scan_add_kernel(uint32_t* __restrict__ out, uint32_t n,
+                                                        const uint32_t* __restrict__ sums) {
+  const uint32_t i = blockIdx.x * kScanChunk + threadIdx.x;
+  if (i < n) out[i] += sums[blockIdx.x];
+}
+
+void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* chunk_sums,
+                 uint32_t* total, hipStream_t s) {
+  const uint32_t nchunks = n ? (n + kScanChunk - 1) / kScanChunk : 1;
+  scan_chunks_kernel<<<nchunks, kScanChunk, 0, s>>>(in, out, n, chunk_sums);
+  scan_sums_kernel<<<1, kScanChunk, 0, s>>>(chunk_sums, nchunks, total);
+  scan_add_kernel<<<nchunks, kScanChunk, 0, s>>>(out, n, chunk_sums);
+}
+
+void launch_compact_count(const View& V, uint32_t n, uint32_t* used, uint32_t* blk_tris,
+                          hipStream_t s, int grid) {
+  compact_tris_kernel<false><<<grid, 512, 0, s>>>(V, n, used, blk_tris, nullptr, nullptr, nullptr);
+}
+
+void launch_compact_emit(const View& V, uint32_t n, const uint32_t* blk_off, const uint32_t* remap,
+                         int32_t* tri_out, const uint32_t* used, float* pos, float* nrm, float* col,
+                         hipStream_t s, int grid) {
+  compact_tris_kernel<true><<<grid, 512, 0, s>>>(V, n, nullptr, nullptr, blk_off, remap, tri_out);
+  compact_vertices_kernel<<<grid, 256, 0, s>>>(V, used, remap, pos, nrm, col, V.max_vertices);
+}
+
 void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int grid) {
   mc_mark_kernel<<<grid / 4, 256, 0, s>>>(One{V}, nullptr, p);
   if (V.va)

@@ -133,6 +133,15 @@ void launch_batch_integrate_tsdf(const BatchDev& B, hipStream_t s, int grid, int
 // meshing.hip
 void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int grid);
 void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int grid);
+// mesh compaction (CompressMesh) and an exclusive scan of n uint32 (chunk_sums:
+// ceil(n / 1024) entries)
+void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* chunk_sums,
+                 uint32_t* total, hipStream_t s);
+void launch_compact_count(const View& V, uint32_t n, uint32_t* used, uint32_t* blk_tris,
+                          hipStream_t s, int grid);
+void launch_compact_emit(const View& V, uint32_t n, const uint32_t* blk_off, const uint32_t* remap,
+                         int32_t* tri_out, const uint32_t* used, float* pos, float* nrm, float* col,
+                         hipStream_t s, int grid);
 
 // raycast.hip
 void launch_raycast(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,

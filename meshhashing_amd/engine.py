@@ -182,6 +182,23 @@ class Volume:
                 "mh_refine_pose")
         return np.array(out[:], np.float32).reshape(4, 4), trace
 
+    def compress_mesh(self) -> dict:
+        """CompressMesh over the current worklist: dense vertices (pos,
+        normal, colour) and triangles (vertex indices)."""
+        nv, nt = C.c_uint32(0), C.c_uint32(0)
+        A.check(self.lib.mh_compress_mesh(self.h, C.byref(nv), C.byref(nt)), "mh_compress_mesh")
+        pos = np.zeros((nv.value, 3), np.float32)
+        nrm = np.zeros((nv.value, 3), np.float32)
+        col = np.zeros((nv.value, 3), np.float32)
+        tri = np.zeros((nt.value, 3), np.int32)
+        A.check(self.lib.mh_compact_mesh_get(self.h, nv.value, A.ptr(pos, C.c_float),
+                                             A.ptr(nrm, C.c_float), A.ptr(col, C.c_float), nt.value,
+                                             A.ptr(tri, C.c_int32)), "mh_compact_mesh_get")
+        return {"pos": pos, "normal": nrm, "color": col, "tri": tri}
+
+    def save_ply(self, path: str):
+        A.check(self.lib.mh_save_ply(self.h, str(path).encode()), "mh_save_ply")
+
     def dump_voxel_field(self, field_ptr: int) -> np.ndarray:
         n = C.c_uint32(0)
         A.check(self.lib.mh_dump_voxel_field(self.h, C.c_void_p(field_ptr), 0, None, C.byref(n)),
