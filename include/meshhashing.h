@@ -186,6 +186,27 @@ int mh_integrate_psdf(mh_volume* v);
 /* Inlier ratio image of the current frame (device, W*H floats). */
 int mh_inlier_ratio_device_ptr(mh_volume* v, float** inlier_dev);
 
+/* ---- block recycling: mapping/recycle.h:11-35 ---------------------------
+ * Each works on the current worklist, as the reference (MainEngine::Meshing
+ * and ::Recycle, engine/main_engine.cc:156-192).
+ * CollectLowSurfelBlocks (recycle.cu:81-115): a block with < 10 inner and
+ *   no boundary surfels (counted by the last mh_marching_cubes) ages by one,
+ *   others are renewed (life 3, core/block.h:13); a block whose life is over
+ *   is flagged when no allocated face neighbour is still alive.  Ageing and
+ *   flagging are separate passes (deterministic).
+ * StarveOccupiedBlockArray (recycle.cu:15-24): weight -= 1, floored at 0.
+ * CollectGarbageBlockArray (recycle.cu:27-77): flag blocks whose observed
+ *   voxels all have |sdf| >= the truncation at 5 m, or with none observed.
+ * RecycleGarbageBlockArray (recycle.cu:117-179, 231-260): flagged blocks
+ *   lose their triangles and the vertices they own left unreferenced; a
+ *   flagged block with no vertex left is removed from the hash table
+ *   (HashTable::FreeEntry, hash_table.h:183-255) and cleared.  freed_host
+ *   (may be NULL, synchronises): blocks freed.                            */
+int mh_collect_low_surfel_blocks(mh_volume* v);
+int mh_starve_blocks(mh_volume* v);
+int mh_collect_garbage_blocks(mh_volume* v);
+int mh_recycle_garbage_blocks(mh_volume* v, uint32_t* freed_host);
+
 /* ---- meshing: MarchingCubes, meshing/marching_cubes.h:18-24 ------------- */
 int mh_marching_cubes(mh_volume* v, const mh_mc_params* p);
 
@@ -229,6 +250,10 @@ int mh_dump_hash(mh_volume* v, uint32_t capacity, int32_t* rec_host,
 /* Current worklist as block positions (3 int32 each), sorted.              */
 int mh_dump_candidates(mh_volume* v, uint32_t capacity, int32_t* pos_host,
                        uint32_t* count_host);
+/* Flags of the last low-surfel or garbage pass (EntryArray::flag,
+ * core/entry_array.h:29-31) as (x, y, z, flag) int32 rows in position order.     */
+int mh_dump_candidate_flags(mh_volume* v, uint32_t capacity, int32_t* rec_host,
+                            uint32_t* count_host);
 /* Mesh in canonical form.  Triangles: 12 int32 each -- the owner voxel
  * (x, y, z) and edge axis of each of its three vertices, in table order;
  * a vertex whose slot is not allocated has x = y = z = INT32_MIN --

@@ -135,6 +135,10 @@ SIGNATURES = {
     "mh_collect_visible": (C.c_int, [VP, P_U32]),
     "mh_collect_all": (C.c_int, [VP, P_U32]),
     "mh_integrate_tsdf": (C.c_int, [VP]),
+    "mh_collect_low_surfel_blocks": (C.c_int, [VP]),
+    "mh_starve_blocks": (C.c_int, [VP]),
+    "mh_collect_garbage_blocks": (C.c_int, [VP]),
+    "mh_recycle_garbage_blocks": (C.c_int, [VP, P_U32]),
     "mh_predict_inlier": (C.c_int, [VP]),
     "mh_integrate_psdf": (C.c_int, [VP]),
     "mh_inlier_ratio_device_ptr": (C.c_int, [VP, C.POINTER(VP)]),
@@ -147,6 +151,7 @@ SIGNATURES = {
     "mh_load_blocks": (C.c_int, [VP, C.c_uint32, P_I32, P_F32, P_F32, P_U32, P_F32, P_F32]),
     "mh_dump_hash": (C.c_int, [VP, C.c_uint32, P_I32, P_U32]),
     "mh_dump_candidates": (C.c_int, [VP, C.c_uint32, P_I32, P_U32]),
+    "mh_dump_candidate_flags": (C.c_int, [VP, C.c_uint32, P_I32, P_U32]),
     "mh_dump_mesh": (C.c_int, [VP, C.c_uint32, P_I32, P_I32, C.c_uint32, P_I32, P_F32, P_I32,
                                P_U32, P_U32]),
     "mh_dump_cube_indices": (C.c_int, [VP, C.c_uint32, P_U16, P_U32]),

@@ -1,6 +1,7 @@
 // capi.hip -- the C ABI (include/meshhashing.h): volume lifecycle, frame
 // upload, stage launchers, canonical dumps and restore.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -315,6 +316,10 @@ int mh_volume_create(const mh_hash_params* hash, const mh_volume_params* vol,
   A(&V.vertices, mesh->max_vertex_count);
   A(&V.vertex_heap, mesh->max_vertex_count);
   A(&V.cand, cap);
+  A(&V.cand_flag, cap);
+  A(&V.life, cap);
+  A(&V.gc_list, cap);
+  A(&V.gc_free, cap);
   A(&V.depth, npix);
   A(&V.inlier, npix);
   A(&V.color_in, npix);
@@ -490,6 +495,38 @@ int mh_collect_visible(mh_volume* v, uint32_t* count_host) {
 int mh_collect_all(mh_volume* v, uint32_t* count_host) {
   if (check_vol(v)) return MH_ERR_ARG;
   return collect(v, 0, count_host);
+}
+
+int mh_collect_low_surfel_blocks(mh_volume* v) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  launch_gc_low_surfel(v->V, v->stream, v->cus * 4);
+  HIP_TRY(hipGetLastError());
+  return MH_OK;
+}
+
+int mh_starve_blocks(mh_volume* v) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  launch_gc_starve(v->V, v->stream, v->grid_wg256());
+  HIP_TRY(hipGetLastError());
+  return MH_OK;
+}
+
+int mh_collect_garbage_blocks(mh_volume* v) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  launch_gc_flag_garbage(v->V, v->stream, v->grid_wg256());
+  HIP_TRY(hipGetLastError());
+  return MH_OK;
+}
+
+int mh_recycle_garbage_blocks(mh_volume* v, uint32_t* freed_host) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  launch_gc_recycle(v->V, v->stream, v->grid_wg256());
+  HIP_TRY(hipGetLastError());
+  if (freed_host) {
+    Counters c = read_counters(v);
+    *freed_host = c.gc_free_count;
+  }
+  return MH_OK;
 }
 
 int mh_integrate_tsdf(mh_volume* v) {
@@ -722,6 +759,33 @@ int mh_dump_candidates(mh_volume* v, uint32_t capacity, int32_t* pos_host, uint3
     pos_host[3 * i + 1] = pos[i].y;
     pos_host[3 * i + 2] = pos[i].z;
   }
+  return MH_OK;
+}
+
+int mh_dump_candidate_flags(mh_volume* v, uint32_t capacity, int32_t* rec_host, uint32_t* count_host) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  HIP_TRY(hipStreamSynchronize(v->stream));
+  const Counters c = read_counters(v);
+  std::vector<int32_t> cand(c.cand_count);
+  std::vector<uint8_t> flag(c.cand_count);
+  if (c.cand_count) {
+    HIP_TRY(hipMemcpy(cand.data(), v->V.cand, cand.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(flag.data(), v->V.cand_flag, flag.size(), hipMemcpyDeviceToHost));
+  }
+  std::vector<uint64_t> keys(c.block_hwm);
+  if (c.block_hwm)
+    HIP_TRY(hipMemcpy(keys.data(), v->V.block_keys, keys.size() * sizeof(uint64_t),
+                      hipMemcpyDeviceToHost));
+  std::vector<std::array<int32_t, 4>> rec;
+  for (size_t k = 0; k < cand.size(); ++k) {
+    const i3 p = unpack_key(keys[cand[k]]);
+    rec.push_back({p.x, p.y, p.z, (int32_t)flag[k]});
+  }
+  std::sort(rec.begin(), rec.end());
+  if (count_host) *count_host = (uint32_t)rec.size();
+  if (!rec_host) return MH_OK;
+  if (capacity < rec.size()) return fail(MH_ERR_CAPACITY, "dump buffer too small");
+  std::memcpy(rec_host, rec.data(), rec.size() * sizeof(rec[0]));
   return MH_OK;
 }
 

@@ -25,6 +25,7 @@ __global__ void reset_pool_kernel(View V) {
     V.block_heap[i] = (int32_t)(cap - 1u - i);
     V.block_keys[i] = kEmptyKey;
     V.block_mark[i] = 0u;
+    V.life[i] = kBlockLife;
     V.surfel[2 * i] = 0;
     V.surfel[2 * i + 1] = 0;
   }
@@ -335,7 +336,10 @@ __global__ __launch_bounds__(256) void collect_kernel(Src src_, const SceneFrame
       if (work) gout = atomicAdd(work_count, cnt);
     }
     out = __shfl(out, 0);
-    if (take) V.cand[out + prefix] = (int32_t)p;
+    if (take) {
+      V.cand[out + prefix] = (int32_t)p;
+      V.cand_flag[out + prefix] = 0;  // no recycling flag outlives its worklist
+    }
     if (work) {
       gout = __shfl(gout, 0);
       if (take) work[gout + prefix] = (scene << kSceneShift) | p;
@@ -627,6 +631,242 @@ void launch_state_restore(const View& V, const float* sdf, const float* weight,
                           const uint32_t* color, hipStream_t s, int grid) {
   state_copy_kernel<<<grid, 256, 0, s>>>(V, const_cast<float*>(sdf), const_cast<float*>(weight),
                                          const_cast<uint32_t*>(color), 1);
+}
+
+// ------------------------------------------------------------ block recycling
+// CollectLowSurfelBlocksKernel (mapping/recycle.cu:81-115) in two passes: a
+// block with fewer than 10 inner and no boundary surfels ages by one frame,
+// any other is renewed; then a block whose life is over is flagged when no
+// allocated face neighbour is still alive.  (The reference ages and reads
+// the neighbours in one launch, so a neighbour may be read before or after
+// its own update; here every block reads the updated values.)
+__global__ void gc_life_kernel(View V) {
+  const uint32_t n = V.ctr->cand_count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int32_t p = V.cand[i];
+    const bool low = V.surfel[2 * p] < 10 && V.surfel[2 * p + 1] == 0;
+    V.life[p] = low ? V.life[p] - 1 : kBlockLife;
+  }
+}
+
+__global__ void gc_low_surfel_flag_kernel(View V) {
+  const uint32_t n = V.ctr->cand_count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int32_t p = V.cand[i];
+    uint8_t f = 0;
+    if (V.life[p] <= 0) {
+      f = 1;
+      const i3 b = unpack_key(V.block_keys[p]);
+      const i3 off[6] = {{0, 0, 1}, {0, 0, -1}, {0, 1, 0}, {0, -1, 0}, {1, 0, 0}, {-1, 0, 0}};
+      for (int j = 0; j < 6; ++j) {
+        const int32_t q = hash_find(V, i3{b.x + off[j].x, b.y + off[j].y, b.z + off[j].z});
+        if (q >= 0 && V.life[q] != 0) f = 0;
+      }
+    }
+    V.cand_flag[i] = f;
+  }
+}
+
+// StarveOccupiedBlocksKernel (recycle.cu:15-24): weight -= 1, floored at 0.
+__global__ __launch_bounds__(256) void gc_starve_kernel(View V) {
+  const uint32_t n = V.ctr->cand_count;
+  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
+    const size_t base = (size_t)V.cand[ci] * kBlockVoxels;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x)
+      V.weight[base + i] = fmaxf(0.0f, V.weight[base + i] - 1.0f);
+  }
+}
+
+// CollectGarbageBlockArrayKernel (recycle.cu:27-77): garbage if every
+// observed voxel (weight >= 1e-6, the double EPSILON) has |sdf| >= the
+// truncation at 5 m, or no voxel is observed.
+__global__ __launch_bounds__(256) void gc_flag_garbage_kernel(View V) {
+  __shared__ float smin[4], smax[4];
+  const uint32_t n = V.ctr->cand_count;
+  const float t = truncate_distance(V.geo, 5.0f);
+  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
+    const size_t base = (size_t)V.cand[ci] * kBlockVoxels;
+    float mn = INFINITY, mx = 0.0f;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x) {
+      const float w = V.weight[base + i];
+      const float a = (double)w < 1e-6 ? INFINITY : fabsf(V.sdf[base + i]);
+      mn = fminf(mn, a);
+      mx = fmaxf(mx, w);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      mn = fminf(mn, __shfl_xor(mn, off));
+      mx = fmaxf(mx, __shfl_xor(mx, off));
+    }
+    if ((threadIdx.x & 63u) == 0) {
+      smin[threadIdx.x >> 6] = mn;
+      smax[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float a = smin[0], b = smax[0];
+      for (int w = 1; w < 4; ++w) {
+        a = fminf(a, smin[w]);
+        b = fmaxf(b, smax[w]);
+      }
+      V.cand_flag[ci] = (a >= t || (double)b < 1e-6) ? 1 : 0;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void gc_begin_kernel(View V) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    V.ctr->gc_count = 0;
+    V.ctr->gc_free_count = 0;
+  }
+}
+
+// Flagged worklist entries -> the garbage list.
+__global__ __launch_bounds__(256) void gc_list_kernel(View V) {
+  const uint32_t n = V.ctr->cand_count;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    const bool take = i < n && V.cand_flag[i];
+    const uint64_t mask = __ballot(take);
+    const uint32_t cnt = (uint32_t)__popcll(mask);
+    const uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t out = 0;
+    if (lane == 0 && cnt) out = atomicAdd(&V.ctr->gc_count, cnt);
+    out = __shfl(out, 0);
+    if (take) V.gc_list[out + prefix] = V.cand[i];
+  }
+}
+
+// Garbage blocks left with no allocated vertex slot are to be freed
+// (RecycleGarbageVerticesKernel's valid_vertex_count == 0, recycle.cu:169-178).
+__global__ __launch_bounds__(256) void gc_count_valid_kernel(View V) {
+  __shared__ int any;
+  const uint32_t n = V.ctr->gc_count;
+  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
+    const int32_t p = V.gc_list[ci];
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    bool has = false;
+    for (uint32_t k = threadIdx.x; k < 3u * kBlockVoxels; k += blockDim.x)
+      has = has || V.vptr[(size_t)p * 3 * kBlockVoxels + k] >= 0;
+    if (has) any = 1;
+    __syncthreads();
+    if (threadIdx.x == 0 && !any) V.gc_free[atomicAdd(&V.ctr->gc_free_count, 1u)] = p;
+    __syncthreads();
+  }
+}
+
+// HashTable::FreeEntry (core/hash_table.h:183-255, the linked-list variant):
+// a bucket slot heading a list takes over the list's first node; a list
+// node is unlinked from its predecessor.  Runs on one lane, so no bucket
+// lock is needed.
+__device__ bool free_entry(const View& V, uint64_t key) {
+  if (key == kEmptyKey) return false;
+  const i3 p = unpack_key(key);
+  const uint32_t first = hash_bucket(p, V.bucket_count) * V.bucket_size;
+  const uint32_t last = first + V.bucket_size - 1;
+  const Entry empty{kEmptyKey, kFreePtr, 0u};
+  for (uint32_t j = 0; j < V.bucket_size; ++j) {
+    const uint32_t i = first + j;
+    const Entry e = V.entries[i];
+    if (e.key != key) continue;
+    if (e.offset != 0) {
+      const uint32_t nx = (i + e.offset) % V.entry_count;
+      V.entries[i] = V.entries[nx];
+      V.entries[nx] = empty;
+    } else {
+      V.entries[i] = empty;
+    }
+    return true;
+  }
+  if (V.entries[last].offset == 0) return false;
+  uint32_t prev = last, i = (last + V.entries[last].offset) % V.entry_count;
+  for (uint32_t it = 0; it < V.linked_list_size; ++it) {
+    const Entry e = V.entries[i];
+    if (e.key == key) {
+      V.entries[i] = empty;
+      V.entries[prev].offset = e.offset;
+      return true;
+    }
+    if (e.offset == 0) return false;
+    prev = i;
+    i = (last + e.offset) % V.entry_count;
+  }
+  return false;
+}
+
+// Frees the entries of the garbage list's empty blocks in list order, returns
+// their pool slots to the heap (Free, hash_table.h:288-292) and keeps only
+// the freed blocks on gc_free.  A slot already free (its key cleared) is
+// skipped, so the heap never takes a slot twice.
+__global__ void gc_free_entries_kernel(View V) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t n = V.ctr->gc_free_count;
+  uint32_t kept = 0;
+  for (uint32_t q = 0; q < n; ++q) {
+    const int32_t p = V.gc_free[q];
+    if (V.ctr->block_top + 1 >= (int32_t)V.capacity) break;
+    if (!free_entry(V, V.block_keys[p])) continue;
+    V.block_keys[p] = kEmptyKey;
+    const int32_t top = V.ctr->block_top + 1;
+    V.ctr->block_top = top;
+    V.block_heap[top] = p;
+    V.gc_free[kept++] = p;
+  }
+  V.ctr->gc_free_count = kept;
+}
+
+// Block::Clear (core/block.h:24-39) for the freed blocks.
+__global__ __launch_bounds__(256) void gc_clear_kernel(View V) {
+  const uint32_t n = V.ctr->gc_free_count;
+  for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
+    const int32_t p = V.gc_free[q];
+    const size_t base = (size_t)p * kBlockVoxels;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x) {
+      V.sdf[base + i] = 0.0f;
+      V.weight[base + i] = 0.0f;
+      V.color[base + i] = 0u;
+      if (V.va) {
+        V.va[base + i] = 0.0f;
+        V.vb[base + i] = 0.0f;
+      }
+      V.vptr[3 * (base + i) + 0] = kFreePtr;
+      V.vptr[3 * (base + i) + 1] = kFreePtr;
+      V.vptr[3 * (base + i) + 2] = kFreePtr;
+      V.cube[base + i] = 0;
+    }
+    if (threadIdx.x == 0) {
+      V.surfel[2 * p] = 0;
+      V.surfel[2 * p + 1] = 0;
+      V.life[p] = kBlockLife;
+      V.block_mark[p] = 0u;
+    }
+  }
+}
+
+void launch_gc_low_surfel(const View& V, hipStream_t s, int grid) {
+  gc_life_kernel<<<grid, 256, 0, s>>>(V);
+  gc_low_surfel_flag_kernel<<<grid, 256, 0, s>>>(V);
+}
+
+void launch_gc_starve(const View& V, hipStream_t s, int grid) {
+  gc_starve_kernel<<<grid, 256, 0, s>>>(V);
+}
+
+void launch_gc_flag_garbage(const View& V, hipStream_t s, int grid) {
+  gc_flag_garbage_kernel<<<grid, 256, 0, s>>>(V);
+}
+
+// RecycleGarbageBlockArray (recycle.cu:231-260) over the flagged worklist.
+void launch_gc_recycle(const View& V, hipStream_t s, int grid) {
+  gc_begin_kernel<<<1, 64, 0, s>>>(V);
+  gc_list_kernel<<<grid, 256, 0, s>>>(V);
+  launch_gc_release_triangles(V, s, grid);
+  launch_gc_recycle_vertices(V, s, grid);
+  gc_count_valid_kernel<<<grid, 256, 0, s>>>(V);
+  gc_free_entries_kernel<<<1, 64, 0, s>>>(V);
+  gc_clear_kernel<<<grid, 256, 0, s>>>(V);
 }
 
 // ------------------------------------------------------------------ launchers

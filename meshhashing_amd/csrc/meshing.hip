@@ -506,6 +506,31 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
   }
 }
 
+// ---------------------------------------------------------------- recycling
+// RecycleGarbageTrianglesKernel (mapping/recycle.cu:117-141): the garbage
+// blocks' cubes lose their triangles (current configuration 0; the previous
+// one stays in the high byte), so the reference counts of their vertices
+// drop accordingly; RecycleGarbageVerticesKernel (:143-179) then frees the
+// blocks' own vertex slots left without references (the MC recycle kernel
+// over the garbage list).
+__global__ __launch_bounds__(512) void gc_release_kernel(View V) {
+  const uint32_t n = V.ctr->gc_count;
+  int delta = 0;
+  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
+    const size_t idx = (size_t)V.gc_list[ci] * kBlockVoxels + threadIdx.x;
+    const uint32_t c = V.cube[idx] & 255u;
+    V.cube[idx] = (uint16_t)(c << 8);
+    delta -= (c == 0 || c == 255) ? 0 : kMc.ntri[c];
+  }
+  for (int off = 32; off > 0; off >>= 1) delta += __shfl_xor(delta, off);
+  if ((threadIdx.x & 63u) == 0 && delta)
+    atomicAdd(&V.ctr->tri_live, (unsigned long long)(long long)delta);
+}
+
+void launch_gc_release_triangles(const View& V, hipStream_t s, int grid) {
+  gc_release_kernel<<<grid, 512, 0, s>>>(V);
+}
+
 // ---------------------------------------------------------------- compaction
 // CompressMesh (visualization/compress_mesh.cu:10-172) over the implicit
 // mesh: the live triangles of a worklist block are its cubes' table rows;
@@ -709,6 +734,10 @@ void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int g
   else
     mc_surfel_kernel<false><<<grid, 512, 0, s>>>(One{V}, nullptr, p);
   mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(One{V}, nullptr);
+}
+
+void launch_gc_recycle_vertices(const View& V, hipStream_t s, int grid) {
+  mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(Listed{V, V.gc_list, &V.ctr->gc_count}, nullptr);
 }
 
 void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int grid) {

@@ -184,12 +184,12 @@ struct Counters {
   uint32_t cand_count;     // worklist length
   uint32_t ovf_count;      // keys queued for list insertion this frame
   int32_t vertex_top;      // vertex heap counter (mesh.h:53-59)
-  int32_t pad_top;
+  uint32_t gc_count;       // blocks on the garbage list (RecycleGarbageBlockArray)
   uint32_t flags;          // MH_STATUS_* bits
   uint32_t dropped;
   uint32_t dda_cap;
   uint32_t list_inserts;
-  uint32_t pad0;
+  uint32_t gc_free_count;  // garbage blocks left with no vertex (to be freed)
   // work counters (accumulated since reset), for throughput and roofline
   unsigned long long voxel_updates;      // voxels whose fused state changed
   unsigned long long blocks_integrated;  // worklist blocks visited by fusion
@@ -244,6 +244,10 @@ struct View {
   uint32_t max_triangles;  // bound on live triangles (MeshParams)
   // worklist
   int32_t* cand;
+  uint8_t* cand_flag;    // per worklist entry: garbage flag (recycle.cu)
+  int32_t* life;         // per block: life_count_down (block.h:18)
+  int32_t* gc_list;      // garbage blocks (pool indices)
+  int32_t* gc_free;      // garbage blocks with no vertex left
   // primal-dual state (lazily allocated), 512 per block
   float* pd_sdf0;
   float* pd_sdf_bar;
@@ -333,6 +337,7 @@ __device__ __forceinline__ void flag(const View& V, uint32_t bit) {
 __device__ __forceinline__ int32_t alloc_block(const View& V) {
   const int32_t addr = atomicSub(&V.ctr->block_top, 1);
   if (addr < 0) {
+    atomicAdd(&V.ctr->block_top, 1);  // undo: the top never stays below -1
     flag(V, MH_STATUS_BLOCK_HEAP_EXHAUSTED);
     return kFreePtr;
   }

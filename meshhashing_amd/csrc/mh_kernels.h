@@ -75,14 +75,32 @@ struct Many {
   }
 };
 
+// A list of one volume's blocks other than its worklist (garbage blocks).
+struct Listed {
+  View V;
+  const int32_t* items;
+  const uint32_t* n;
+  __device__ __forceinline__ const View& view(uint32_t) const { return V; }
+  __device__ __forceinline__ uint32_t count() const { return *n; }
+  __device__ __forceinline__ int32_t item(uint32_t ci, uint32_t* scene) const {
+    *scene = 0;
+    return items[ci];
+  }
+  __device__ __forceinline__ int32_t lane_item(uint32_t ci, uint32_t* scene) const {
+    return item(ci, scene);
+  }
+};
+
 // Batched kernels also take the scene table as a __restrict__ kernel
 // argument: stores through the per-scene pointers then provably leave the
 // table unchanged, so per-scene constants load into scalar registers.
 __device__ __forceinline__ One with_table(const One& o, const SceneFrame*) { return o; }
+__device__ __forceinline__ Listed with_table(const Listed& o, const SceneFrame*) { return o; }
 __device__ __forceinline__ Many with_table(const Many& m, const SceneFrame* t) {
   return Many{t, m.work, m.work_count};
 }
 inline const SceneFrame* table_of(const One&) { return nullptr; }
+inline const SceneFrame* table_of(const Listed&) { return nullptr; }
 inline const SceneFrame* table_of(const Many& m) { return m.f; }
 
 // Device-side description of a batch, for the batched launchers.
@@ -130,7 +148,16 @@ void launch_batch_alloc(const BatchDev& B, hipStream_t s);
 void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid);
 void launch_batch_integrate_tsdf(const BatchDev& B, hipStream_t s, int grid, int tiling);
 
+// block recycling (mapping/recycle.cu, core/hash_table.h:183-255)
+constexpr int32_t kBlockLife = 3;  // BLOCK_LIFE, core/block.h:13
+void launch_gc_low_surfel(const View& V, hipStream_t s, int grid);
+void launch_gc_starve(const View& V, hipStream_t s, int grid);
+void launch_gc_flag_garbage(const View& V, hipStream_t s, int grid);
+void launch_gc_recycle(const View& V, hipStream_t s, int grid);
+
 // meshing.hip
+void launch_gc_release_triangles(const View& V, hipStream_t s, int grid);
+void launch_gc_recycle_vertices(const View& V, hipStream_t s, int grid);
 void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int grid);
 void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int grid);
 // mesh compaction (CompressMesh) and an exclusive scan of n uint32 (chunk_sums:

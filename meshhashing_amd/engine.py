@@ -1,6 +1,6 @@
 """Python view of the C ABI: a Volume handle plus the reference engine's
-per-frame driver (MainEngine::Mapping / Meshing / Visualize,
-engine/main_engine.cc:100-174,231-290).
+per-frame driver (MainEngine::Mapping / Meshing / Recycle,
+engine/main_engine.cc:100-192).
 
 Every method calls libmeshhashing_amd.so; nothing here computes on the CPU
 except array bookkeeping for dumps.
@@ -123,6 +123,22 @@ class Volume:
 
     def integrate_tsdf(self):
         A.check(self.lib.mh_integrate_tsdf(self.h), "mh_integrate_tsdf")
+
+    def collect_low_surfel_blocks(self):
+        A.check(self.lib.mh_collect_low_surfel_blocks(self.h), "mh_collect_low_surfel_blocks")
+
+    def starve_blocks(self):
+        A.check(self.lib.mh_starve_blocks(self.h), "mh_starve_blocks")
+
+    def collect_garbage_blocks(self):
+        A.check(self.lib.mh_collect_garbage_blocks(self.h), "mh_collect_garbage_blocks")
+
+    def recycle_garbage_blocks(self, sync: bool = True) -> int | None:
+        """Blocks freed (sync=True waits for the stream)."""
+        n = C.c_uint32(0)
+        A.check(self.lib.mh_recycle_garbage_blocks(self.h, C.byref(n) if sync else None),
+                "mh_recycle_garbage_blocks")
+        return n.value if sync else None
 
     def tune_integrate(self, reps: int = 3) -> A.TuneResult:
         r = A.TuneResult()
@@ -286,6 +302,14 @@ class Volume:
                 "mh_dump_candidates")
         return pos
 
+    def dump_candidate_flags(self) -> np.ndarray:
+        n = C.c_uint32(0)
+        A.check(self.lib.mh_dump_candidate_flags(self.h, 0, None, C.byref(n)), "count")
+        rec = np.zeros((n.value, 4), np.int32)
+        A.check(self.lib.mh_dump_candidate_flags(self.h, n.value, A.ptr(rec, C.c_int32), C.byref(n)),
+                "mh_dump_candidate_flags")
+        return rec
+
     def dump_cube_indices(self) -> np.ndarray:
         n = C.c_uint32(0)
         A.check(self.lib.mh_dump_cube_indices(self.h, 0, None, C.byref(n)), "count")
@@ -322,16 +346,24 @@ class Volume:
 
 
 class MainEngine:
-    """Per-frame driver with the reference's stage order.
+    """Per-frame driver with the reference's stage order and recycling
+    schedule (engine/main_engine.cc:100-192).
 
     Mapping: AllocBlockArray -> CollectBlocksInFrustum -> UpdateBlocksSimple
-    (or PredictOutlierRatio + UpdateBlocksBayesian); Meshing: MarchingCubes on
-    the same worklist; Visualize: RayCaster::Cast.
+    (or PredictOutlierRatio + UpdateBlocksBayesian), after which the frame
+    counter advances.  Meshing: MarchingCubes on the same worklist, then
+    CollectLowSurfelBlocks, and RecycleGarbageBlockArray when the counter is
+    a multiple of 10.  Recycle (plain TSDF only): when counter % 15 == 14,
+    StarveOccupiedBlockArray -> CollectGarbageBlockArray ->
+    RecycleGarbageBlockArray.  recycle=False leaves blocks allocated for
+    good (Meshing is then MarchingCubes alone).
     """
 
-    def __init__(self, cfg: Config):
+    def __init__(self, cfg: Config, recycle: bool = True):
         self.cfg = cfg
         self.vol = Volume(cfg)
+        self.recycle_blocks = recycle
+        self.integrated_frame_count = 0
 
     def mapping(self):
         v = self.vol
@@ -342,15 +374,31 @@ class MainEngine:
             v.integrate_psdf()
         else:
             v.integrate_tsdf()
+        self.integrated_frame_count += 1
 
     def meshing(self):
-        self.vol.marching_cubes()
+        v = self.vol
+        v.marching_cubes()
+        if not self.recycle_blocks:
+            return
+        v.collect_low_surfel_blocks()
+        if self.integrated_frame_count % 10 == 0:
+            v.recycle_garbage_blocks(sync=False)
+
+    def recycle(self):
+        if not self.recycle_blocks or self.cfg.psdf or self.integrated_frame_count % 15 != 14:
+            return
+        v = self.vol
+        v.starve_blocks()
+        v.collect_garbage_blocks()
+        v.recycle_garbage_blocks(sync=False)
 
     def frame(self, depth_u16: np.ndarray, wTc: np.ndarray, mesh: bool = True):
         self.vol.upload_depth(depth_u16, wTc)
         self.mapping()
         if mesh:
             self.meshing()
+        self.recycle()
 
 
 class Batch:

@@ -196,7 +196,12 @@ struct Block {
   // primal-dual (core/voxel.h:62-82)
   double sdf0[512], sdf_bar[512], wpd[512], p[512][3];
   uint8_t mask[512];
+  // recycling (core/block.h:16-18)
+  int inner = 0, boundary = 0, life = 3;
+  bool freed = false;
   void clear() {
+    inner = boundary = 0;
+    life = 3;
     for (int i = 0; i < 512; ++i) {
       sdf[i] = w[i] = a[i] = b[i] = 0.0;
       col[i][0] = col[i][1] = col[i][2] = 0;
@@ -240,6 +245,8 @@ struct orc_volume {
   std::vector<Block> blocks;  // block pool, index = ptr
   std::unordered_map<I3, int, I3Hash> where;
   std::vector<int> cand;  // worklist (pool indices)
+  std::vector<uint8_t> cand_flag;  // garbage flag per worklist entry
+  std::vector<int> free_ptrs;      // pool slots returned by FreeEntry (LIFO)
   // frame
   std::vector<float> depth;
   std::vector<double> inlier;
@@ -287,6 +294,16 @@ bool in_table(const orc_volume* v, I3 p) {
 }
 
 int new_block(orc_volume* v, I3 p) {
+  if (!v->free_ptrs.empty()) {  // Free / Alloc share one LIFO stack (hash_table.h:279-292)
+    const int ptr = v->free_ptrs.back();
+    v->free_ptrs.pop_back();
+    Block& B = v->blocks[ptr];
+    B.clear();
+    B.freed = false;
+    B.pos = p;
+    v->where[p] = ptr;
+    return ptr;
+  }
   const int ptr = (int)v->blocks.size();
   if ((uint32_t)ptr >= v->hp.value_capacity) return kFree;
   v->blocks.emplace_back();
@@ -294,6 +311,40 @@ int new_block(orc_volume* v, I3 p) {
   v->blocks.back().pos = p;
   v->where[p] = ptr;
   return ptr;
+}
+
+// HashTable::FreeEntry (hash_table.h:183-255, linked-list variant).
+bool free_entry(orc_volume* v, I3 p) {
+  const uint32_t N = v->hp.entry_count;
+  const uint32_t first = bucket_of(p, v->hp.bucket_count) * v->hp.bucket_size;
+  const uint32_t last = first + v->hp.bucket_size - 1;
+  for (uint32_t j = 0; j < v->hp.bucket_size; ++j) {
+    const uint32_t i = first + j;
+    const Entry e = v->entries[i];
+    if (!(e.ptr != kFree && e.pos == p)) continue;
+    if (e.offset != 0) {
+      const uint32_t nx = (i + e.offset) % N;
+      v->entries[i] = v->entries[nx];
+      v->entries[nx] = Entry{};
+    } else {
+      v->entries[i] = Entry{};
+    }
+    return true;
+  }
+  if (v->entries[last].offset == 0) return false;
+  uint32_t prev = last, i = (last + v->entries[last].offset) % N;
+  for (uint32_t it = 0; it < v->hp.linked_list_size; ++it) {
+    const Entry e = v->entries[i];
+    if (e.ptr != kFree && e.pos == p) {
+      v->entries[i] = Entry{};
+      v->entries[prev].offset = e.offset;
+      return true;
+    }
+    if (e.offset == 0) return false;
+    prev = i;
+    i = (last + e.offset) % N;
+  }
+  return false;
 }
 
 // Returns false when the bucket is full (caller defers to list insertion).
@@ -594,7 +645,9 @@ int orc_alloc(orc_volume* v) {
 uint32_t orc_collect(orc_volume* v, int frustum) {
   v->cand.clear();
   for (size_t p = 0; p < v->blocks.size(); ++p)
-    if (!frustum || block_in_frustum(v->g, v->cTw, v->blocks[p].pos, v->cam)) v->cand.push_back((int)p);
+    if (v->blocks[p].freed) continue;
+    else if (!frustum || block_in_frustum(v->g, v->cTw, v->blocks[p].pos, v->cam)) v->cand.push_back((int)p);
+  v->cand_flag.assign(v->cand.size(), 0);
   return (uint32_t)v->cand.size();
 }
 
@@ -745,6 +798,27 @@ int orc_integrate_psdf(orc_volume* v) {
 
 // Marching cubes (meshing/marching_cubes.cu:16-321) over the worklist, with
 // canonical vertices: (owner voxel, axis), endpoints owner -> owner + e_axis.
+// Releases the unreferenced vertex slots owned by the given blocks, after
+// counting each block's allocated slots as inner (every local coordinate in
+// 1..6) or boundary surfels.
+static void recycle_vertices(orc_volume* v, const std::vector<int>& ptrs) {
+  for (int ptr : ptrs) {
+    Block& B = v->blocks[ptr];
+    B.inner = B.boundary = 0;
+    for (int i = 0; i < 512; ++i) {
+      const I3 vp = voxel_of(B, i);
+      const int lx = i % 8, ly = (i / 8) % 8, lz = i / 64;
+      const bool inner = lx >= 1 && ly >= 1 && lz >= 1 && lx < 7 && ly < 7 && lz < 7;
+      for (int ax = 0; ax < 3; ++ax) {
+        auto it = v->verts.find(VKey{{vp.x, vp.y, vp.z, ax}});
+        if (it == v->verts.end()) continue;
+        if (inner) ++B.inner; else ++B.boundary;
+        if (it->second.ref == 0) v->verts.erase(it);
+      }
+    }
+  }
+}
+
 int orc_marching_cubes(orc_volume* v, const mh_mc_params* mp) {
   const Geo& g = v->g;
   const float inv = 1.0f / g.vs;
@@ -884,18 +958,102 @@ int orc_marching_cubes(orc_volume* v, const mh_mc_params* mp) {
       }
     }
   }
-  // RecycleVerticesKernel (:303-321)
+  // RecycleVerticesKernel (:303-321); the surfel counts of
+  // TriangleExtractionKernel (:187-205) are taken over the allocated vertex
+  // slots before they are released.
+  recycle_vertices(v, v->cand);
+  return 0;
+}
+
+// ---------------------------------------------------------------- recycling
+// CollectLowSurfelBlocksKernel (mapping/recycle.cu:81-115), in two passes
+// (ageing, then flagging against the aged neighbours).
+int orc_collect_low_surfel(orc_volume* v) {
   for (int ptr : v->cand) {
-    const Block& B = v->blocks[ptr];
-    for (int i = 0; i < 512; ++i) {
-      const I3 vp = voxel_of(B, i);
-      for (int ax = 0; ax < 3; ++ax) {
-        auto it = v->verts.find(VKey{{vp.x, vp.y, vp.z, ax}});
-        if (it != v->verts.end() && it->second.ref == 0) v->verts.erase(it);
-      }
+    Block& B = v->blocks[ptr];
+    B.life = (B.inner < 10 && B.boundary == 0) ? B.life - 1 : 3;
+  }
+  v->cand_flag.assign(v->cand.size(), 0);
+  const I3 off[6] = {{0, 0, 1}, {0, 0, -1}, {0, 1, 0}, {0, -1, 0}, {1, 0, 0}, {-1, 0, 0}};
+  for (size_t i = 0; i < v->cand.size(); ++i) {
+    const Block& B = v->blocks[v->cand[i]];
+    if (B.life > 0) continue;
+    uint8_t f = 1;
+    for (const I3& o : off) {
+      const int q = find_block(v, I3{B.pos.x + o.x, B.pos.y + o.y, B.pos.z + o.z});
+      if (q >= 0 && v->blocks[q].life != 0) f = 0;
     }
+    v->cand_flag[i] = f;
   }
   return 0;
+}
+
+// StarveOccupiedBlocksKernel (recycle.cu:15-24).
+int orc_starve(orc_volume* v) {
+  for (int ptr : v->cand) {
+    Block& B = v->blocks[ptr];
+    for (int i = 0; i < 512; ++i) B.w[i] = std::max(0.0, B.w[i] - 1.0);
+  }
+  return 0;
+}
+
+// CollectGarbageBlockArrayKernel (recycle.cu:27-77); EPSILON is the double
+// 1e-6 compared with the FP32 weight.
+int orc_collect_garbage(orc_volume* v) {
+  const float t = v->g.truncate(5.0f);
+  v->cand_flag.assign(v->cand.size(), 0);
+  for (size_t k = 0; k < v->cand.size(); ++k) {
+    const Block& B = v->blocks[v->cand[k]];
+    float mn = INFINITY, mx = 0.0f;
+    for (int i = 0; i < 512; ++i) {
+      const float w = (float)B.w[i];
+      mn = std::min(mn, (double)w < 1e-6 ? INFINITY : std::fabs((float)B.sdf[i]));
+      mx = std::max(mx, w);
+    }
+    v->cand_flag[k] = (mn >= t || (double)mx < 1e-6) ? 1 : 0;
+  }
+  return 0;
+}
+
+// RecycleGarbageBlockArray (recycle.cu:117-179, 231-260): the flagged
+// blocks' triangles are released (their cubes get configuration 0), their
+// own unreferenced vertices freed, and a flagged block with no vertex left
+// is removed from the table (FreeEntry) and cleared.
+uint32_t orc_recycle_garbage(orc_volume* v) {
+  std::vector<int> gc;
+  for (size_t k = 0; k < v->cand.size() && k < v->cand_flag.size(); ++k)
+    if (v->cand_flag[k]) gc.push_back(v->cand[k]);
+  for (int ptr : gc) {
+    Block& B = v->blocks[ptr];
+    for (int i = 0; i < 512; ++i) {
+      const I3 vp = voxel_of(B, i);
+      for (int t = 0; t < 5; ++t) {
+        auto it = v->tris.find(TKey{{vp.x, vp.y, vp.z, t}});
+        if (it == v->tris.end()) continue;
+        for (const VKey& k : it->second) v->verts[k].ref -= 1;
+        v->tris.erase(it);
+      }
+      B.cube[i] = (uint16_t)((B.cube[i] & 255) << 8);
+    }
+  }
+  recycle_vertices(v, gc);
+  uint32_t freed = 0;
+  for (int ptr : gc) {
+    Block& B = v->blocks[ptr];
+    bool any = false;  // a vertex slot still referenced by a neighbour's triangle
+    for (int i = 0; i < 512 && !any; ++i) {
+      const I3 vp = voxel_of(B, i);
+      for (int ax = 0; ax < 3 && !any; ++ax) any = v->verts.count(VKey{{vp.x, vp.y, vp.z, ax}}) > 0;
+    }
+    if (any) continue;
+    if (!free_entry(v, B.pos)) continue;
+    v->where.erase(B.pos);
+    B.clear();
+    B.freed = true;
+    v->free_ptrs.push_back(ptr);
+    ++freed;
+  }
+  return freed;
 }
 
 // CastKernel (visualization/ray_caster.cu:16-150) with BisectionIntersection
@@ -1320,13 +1478,14 @@ int orc_pd_iterate(orc_volume* v, float lambda, float sigma, float tau, double* 
 
 // ---------------------------------------------------------------- dumps
 static std::vector<int> sorted_ptrs(const orc_volume* v) {
-  std::vector<int> idx(v->blocks.size());
-  for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+  std::vector<int> idx;
+  for (size_t i = 0; i < v->blocks.size(); ++i)
+    if (!v->blocks[i].freed) idx.push_back((int)i);
   std::sort(idx.begin(), idx.end(), [&](int a, int b) { return v->blocks[a].pos < v->blocks[b].pos; });
   return idx;
 }
 
-uint32_t orc_block_count(const orc_volume* v) { return (uint32_t)v->blocks.size(); }
+uint32_t orc_block_count(const orc_volume* v) { return (uint32_t)v->where.size(); }
 
 int orc_dump_blocks(const orc_volume* v, int32_t* pos, double* sdf, double* w, uint32_t* col,
                     double* a, double* b, uint16_t* cube) {
@@ -1414,6 +1573,19 @@ uint32_t orc_dump_candidates(const orc_volume* v, int32_t* pos) {
   if (pos)
     for (size_t i = 0; i < ps.size(); ++i) { pos[3 * i] = ps[i].x; pos[3 * i + 1] = ps[i].y; pos[3 * i + 2] = ps[i].z; }
   return (uint32_t)ps.size();
+}
+
+// Flags of the last low-surfel / garbage pass as (x, y, z, flag) rows in
+// block-position order.
+uint32_t orc_dump_cand_flags(const orc_volume* v, int32_t* rec) {
+  std::vector<std::array<int32_t, 4>> r;
+  for (size_t k = 0; k < v->cand.size() && k < v->cand_flag.size(); ++k) {
+    const I3 p = v->blocks[v->cand[k]].pos;
+    r.push_back({p.x, p.y, p.z, (int32_t)v->cand_flag[k]});
+  }
+  std::sort(r.begin(), r.end());
+  if (rec) std::memcpy(rec, r.data(), r.size() * sizeof(r[0]));
+  return (uint32_t)r.size();
 }
 
 uint32_t orc_tri_count(const orc_volume* v) { return (uint32_t)v->tris.size(); }

@@ -80,6 +80,11 @@ def lib() -> C.CDLL:
                                               P(C.c_double)]),
             "orc_perturb_sdf": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_double]),
             "orc_set_color": (C.c_int, [vp, P(C.c_uint8)]),
+            "orc_collect_low_surfel": (C.c_int, [vp]),
+            "orc_starve": (C.c_int, [vp]),
+            "orc_collect_garbage": (C.c_int, [vp]),
+            "orc_recycle_garbage": (C.c_uint32, [vp]),
+            "orc_dump_cand_flags": (C.c_uint32, [vp, P(C.c_int32)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -181,6 +186,25 @@ class Oracle:
             self.integrate_tsdf()
         if mesh:
             self.marching_cubes()
+
+    # -- recycling (mapping/recycle.cu) ---------------------------------------
+    def collect_low_surfel_blocks(self):
+        self.L.orc_collect_low_surfel(self.h)
+
+    def starve_blocks(self):
+        self.L.orc_starve(self.h)
+
+    def collect_garbage_blocks(self):
+        self.L.orc_collect_garbage(self.h)
+
+    def recycle_garbage_blocks(self) -> int:
+        return int(self.L.orc_recycle_garbage(self.h))
+
+    def dump_cand_flags(self) -> np.ndarray:
+        n = self.L.orc_dump_cand_flags(self.h, None)
+        rec = np.zeros((n, 4), np.int32)
+        self.L.orc_dump_cand_flags(self.h, _p(rec, C.c_int32))
+        return rec
 
     # -- dumps ---------------------------------------------------------------
     def dump_blocks(self) -> dict:
