@@ -302,6 +302,36 @@ typedef struct mh_tune_result {
 int mh_tune_integrate(mh_volume* v, uint32_t reps, mh_tune_result* out);
 int mh_set_integrate_tiling(mh_volume* v, int tiling, int workgroups_per_cu);
 
+/* ---- localisation: PointToSurface, localizing/point_to_psdf.h -----------
+ * Normal equations of the current frame's depth against the volume at the
+ * camera->world pose wTc (PointToSurfaceKernel, point_to_psdf.cu:19-82):
+ * every pixel with odd x and odd y and a depth that is finite, non-zero and
+ * below sdf_upper_bound, whose world point has an SDF value and gradient
+ * (GetSpatialValue / GetSpatialSDFGradient, spatial_query.h:23-97), adds
+ * J J^T to A, d J to b, d^2 to err and 1 to count, with d the interpolated
+ * SDF and J = (g, p x g) for the tangent (translation, rotation).
+ * MH_P2S_REFERENCE keeps the reference's J, whose third entry is g.x
+ * (point_to_psdf.cu:69-74); MH_P2S_CORRECTED uses g.z there.  Sums are
+ * FP64 in a fixed order (the reference: float atomics).  Synchronises.   */
+#define MH_P2S_REFERENCE 0
+#define MH_P2S_CORRECTED 1
+typedef struct {
+  double A[36];    /* row-major 6x6 */
+  double b[6];
+  double err;      /* sum of d^2 */
+  uint32_t count;  /* pixels used */
+} mh_point_to_psdf_result;
+int mh_point_to_psdf(mh_volume* v, const float wTc[16], int jacobian,
+                     mh_point_to_psdf_result* out);
+/* MainEngine::Localizing (engine/main_engine.cc:28-98): `iterations` steps
+ * of (A + lambda I) dxi = -b, wTc <- exp(dxi) wTc, stopping early when no
+ * pixel is used; the reference calls it with lambda = 1e5 and 20 steps
+ * (app/slam.cc:94).  wTc is updated in place and becomes the frame's pose
+ * (Sensor::set_transform).  err_trace / count_trace (may be NULL) get each
+ * step's err and count.  The solve and exponential are FP64.            */
+int mh_localize(mh_volume* v, uint32_t iterations, float lambda, int jacobian, float wTc[16],
+                double* err_trace, uint32_t* count_trace, uint32_t* iterations_done);
+
 /* ---- mesh compaction and PLY output -------------------------------------
  * CompressMesh (visualization/compress_mesh.cu:137-172) over the current
  * worklist (mh_collect_all + mh_marching_cubes first for the whole mesh, as

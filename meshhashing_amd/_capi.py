@@ -83,6 +83,15 @@ class RaycastGrad(C.Structure):
                 ("pixels", C.c_uint32), ("outliers", C.c_uint32)]
 
 
+class PointToPsdf(C.Structure):
+    _fields_ = [("A", C.c_double * 36), ("b", C.c_double * 6), ("err", C.c_double),
+                ("count", C.c_uint32)]
+
+
+MH_P2S_REFERENCE = 0
+MH_P2S_CORRECTED = 1
+
+
 class TuneResult(C.Structure):
     _fields_ = [("tiling", C.c_int32), ("workgroup_size", C.c_int32),
                 ("blocks_per_workgroup", C.c_int32), ("voxels_per_lane", C.c_int32),
@@ -160,6 +169,8 @@ SIGNATURES = {
     "mh_compress_mesh": (C.c_int, [VP, P_U32, P_U32]),
     "mh_compact_mesh_get": (C.c_int, [VP, C.c_uint32, P_F32, P_F32, P_F32, C.c_uint32, P_I32]),
     "mh_save_ply": (C.c_int, [VP, C.c_char_p]),
+    "mh_point_to_psdf": (C.c_int, [VP, P_F32, C.c_int, C.POINTER(PointToPsdf)]),
+    "mh_localize": (C.c_int, [VP, C.c_uint32, C.c_float, C.c_int, P_F32, P_F64, P_U32, P_U32]),
     "mh_raycast_backward": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, VP, VP,
                                       C.POINTER(RaycastGrad)]),
     "mh_refine_pose": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, VP, C.c_uint32, P_F32,

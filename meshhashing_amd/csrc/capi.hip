@@ -1406,6 +1406,21 @@ int mh_batch_synchronize(mh_batch* b) {
 // ------------------------------------------------------------ raycast adjoint
 namespace {
 
+// FP64 partial-sum rows (up to 30 slots each) for `rows` workgroups, and the
+// 30-slot result.
+hipError_t ensure_partials(mh_volume* v, uint32_t rows) {
+  if (rows > v->rgrad_cap) {
+    if (v->rgrad_partials) (void)hipFree(v->rgrad_partials);
+    v->rgrad_partials = nullptr;
+    v->rgrad_cap = 0;
+    const hipError_t e = dalloc(&v->rgrad_partials, (size_t)rows * 30);
+    if (e != hipSuccess) return e;
+    v->rgrad_cap = rows;
+  }
+  if (!v->rgrad_out) return dalloc(&v->rgrad_out, 30);
+  return hipSuccess;
+}
+
 int raycast_grad(mh_volume* v, const mh_raycaster_params* p, const float cTw[16], const float* obs,
                  float* grad_sdf, mh_raycast_grad* out, float max_residual = INFINITY,
                  float max_slope = INFINITY) {
@@ -1413,14 +1428,7 @@ int raycast_grad(mh_volume* v, const mh_raycaster_params* p, const float cTw[16]
   const int rc = build_ray(v, p, cTw, &R);
   if (rc) return rc;
   const uint32_t n = raycast_grad_partials(R);
-  if (n > v->rgrad_cap) {
-    if (v->rgrad_partials) (void)hipFree(v->rgrad_partials);
-    v->rgrad_partials = nullptr;
-    v->rgrad_cap = 0;
-    HIP_TRY(dalloc(&v->rgrad_partials, (size_t)n * 30));
-    v->rgrad_cap = n;
-  }
-  if (!v->rgrad_out) HIP_TRY(dalloc(&v->rgrad_out, 30));
+  HIP_TRY(ensure_partials(v, n));
   launch_raycast_backward(v->V, R, obs, grad_sdf, v->rgrad_partials, v->rgrad_out, max_residual,
                           max_slope, v->stream);
   HIP_TRY(hipGetLastError());
@@ -1609,6 +1617,79 @@ int mh_dump_voxel_field(mh_volume* v, const float* field_dev, uint32_t capacity,
   if (!out_host) return MH_OK;
   if (capacity < blocks.size()) return fail(MH_ERR_CAPACITY, "buffer too small");
   return gather_field(v, field_dev, blocks, read_counters(v).block_hwm, 1, out_host);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ localisation
+namespace {
+
+// PointToSurface (localizing/point_to_psdf.cu:84-147) on the current frame.
+int point_to_psdf(mh_volume* v, const float wTc[16], int jacobian, mh_point_to_psdf_result* out) {
+  flush_frame(v);
+  HIP_TRY(ensure_partials(v, point_to_psdf_partials(v->V)));
+  M4 T;
+  std::memcpy(T.m, wTc, sizeof(T.m));
+  launch_point_to_psdf(v->V, T, jacobian == MH_P2S_CORRECTED ? 1 : 0, v->rgrad_partials,
+                       v->rgrad_out, v->stream);
+  HIP_TRY(hipGetLastError());
+  double h[kP2SSlots];
+  HIP_TRY(hipMemcpyAsync(h, v->rgrad_out, sizeof(h), hipMemcpyDeviceToHost, v->stream));
+  HIP_TRY(hipStreamSynchronize(v->stream));
+  int t = 0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) {
+      out->A[6 * i + j] = h[t];
+      out->A[6 * j + i] = h[t];
+      ++t;
+    }
+  for (int i = 0; i < 6; ++i) out->b[i] = h[21 + i];
+  out->err = h[27];
+  out->count = (uint32_t)h[28];
+  return MH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mh_point_to_psdf(mh_volume* v, const float wTc[16], int jacobian, mh_point_to_psdf_result* out) {
+  if (check_vol(v) || !wTc || !out) return fail(MH_ERR_ARG, "null argument");
+  if (jacobian != MH_P2S_REFERENCE && jacobian != MH_P2S_CORRECTED)
+    return fail(MH_ERR_ARG, "jacobian must be MH_P2S_REFERENCE or MH_P2S_CORRECTED");
+  if (!v->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
+  return point_to_psdf(v, wTc, jacobian, out);
+}
+
+int mh_localize(mh_volume* v, uint32_t iterations, float lambda, int jacobian, float wTc[16],
+                double* err_trace, uint32_t* count_trace, uint32_t* iterations_done) {
+  if (check_vol(v) || !wTc) return fail(MH_ERR_ARG, "null argument");
+  if (jacobian != MH_P2S_REFERENCE && jacobian != MH_P2S_CORRECTED)
+    return fail(MH_ERR_ARG, "jacobian must be MH_P2S_REFERENCE or MH_P2S_CORRECTED");
+  if (!v->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
+  uint32_t it = 0;
+  for (; it < iterations; ++it) {
+    mh_point_to_psdf_result r;
+    const int rc = point_to_psdf(v, wTc, jacobian, &r);
+    if (rc) return rc;
+    if (err_trace) err_trace[it] = r.err;
+    if (count_trace) count_trace[it] = r.count;
+    if (r.count == 0) break;
+    // SolveAndConvertDeltaXi (main_engine.cc:28-54): (A + lambda I) dxi = -b,
+    // dT = exp(dxi) with dxi = (translation, rotation), wTc <- dT wTc.
+    double H[36], x[6];
+    for (int k = 0; k < 36; ++k) H[k] = r.A[k] + (k % 7 == 0 ? (double)lambda : 0.0);
+    if (!solve6(H, r.b, 0.0, x)) break;
+    const double xi[6] = {x[3], x[4], x[5], x[0], x[1], x[2]};
+    double E[16], T[16], Tn[16];
+    se3_exp(xi, E);
+    for (int k = 0; k < 16; ++k) T[k] = wTc[k];
+    matmul4(E, T, Tn);
+    for (int k = 0; k < 16; ++k) wTc[k] = (float)Tn[k];
+  }
+  if (iterations_done) *iterations_done = it;
+  set_pose(v, wTc);  // Sensor::set_transform
+  return MH_OK;
 }
 
 }  // extern "C"

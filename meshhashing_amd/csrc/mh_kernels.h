@@ -175,11 +175,18 @@ void launch_raycast(const View& V, const RayConst& p, float* depth, float* verte
                     hipStream_t s);
 void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,
                           hipStream_t s);
-// adjoint: 29 FP64 sums (loss, dL/dxi[6], J^T J upper triangle[21], pixels)
+// adjoint: 30 FP64 sums (loss, dL/dxi[6], J^T J upper triangle[21], pixels,
+// gated-out pixels)
 uint32_t raycast_grad_partials(const RayConst& p);
 void launch_raycast_backward(const View& V, const RayConst& p, const float* obs, float* grad_sdf,
                              double* partials, double* out, float max_residual, float max_slope,
                              hipStream_t s);
+// point-to-PSDF normal equations: 29 FP64 sums (A upper triangle[21], b[6],
+// err, count) over the current frame's depth
+constexpr int kP2SSlots = 29;
+uint32_t point_to_psdf_partials(const View& V);
+void launch_point_to_psdf(const View& V, const M4& wTc, int corrected, double* partials, double* out,
+                          hipStream_t s);
 
 // primal_dual.hip
 void launch_pd_neighbors(const View& V, int32_t* nbr, hipStream_t s, int grid);

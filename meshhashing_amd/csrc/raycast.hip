@@ -398,13 +398,94 @@ __global__ __launch_bounds__(64) void raycast_backward_kernel(View V, RayConst R
 }
 
 // Sums the per-workgroup partials in workgroup order (deterministic).
-__global__ void raycast_grad_reduce_kernel(const double* __restrict__ partials, uint32_t n,
-                                           double* __restrict__ out) {
+// Sums n rows of `slots` FP64 partials column by column, in row order.
+__global__ void slot_reduce_kernel(const double* __restrict__ partials, uint32_t n, uint32_t slots,
+                                   double* __restrict__ out) {
   const uint32_t k = threadIdx.x;
-  if (k >= (uint32_t)kGradSlots) return;
+  if (k >= slots) return;
   double s = 0.0;
-  for (uint32_t i = 0; i < n; ++i) s += partials[(size_t)i * kGradSlots + k];
+  for (uint32_t i = 0; i < n; ++i) s += partials[(size_t)i * slots + k];
   out[k] = s;
+}
+
+// ------------------------------------------------------------ localisation
+// PointToSurfaceKernel (localizing/point_to_psdf.cu:19-82): one lane per odd
+// pixel (both coordinates odd), 8 x 8 of them per 64-lane workgroup, i.e.
+// a 16 x 16 pixel tile as in the reference.  A depth that is -inf, 0 or at
+// least sdf_upper_bound is skipped; the world point wTc * p, the SDF
+// (GetSpatialValue) and its central difference at +-vs/2
+// (GetSpatialSDFGradient, spatial_query.h:71-97) are the reference's FP32
+// expressions.  J = (g, p x g) for the tangent (translation, rotation);
+// `corrected` = 0 keeps the reference's g.x in J's third entry (:69-74).
+// The per-pixel products J J^T, d J and d^2 are FP32, as the reference's
+// mat6x6 arithmetic; they are summed in FP64 per wave and then per
+// workgroup in a fixed order instead of float atomics.  Slots: A's upper
+// triangle (21), b (6), err, count.
+__global__ __launch_bounds__(64) void point_to_psdf_kernel(View V, M4 wTc, float inv_vs, int corrected,
+                                                           double* __restrict__ partials) {
+  const uint32_t x = 2 * (blockIdx.x * 8 + (threadIdx.x & 7)) + 1;
+  const uint32_t y = 2 * (blockIdx.y * 8 + (threadIdx.x >> 3)) + 1;
+  double acc[kP2SSlots];
+#pragma unroll
+  for (int k = 0; k < kP2SSlots; ++k) acc[k] = 0.0;
+  if (x < V.cam.width && y < V.cam.height) {
+    const float depth = V.depth[(size_t)y * V.cam.width + x];
+    if (!(depth == -INFINITY || depth == 0.0f || depth >= V.geo.sdf_upper_bound)) {
+      const f3 pw = xform(wTc, reproject(V.cam, x, y, depth));
+      BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr};
+      float d;
+      if (ray_sdf(V, cache, pw, inv_vs, &d)) {
+        const float h = 0.5f * V.geo.voxel_size;
+        bool ok = true;
+        float sp[3] = {0, 0, 0}, sn[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 3 && ok; ++k) {
+          const f3 dp = mk3(k == 0 ? h : 0.0f, k == 1 ? h : 0.0f, k == 2 ? h : 0.0f);
+          ok = ray_sdf(V, cache, pw - dp, inv_vs, &sn[k]);
+          if (ok) ok = ray_sdf(V, cache, pw + dp, inv_vs, &sp[k]);
+        }
+        if (ok) {
+          const float vs = V.geo.voxel_size;
+          const f3 g = mk3((sp[0] - sn[0]) / vs, (sp[1] - sn[1]) / vs, (sp[2] - sn[2]) / vs);
+          const float J[6] = {g.x, g.y, corrected ? g.z : g.x,
+                              -g.y * pw.z + g.z * pw.y,
+                              -g.z * pw.x + g.x * pw.z,
+                              -g.x * pw.y + g.y * pw.x};
+          int t = 0;
+#pragma unroll
+          for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = i; j < 6; ++j) acc[t++] = (double)(J[i] * J[j]);
+#pragma unroll
+          for (int i = 0; i < 6; ++i) acc[21 + i] = (double)(d * J[i]);
+          acc[27] = (double)(d * d);
+          acc[28] = 1.0;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kP2SSlots; ++k) {
+    double v = acc[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    acc[k] = v;
+  }
+  if (threadIdx.x == 0) {
+    const size_t wg = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+#pragma unroll
+    for (int k = 0; k < kP2SSlots; ++k) partials[wg * kP2SSlots + k] = acc[k];
+  }
+}
+
+uint32_t point_to_psdf_partials(const View& V) {
+  return ((V.cam.width / 2 + 7) / 8) * ((V.cam.height / 2 + 7) / 8);
+}
+
+void launch_point_to_psdf(const View& V, const M4& wTc, int corrected, double* partials, double* out,
+                          hipStream_t s) {
+  const dim3 grid((V.cam.width / 2 + 7) / 8, (V.cam.height / 2 + 7) / 8);
+  point_to_psdf_kernel<<<grid, 64, 0, s>>>(V, wTc, 1.0f / V.geo.voxel_size, corrected, partials);
+  slot_reduce_kernel<<<1, 64, 0, s>>>(partials, grid.x * grid.y, kP2SSlots, out);
 }
 
 __device__ __forceinline__ const RayConst& ray_of(const One&, uint32_t, const RayConst& r) { return r; }
@@ -460,7 +541,7 @@ void launch_raycast_backward(const View& V, const RayConst& p, const float* obs,
   dim3 grid((p.cam.width + 7) / 8, (p.cam.height + 7) / 8);
   raycast_backward_kernel<<<grid, 64, 0, s>>>(V, p, obs, grad_sdf, partials, max_residual,
                                               max_slope);
-  raycast_grad_reduce_kernel<<<1, 64, 0, s>>>(partials, grid.x * grid.y, out);
+  slot_reduce_kernel<<<1, 64, 0, s>>>(partials, grid.x * grid.y, kGradSlots, out);
 }
 
 void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,

@@ -198,6 +198,32 @@ class Volume:
                 "mh_refine_pose")
         return np.array(out[:], np.float32).reshape(4, 4), trace
 
+    def point_to_psdf(self, wTc: np.ndarray, corrected: bool = False) -> dict:
+        """PointToSurface on the current frame (mh_point_to_psdf): A (6x6),
+        b (6), err, count; the tangent is (translation, rotation)."""
+        r = A.PointToPsdf()
+        A.check(self.lib.mh_point_to_psdf(self.h, A.f32x16(wTc),
+                                          A.MH_P2S_CORRECTED if corrected else A.MH_P2S_REFERENCE,
+                                          C.byref(r)), "mh_point_to_psdf")
+        return {"A": np.array(r.A[:]).reshape(6, 6), "b": np.array(r.b[:]), "err": r.err,
+                "count": r.count}
+
+    def localize(self, wTc: np.ndarray, iterations: int = 20, lam: float = 1e5,
+                 corrected: bool = False):
+        """MainEngine::Localizing (mh_localize): the refined wTc (also the
+        frame's pose from now on), per-step err and count, and the number
+        of steps taken (fewer than asked if a step used no pixel)."""
+        T = (C.c_float * 16)(*np.asarray(wTc, np.float32).reshape(16).tolist())
+        err = np.zeros(iterations, np.float64)
+        cnt = np.zeros(iterations, np.uint32)
+        done = C.c_uint32(0)
+        A.check(self.lib.mh_localize(self.h, iterations, lam,
+                                     A.MH_P2S_CORRECTED if corrected else A.MH_P2S_REFERENCE, T,
+                                     A.ptr(err, C.c_double), A.ptr(cnt, C.c_uint32), C.byref(done)),
+                "mh_localize")
+        return {"wTc": np.array(T[:], np.float32).reshape(4, 4), "err": err, "count": cnt,
+                "steps": done.value}
+
     def compress_mesh(self) -> dict:
         """CompressMesh over the current worklist: dense vertices (pos,
         normal, colour) and triangles (vertex indices)."""

@@ -1131,6 +1131,52 @@ int orc_raycast(orc_volume* v, const mh_raycaster_params* rp, const float* cTw, 
   return 0;
 }
 
+// PointToSurfaceKernel (localizing/point_to_psdf.cu:19-82) on the current
+// frame's depth: odd pixels (both coordinates odd) with a depth that is
+// finite, non-zero and below sdf_upper_bound; the world point wTc * p in
+// FP32; GetSpatialValue and GetSpatialSDFGradient (spatial_query.h:23-97) in
+// FP32; J = (g, p x g) for the tangent (translation, rotation).  mode 0 puts
+// g.x in J's third slot, as the reference does (:69-74); mode 1 puts g.z.
+// Per pixel J J^T, d J and d^2 are FP32 products (the reference's mat6x6
+// arithmetic), summed in FP64.  out: A[36] row-major, b[6], err, count.
+int orc_point_to_psdf(const orc_volume* v, const float* wTc, int mode, double* out) {
+  const Cam& c = v->cam;
+  const float vs = v->g.vs;
+  const float h = 0.5f * vs;
+  for (int k = 0; k < 44; ++k) out[k] = 0.0;
+  for (uint32_t y = 1; y < c.h; y += 2) {
+    for (uint32_t x = 1; x < c.w; x += 2) {
+      const float depth = v->depth[(size_t)y * c.w + x];
+      if (!depth_ok(depth, v->g.ub)) continue;
+      const F3 pw = xf(wTc, c.reproj(x, y, depth));
+      float d;
+      if (!spatial32(v, pw, &d)) continue;
+      bool ok = true;
+      float sn[3] = {0, 0, 0}, sp[3] = {0, 0, 0};
+      for (int k = 0; k < 3 && ok; ++k) {
+        F3 pm = pw, pq = pw;
+        if (k == 0) { pm.x = pw.x - h; pq.x = pw.x + h; }
+        if (k == 1) { pm.y = pw.y - h; pq.y = pw.y + h; }
+        if (k == 2) { pm.z = pw.z - h; pq.z = pw.z + h; }
+        ok = spatial32(v, pm, &sn[k]);
+        if (ok) ok = spatial32(v, pq, &sp[k]);
+      }
+      if (!ok) continue;
+      const F3 g{(sp[0] - sn[0]) / vs, (sp[1] - sn[1]) / vs, (sp[2] - sn[2]) / vs};
+      const float J[6] = {g.x, g.y, mode ? g.z : g.x,
+                          -g.y * pw.z + g.z * pw.y,
+                          -g.z * pw.x + g.x * pw.z,
+                          -g.x * pw.y + g.y * pw.x};
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) out[6 * i + j] += (double)(J[i] * J[j]);
+      for (int i = 0; i < 6; ++i) out[36 + i] += (double)(d * J[i]);
+      out[42] += (double)(d * d);
+      out[43] += 1.0;
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"
 
 namespace {

@@ -80,6 +80,7 @@ def lib() -> C.CDLL:
                                               P(C.c_double)]),
             "orc_perturb_sdf": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_double]),
             "orc_set_color": (C.c_int, [vp, P(C.c_uint8)]),
+            "orc_point_to_psdf": (C.c_int, [vp, P(C.c_float), C.c_int, P(C.c_double)]),
             "orc_collect_low_surfel": (C.c_int, [vp]),
             "orc_starve": (C.c_int, [vp]),
             "orc_collect_garbage": (C.c_int, [vp]),
@@ -186,6 +187,15 @@ class Oracle:
             self.integrate_tsdf()
         if mesh:
             self.marching_cubes()
+
+    # -- localisation (localizing/point_to_psdf.cu) ---------------------------
+    def point_to_psdf(self, wTc, corrected: bool = False) -> dict:
+        """Normal equations of the current frame's depth against the volume
+        at camera->world pose wTc: A (6x6), b (6), err, count."""
+        out = np.zeros(44, np.float64)
+        self.L.orc_point_to_psdf(self.h, _m16(wTc), 1 if corrected else 0, _p(out, C.c_double))
+        return {"A": out[:36].reshape(6, 6).copy(), "b": out[36:42].copy(), "err": out[42],
+                "count": int(out[43])}
 
     # -- recycling (mapping/recycle.cu) ---------------------------------------
     def collect_low_surfel_blocks(self):
