@@ -18,7 +18,9 @@ batched form: 64 independent 320x240 scenes with voxel sizes 1-4 cm, per-
 scene throughput of the fused batch against one-scene-at-a-time runs.
 `pose_refinement` holds the differentiable raycast: Gauss-Newton pose
 refinement of 32 perturbed 640x480 views and the adjoint's error against
-the CPU FP64 adjoint.
+the CPU FP64 adjoint, beside the reference's own point-to-PSDF localiser
+(20 steps, its Jacobian and lambda, and the corrected Jacobian) from the
+same starting poses.
 """
 from __future__ import annotations
 
@@ -233,6 +235,7 @@ def c5_pose_refinement(dev, stream, n_poses=32, iterations=10, grad_check=True):
     q = replace(C5, frames=n_poses)
     rng = np.random.default_rng(5)
     err0, err1, ms = [], [], []
+    loc = {"reference": ([], []), "corrected": ([], [])}
     first = None
     for i in range(n_poses):
         gt = synth_pose(q, i).astype(np.float64)
@@ -246,7 +249,20 @@ def c5_pose_refinement(dev, stream, n_poses=32, iterations=10, grad_check=True):
         t0 = time.perf_counter()
         out, trace = vol.refine_pose(start.astype(np.float32), obs.data_ptr(), iterations)
         ms.append(1e3 * (time.perf_counter() - t0))
-        for lst, T in ((err0, start), (err1, out)):
+        # the reference's own solver from the same start: point-to-PSDF on
+        # the observed frame, 20 steps at lambda 1e5 with its Jacobian
+        # (MainEngine::Localizing as app/slam.cc calls it), and the
+        # corrected Jacobian at lambda 4e3
+        vol.upload_depth(synth_depth(q, i), start.astype(np.float32))
+        found = {}
+        for key, corr, lam in (("reference", False, 1e5), ("corrected", True, 4e3)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            found[key] = vol.localize(start.astype(np.float32), iterations=20, lam=lam,
+                                      corrected=corr)["wTc"]
+            loc[key][1].append(1e3 * (time.perf_counter() - t0))
+        for lst, T in ((err0, start), (err1, out), (loc["reference"][0], found["reference"]),
+                       (loc["corrected"][0], found["corrected"])):
             D = np.linalg.inv(gt) @ T.astype(np.float64)
             ang = np.degrees(np.arccos(np.clip((np.trace(D[:3, :3]) - 1) / 2, -1, 1)))
             lst.append((ang, 1e3 * np.linalg.norm(D[:3, 3])))
@@ -260,6 +276,12 @@ def c5_pose_refinement(dev, stream, n_poses=32, iterations=10, grad_check=True):
         "final_error_deg_mm_max": [float(max(e[0] for e in err1)), float(max(e[1] for e in err1))],
         "ms_per_pose": float(np.median(ms)),
     }
+    for key, (errs, times) in loc.items():
+        res[f"point_to_psdf_{key}"] = {
+            "final_error_deg_mm_median": [float(np.median([e[0] for e in errs])),
+                                          float(np.median([e[1] for e in errs]))],
+            "ms_per_pose": float(np.median(times)),
+        }
     if grad_check:
         sys.path.insert(0, str(ROOT / "oracle"))
         import oracle as orc

@@ -116,6 +116,8 @@ struct mh_volume {
   double* rgrad_partials = nullptr;     // raycast adjoint: per-workgroup sums
   uint32_t rgrad_cap = 0;
   double* rgrad_out = nullptr;
+  uint32_t* ray_range = nullptr;        // raycast: camera-z range per 8x8 tile
+  uint32_t ray_range_words = 0;
   int tiling = kDefaultIntegrateTiling;  // integration tiling (kIntegrateTilings)
   int grid_per_cu = 8;                   // integration workgroups per CU
   std::vector<void*> owned;
@@ -191,6 +193,23 @@ int build_ray(const mh_volume* v, const mh_raycaster_params* p, const float cTw[
   R->inv_vs = 1.0f / v->vp.voxel_size;
   std::memcpy(R->cTw.m, cTw, sizeof(float) * 16);
   invert4<float>(cTw, R->wTc.m);
+  R->range = nullptr;
+  R->tiles_x = (p->width + 7) / 8;
+  R->tiles = R->tiles_x * ((p->height + 7) / 8);
+  return MH_OK;
+}
+
+// The volume's tile-range buffer, grown to the raycast's size.
+int attach_range(mh_volume* v, RayConst* R) {
+  const uint32_t words = ray_range_words(R->cam.width, R->cam.height);
+  if (words > v->ray_range_words) {
+    if (v->ray_range) (void)hipFree(v->ray_range);
+    v->ray_range = nullptr;
+    v->ray_range_words = 0;
+    HIP_TRY(dalloc(&v->ray_range, words));
+    v->ray_range_words = words;
+  }
+  R->range = v->ray_range;
   return MH_OK;
 }
 
@@ -360,6 +379,7 @@ int mh_volume_destroy(mh_volume* v) {
   for (void* p : v->owned) (void)hipFree(p);
   if (v->rgrad_partials) (void)hipFree(v->rgrad_partials);
   if (v->rgrad_out) (void)hipFree(v->rgrad_out);
+  if (v->ray_range) (void)hipFree(v->ray_range);
   delete v;
   return MH_OK;
 }
@@ -572,7 +592,9 @@ int mh_raycast(mh_volume* v, const mh_raycaster_params* p, const float cTw[16], 
                float* vertex_dev, float* normal_dev) {
   if (check_vol(v) || !p || !cTw) return fail(MH_ERR_ARG, "null argument");
   RayConst R;
-  const int rc = build_ray(v, p, cTw, &R);
+  int rc = build_ray(v, p, cTw, &R);
+  if (rc) return rc;
+  rc = attach_range(v, &R);
   if (rc) return rc;
   launch_raycast(v->V, R, depth_dev, vertex_dev, normal_dev, v->stream);
   HIP_TRY(hipGetLastError());
@@ -1073,6 +1095,7 @@ struct mh_batch {
   uint32_t* d_work = nullptr;
   uint32_t* d_work_count = nullptr;
   uint16_t* d_raw = nullptr;       // staging for host depth uploads
+  uint32_t* d_range = nullptr;     // raycast tile ranges, ray_range_words(W, H) per scene
   const uint16_t* raw = nullptr;   // depth of the current frame (n*W*H)
   bool have_frame = false;
   std::vector<mh_raycaster_params> rp;
@@ -1102,7 +1125,8 @@ void batch_free(mh_batch* b) {
     if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
     if (b->pinned[k]) (void)hipHostFree(b->pinned[k]);
   }
-  for (void* p : {(void*)b->d_frames, (void*)b->d_work, (void*)b->d_work_count, (void*)b->d_raw})
+  for (void* p : {(void*)b->d_frames, (void*)b->d_work, (void*)b->d_work_count, (void*)b->d_raw,
+                  (void*)b->d_range})
     if (p) (void)hipFree(p);
   for (mh_volume* v : b->vols) mh_volume_destroy(v);
   if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
@@ -1314,6 +1338,7 @@ int mh_batch_set_raycast(mh_batch* b, const mh_raycaster_params* p, float* depth
     const int rc = build_ray(b->vols[s], &p[s], b->vols[s]->V.cTw.m, &R);
     if (rc) return rc;
   }
+  if (!b->d_range) HIP_TRY(dalloc(&b->d_range, (size_t)b->n * ray_range_words(b->W, b->H)));
   b->rp.assign(p, p + b->n);
   b->out_depth = depth_dev;
   b->out_vertex = vertex_dev;
@@ -1351,6 +1376,7 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
     if (flags & MH_RUN_RAYCAST) {
       const int rc = build_ray(v, &b->rp[s], v->V.cTw.m, &f.ray);
       if (rc) return rc;
+      f.ray.range = b->d_range + (size_t)s * ray_range_words(b->W, b->H);
     }
     f.V = v->V;
     f.V.raw = b->raw + (size_t)s * b->W * b->H;
@@ -1425,7 +1451,9 @@ int raycast_grad(mh_volume* v, const mh_raycaster_params* p, const float cTw[16]
                  float* grad_sdf, mh_raycast_grad* out, float max_residual = INFINITY,
                  float max_slope = INFINITY) {
   RayConst R;
-  const int rc = build_ray(v, p, cTw, &R);
+  int rc = build_ray(v, p, cTw, &R);
+  if (rc) return rc;
+  rc = attach_range(v, &R);
   if (rc) return rc;
   const uint32_t n = raycast_grad_partials(R);
   HIP_TRY(ensure_partials(v, n));

@@ -23,7 +23,14 @@ struct RayConst {
   int enable_gradients;
   float inv_vs;  // 1 / voxel_size (fast empty-space test only)
   M4 cTw, wTc;
+  // Camera-z range of the allocated blocks over each 8 x 8 pixel tile
+  // (2 words per tile, float bits: zmin, zmax; then one word that, when
+  // non-zero, disables the range test), or null.  Filled by the range pass.
+  uint32_t* range;
+  uint32_t tiles_x, tiles;
 };
+// Words of a range buffer for a W x H raycast.
+inline uint32_t ray_range_words(uint32_t w, uint32_t h) { return 2u * ((w + 7) / 8) * ((h + 7) / 8) + 1u; }
 
 // Everything a batched kernel needs about one scene for one frame.
 struct SceneFrame {

@@ -60,20 +60,12 @@ __device__ __forceinline__ bool fast_corner(float pc, float inv_vs, int* c) {
   return true;
 }
 
-// GetSpatialValue restricted to the SDF (the only field the march uses).
-__device__ __forceinline__ bool ray_sdf(const View& V, BlockCache& cache, f3 pos, float inv_vs,
-                                        float* out) {
+// GetSpatialValue restricted to the SDF (the only field the march uses),
+// without the empty-space test.
+__device__ __forceinline__ bool ray_sdf_full(const View& V, BlockCache& cache, f3 pos, float* out) {
   const float vs = V.geo.voxel_size;
   const float hv = 0.5f * vs;
   const float pcx = pos.x - hv, pcy = pos.y - hv, pcz = pos.z - hv;
-  {
-    // empty-space fast path: the lowest corner's block is missing
-    int c0x, c0y, c0z;
-    if (fast_corner(pcx, inv_vs, &c0x) && fast_corner(pcy, inv_vs, &c0y) &&
-        fast_corner(pcz, inv_vs, &c0z) &&
-        cached_find(V, cache, i3{voxel_to_block1(c0x), voxel_to_block1(c0y), voxel_to_block1(c0z)}) < 0)
-      return false;
-  }
   const float qx = pos.x / vs, qy = pos.y / vs, qz = pos.z / vs;
   const float rx = qx - floorf(qx), ry = qy - floorf(qy), rz = qz - floorf(qz);
   const int cx[2] = {vox1(pcx + 0.0f, vs), vox1(pcx + vs, vs)};
@@ -107,6 +99,18 @@ __device__ __forceinline__ bool ray_sdf(const View& V, BlockCache& cache, f3 pos
   return true;
 }
 
+// GetSpatialValue's SDF with the empty-space test first.
+__device__ __forceinline__ bool ray_sdf(const View& V, BlockCache& cache, f3 pos, float inv_vs,
+                                        float* out) {
+  const float hv = 0.5f * V.geo.voxel_size;
+  int c0x, c0y, c0z;
+  if (fast_corner(pos.x - hv, inv_vs, &c0x) && fast_corner(pos.y - hv, inv_vs, &c0y) &&
+      fast_corner(pos.z - hv, inv_vs, &c0z) &&
+      cached_find(V, cache, i3{voxel_to_block1(c0x), voxel_to_block1(c0y), voxel_to_block1(c0z)}) < 0)
+    return false;
+  return ray_sdf_full(V, cache, pos, out);
+}
+
 // The accepted crossing of one ray: the bracketing valid samples (t_p, s_p)
 // and (t_k, s_k), and the last regula-falsi point m.
 struct Hit {
@@ -117,11 +121,20 @@ struct Hit {
 // crossing between the previous valid sample and the current one, three
 // regula-falsi steps (isosurface_intersection.h:24-55), then the acceptance
 // test.  A failed refinement or test continues the march.
+//
+// [t_lo, t_hi] bounds the ray parameters at which a sample can be valid (the
+// tile's block range): earlier samples are stepped over without sampling,
+// so the sample sequence t_min, t_min + step, ... stays the reference's, and
+// the march ends after t_hi.
 __device__ __forceinline__ bool find_surface(const View& V, const RayConst& R, f3 o, f3 dir, float t_min,
-                                             float t_max, BlockCache& cache, Hit* hit, uint32_t* nsamp) {
+                                             float t_max, float t_lo, float t_hi, BlockCache& cache,
+                                             Hit* hit, uint32_t* nsamp) {
   float prev_sdf = 0.0f, prev_t = 0.0f;
   bool prev_ok = false;
-  for (float t = t_min; t < t_max; t += R.step) {
+  if (!(t_lo < t_max) || t_hi < t_lo) return false;  // no block along this ray
+  float t = t_min;
+  while (t < t_lo) t += R.step;
+  for (; t < t_max && t <= t_hi; t += R.step) {
     const f3 p = mk3(o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z);
     float s;
     if (!ray_sdf(V, cache, p, R.inv_vs, &s)) continue;
@@ -159,6 +172,21 @@ __device__ __forceinline__ bool find_surface(const View& V, const RayConst& R, f
   return false;
 }
 
+// The ray parameters between which pixel (x, y) can meet an allocated block:
+// the tile's camera-z range over dir_c.z, widened by 0.1 %.  Without a range
+// pass, the whole [t_min, t_max].
+__device__ __forceinline__ void ray_bounds(const RayConst& R, uint32_t x, uint32_t y, float dcz, float t_min,
+                                           float t_max, float* t_lo, float* t_hi) {
+  *t_lo = t_min;
+  *t_hi = t_max;
+  if (!R.range || R.range[2 * R.tiles]) return;
+  const uint32_t tile = (y >> 3) * R.tiles_x + (x >> 3);
+  const float zmin = __uint_as_float(R.range[2 * tile]);
+  const float zmax = __uint_as_float(R.range[2 * tile + 1]);
+  *t_lo = fmaxf(t_min, 0.999f * (zmin / dcz));
+  *t_hi = fminf(t_max, 1.001f * (zmax / dcz));
+}
+
 // March one pixel; returns the number of valid samples taken.
 __device__ __forceinline__ uint32_t cast_pixel(const View& V, const RayConst& R, uint32_t x, uint32_t y,
                                float* res_d, f3* res_v, f3* res_n) {
@@ -175,7 +203,9 @@ __device__ __forceinline__ uint32_t cast_pixel(const View& V, const RayConst& R,
 
   BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr};
   Hit hit;
-  if (find_surface(V, R, o, dir, t_min, t_max, cache, &hit, &nsamp)) {
+  float t_lo, t_hi;
+  ray_bounds(R, x, y, dir_c.z, t_min, t_max, &t_lo, &t_hi);
+  if (find_surface(V, R, o, dir, t_min, t_max, t_lo, t_hi, cache, &hit, &nsamp)) {
     const float m = hit.m;
     const float d = m * dir_c.z;
     out_d = d;
@@ -290,7 +320,9 @@ __device__ __forceinline__ bool depth_tangent(const View& V, const RayConst& R, 
   const f3 dir = normalize(xform_dir(R.wTc, dir_c));
   Hit hit;
   uint32_t ns = 0;
-  if (!find_surface(V, R, o, dir, t_min, t_max, cache, &hit, &ns)) return false;
+  float t_lo, t_hi;
+  ray_bounds(R, x, y, dir_c.z, t_min, t_max, &t_lo, &t_hi);
+  if (!find_surface(V, R, o, dir, t_min, t_max, t_lo, t_hi, cache, &hit, &ns)) return false;
   float l = hit.tp, r = hit.tk, ls, rs;
   float dl[kTan], dr[kTan], dls[kTan], drs[kTan], dm[kTan];
   f3 g;
@@ -525,9 +557,82 @@ __global__ __launch_bounds__(64) void raycast_kernel(Src src_, const SceneFrame*
   wave_add_u64(&V.ctr->ray_samples, ns);
 }
 
+// ------------------------------------------------------------ block ranges
+// Every valid sample lies in the box of the block holding its lowest
+// trilinear corner, [8b, 8b + 8] voxels per axis; with one more voxel on each
+// side the box bounds it beyond any rounding.  Each allocated block's box is
+// projected (8 corners, all at z >= min_depth / 2, else the range test is
+// switched off for the image) and its camera-z extent merged into the 8 x 8
+// pixel tiles its projected rectangle, widened by a pixel, overlaps.  A ray
+// then meets no valid sample outside its tile's [zmin, zmax].
+template <class Src>
+__global__ __launch_bounds__(256) void ray_range_clear_kernel(Src src_, const SceneFrame* __restrict__ tab, RayConst R1) {
+  const Src src = with_table(src_, tab);
+  const RayConst& R = ray_of(src, blockIdx.y, R1);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= 2 * R.tiles; i += gridDim.x * blockDim.x)
+    R.range[i] = i == 2 * R.tiles ? 0u : ((i & 1u) ? 0u : 0x7f800000u);  // zmin +inf, zmax 0
+}
+
+template <class Src>
+__global__ __launch_bounds__(256) void ray_range_kernel(Src src_, const SceneFrame* __restrict__ tab, RayConst R1) {
+  const Src src = with_table(src_, tab);
+  const uint32_t scene = blockIdx.y;
+  const View& V = src.view(scene);
+  const RayConst& R = ray_of(src, scene, R1);
+  const uint32_t n = V.ctr->block_hwm;
+  const float vs = V.geo.voxel_size;
+  const float zcut = 0.5f * R.min_depth;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    const uint64_t key = V.block_keys[p];
+    if (key == kEmptyKey) continue;
+    const i3 b = unpack_key(key);
+    const float lo[3] = {(float)(8 * b.x - 1) * vs, (float)(8 * b.y - 1) * vs, (float)(8 * b.z - 1) * vs};
+    const float hi[3] = {(float)(8 * b.x + 9) * vs, (float)(8 * b.y + 9) * vs, (float)(8 * b.z + 9) * vs};
+    float zmin = INFINITY, zmax = -INFINITY, umin = INFINITY, umax = -INFINITY, vmin = INFINITY,
+          vmax = -INFINITY;
+    bool near = false;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const f3 pc = xform(R.cTw, mk3((c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]));
+      near = near || !(pc.z >= zcut);
+      const float z = fmaxf(pc.z, zcut);
+      const float u = R.cam.fx * pc.x / z + R.cam.cx, v = R.cam.fy * pc.y / z + R.cam.cy;
+      zmin = fminf(zmin, pc.z);
+      zmax = fmaxf(zmax, pc.z);
+      umin = fminf(umin, u);
+      umax = fmaxf(umax, u);
+      vmin = fminf(vmin, v);
+      vmax = fmaxf(vmax, v);
+    }
+    if (zmax < R.min_depth || zmin > R.max_depth) continue;  // no ray sample reaches it
+    if (near) {
+      atomicOr(&R.range[2 * R.tiles], 1u);
+      continue;
+    }
+    const float W = (float)R.cam.width, H = (float)R.cam.height;
+    if (!(umax >= -1.0f && umin <= W && vmax >= -1.0f && vmin <= H)) continue;  // off the image
+    const int x0 = (int)fmaxf(floorf(umin) - 1.0f, 0.0f), x1 = (int)fminf(ceilf(umax) + 1.0f, W - 1.0f);
+    const int y0 = (int)fmaxf(floorf(vmin) - 1.0f, 0.0f), y1 = (int)fminf(ceilf(vmax) + 1.0f, H - 1.0f);
+    const uint32_t zlo = __float_as_uint(fmaxf(zmin, 0.0f)), zhi = __float_as_uint(zmax);
+    for (int ty = y0 >> 3; ty <= (y1 >> 3); ++ty)
+      for (int tx = x0 >> 3; tx <= (x1 >> 3); ++tx) {
+        const uint32_t t = (uint32_t)ty * R.tiles_x + (uint32_t)tx;
+        atomicMin(&R.range[2 * t], zlo);
+        atomicMax(&R.range[2 * t + 1], zhi);
+      }
+  }
+}
+
+template <class Src>
+void launch_ray_range(Src src, const SceneFrame* tab, const RayConst& R, uint32_t nscenes, hipStream_t s) {
+  ray_range_clear_kernel<<<dim3(8, nscenes), 256, 0, s>>>(src, tab, R);
+  ray_range_kernel<<<dim3(128, nscenes), 256, 0, s>>>(src, tab, R);
+}
+
 void launch_raycast(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,
                     hipStream_t s) {
   dim3 grid((p.cam.width + 7) / 8, (p.cam.height + 7) / 8);
+  if (p.range) launch_ray_range(One{V}, nullptr, p, 1, s);
   raycast_kernel<<<grid, 64, 0, s>>>(One{V}, nullptr, p, depth, vertex, normal);
 }
 
@@ -539,6 +644,7 @@ void launch_raycast_backward(const View& V, const RayConst& p, const float* obs,
                              double* partials, double* out, float max_residual, float max_slope,
                              hipStream_t s) {
   dim3 grid((p.cam.width + 7) / 8, (p.cam.height + 7) / 8);
+  if (p.range) launch_ray_range(One{V}, nullptr, p, 1, s);
   raycast_backward_kernel<<<grid, 64, 0, s>>>(V, p, obs, grad_sdf, partials, max_residual,
                                               max_slope);
   slot_reduce_kernel<<<1, 64, 0, s>>>(partials, grid.x * grid.y, kGradSlots, out);
@@ -547,6 +653,7 @@ void launch_raycast_backward(const View& V, const RayConst& p, const float* obs,
 void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,
                           hipStream_t s) {
   dim3 grid((B.width + 7) / 8, (B.height + 7) / 8, B.nscenes);
+  launch_ray_range(B.src(), B.frames, RayConst{}, B.nscenes, s);
   raycast_kernel<<<grid, 64, 0, s>>>(B.src(), B.frames, RayConst{}, depth, vertex, normal);
 }
 
