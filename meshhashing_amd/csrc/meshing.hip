@@ -298,8 +298,41 @@ struct McCounts {
   }
 };
 
+// Exclusive scan of one value per lane over a 512-lane workgroup (eight
+// waves); *total gets the sum.  Every lane must call it; it synchronises the
+// workgroup twice.
+__device__ __forceinline__ uint32_t block_excl_scan512(uint32_t v, uint32_t* wave_sums, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  if (lane == 63) wave_sums[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t w = 0; w < 8; ++w) {
+    const uint32_t ws = wave_sums[w];
+    if (w < wave) before += ws;
+    all += ws;
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+// Vertex slot of edge e of the cube at local voxel o: the slot of the edge's
+// owner voxel (o + the edge's owner offset, possibly in a +x/+y/+z neighbour).
+__device__ __forceinline__ int32_t* edge_slot(const View& V, const int32_t* nb, i3 o, int e) {
+  const int ox = o.x + edge_owner_off(e, 0), oy = o.y + edge_owner_off(e, 1), oz = o.z + edge_owner_off(e, 2);
+  const int dx = ox >> 3, dy = oy >> 3, dz = oz >> 3;
+  const int32_t optr = nb[nb27(dx, dy, dz)];
+  const uint32_t oi = (uint32_t)((ox - 8 * dx) + 8 * (oy - 8 * dy) + 64 * (oz - 8 * dz));
+  return &V.vptr[((size_t)optr * kBlockVoxels + oi) * 3 + edge_axis(e)];
+}
+
 template <bool PSDF, class Src>
-__global__ __launch_bounds__(512) void mc_surfel_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1) {
+__global__ __launch_bounds__(512, 4) void mc_surfel_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1) {
   const Src src = with_table(src_, tab);
   __shared__ Shell<PSDF> S;
   __shared__ int32_t nb[27];
@@ -308,6 +341,10 @@ __global__ __launch_bounds__(512) void mc_surfel_kernel(Src src_, const SceneFra
   __shared__ int32_t qptr[kQueue];
   __shared__ uint16_t qkey[kQueue];
   __shared__ uint32_t qn;
+  __shared__ uint32_t scan_wave[8];
+  __shared__ int32_t pop_top;
+  __shared__ uint16_t items[12 * kBlockVoxels];
+  __shared__ uint32_t nitems;
   const uint32_t n = src.count();
   McCounts cnt;
   uint32_t cur = ~0u;
@@ -346,33 +383,65 @@ __global__ __launch_bounds__(512) void mc_surfel_kernel(Src src_, const SceneFra
     cnt.delta += kMc.ntri[cube] - kMc.ntri[prev];
     if ((uint32_t)cube != prev) cnt.assigned += (uint32_t)kMc.ntri[cube];
     const unsigned em = (cube != 0 && cube != 255) ? kMc.edges[cube] : 0u;
-    // uniform loop: every lane takes part in the wave-aggregated pops
-    for (int e = 0; e < 12; ++e) {
-      const bool active = (em >> e) & 1u;
-      const int ax = edge_axis(e);
-      const int offx = edge_owner_off(e, 0), offy = edge_owner_off(e, 1), offz = edge_owner_off(e, 2);
-      const int ox = o.x + offx, oy = o.y + offy, oz = o.z + offz;
+    // The crossed edges of the block's cubes become a dense item list
+    // (voxel | edge << 9), so the slot work below runs one item per lane
+    // instead of twelve edge rounds on every cube.
+    {
+      const uint32_t mine = (uint32_t)__popc(em);
+      uint32_t total;
+      uint32_t at = block_excl_scan512(mine, scan_wave, &total);
+      for (unsigned m = em; m; m &= m - 1u) items[at++] = (uint16_t)(i | ((uint32_t)(__ffs(m) - 1) << 9));
+      if (threadIdx.x == 0) nitems = total;
+    }
+    __syncthreads();
+    const uint32_t n_items = nitems;
+    for (uint32_t c0 = 0; c0 < n_items; c0 += blockDim.x) {
+      // Vertex slot of each item: read it; claim it if free (CAS to the
+      // lock value); pop fresh vertices for all claims of the workgroup
+      // with one heap operation; publish them.
+      const uint32_t jt = c0 + threadIdx.x;
+      const bool has = jt < n_items;
+      i3 oo{0, 0, 0};
+      int e = 0;
       int32_t* slot = nullptr;
       int32_t cur = kFreePtr;
       bool claimed = false;
-      if (active) {
-        const int dx = ox >> 3, dy = oy >> 3, dz = oz >> 3;
-        const int32_t optr = nb[nb27(dx, dy, dz)];
-        const uint32_t oi = (uint32_t)((ox - 8 * dx) + 8 * (oy - 8 * dy) + 64 * (oz - 8 * dz));
-        slot = &V.vptr[((size_t)optr * kBlockVoxels + oi) * 3 + ax];
+      if (has) {
+        const uint32_t it = items[jt];
+        oo = devectorize(it & 511u);
+        e = (int)(it >> 9);
+        slot = edge_slot(V, nb, oo, e);
         cur = *slot;
         if (cur == kFreePtr) {
-          cur = atomicCAS(slot, kFreePtr, kLockPtr);
-          claimed = cur == kFreePtr;
+          const int32_t old = atomicCAS(slot, kFreePtr, kLockPtr);
+          claimed = old == kFreePtr;
+          if (!claimed) cur = old;
         }
       }
-      const int32_t fresh = heap_pop_wave(&V.ctr->vertex_top, V.vertex_heap, claimed,
-                                          &V.ctr->flags, MH_STATUS_VERTEX_HEAP_EXHAUSTED);
-      if (claimed) {
-        atomicExch(slot, fresh);
-        cur = fresh;
+      uint32_t nclaims;
+      const uint32_t rank = block_excl_scan512(claimed ? 1u : 0u, scan_wave, &nclaims);
+      if (nclaims) {  // uniform
+        if (threadIdx.x == 0) {
+          const int32_t top = atomicSub(&V.ctr->vertex_top, (int32_t)nclaims);
+          const int32_t avail = top + 1 > 0 ? top + 1 : 0;
+          if (avail < (int32_t)nclaims) {
+            atomicAdd(&V.ctr->vertex_top, (int32_t)nclaims - avail);
+            atomicOr(&V.ctr->flags, MH_STATUS_VERTEX_HEAP_EXHAUSTED);
+          }
+          pop_top = top;
+        }
+        __syncthreads();
+        if (claimed) {
+          const int32_t hidx = pop_top - (int32_t)rank;
+          const int32_t fresh = hidx >= 0 ? V.vertex_heap[hidx] : kFreePtr;
+          atomicExch(slot, fresh);
+          cur = fresh;
+        }
       }
-      if (!active || cur < 0) continue;
+      if (!has || cur < 0) continue;
+      const int ax = edge_axis(e);
+      const int offx = edge_owner_off(e, 0), offy = edge_owner_off(e, 1), offz = edge_owner_off(e, 2);
+      const int ox = oo.x + offx, oy = oo.y + offy, oz = oo.z + offz;
       // writer rule: offsets across the edge in the two other axes (b1 < b2)
       const int b1 = ax == 0 ? 1 : 0, b2 = ax == 2 ? 1 : 2;
       const int own[3] = {offx, offy, offz};
@@ -446,10 +515,29 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
   __shared__ uint8_t cfg[kCfg * kCfg * kCfg];
   __shared__ int counts[2];
   const uint32_t n = src.count();
+  static_assert(kBlockVoxels == 2 * 256, "two voxels per lane");
   for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
     uint32_t scene;
     const int32_t ptr = src.item(ci, &scene);
     const View& V = src.view(scene);
+    // This lane's vertex slots (voxels threadIdx.x and threadIdx.x + 256);
+    // a block without any is done at once (no slot to free, no surfels).
+    int32_t vps[2][3];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax) {
+        vps[r][ax] = V.vptr[3 * ((size_t)ptr * kBlockVoxels + threadIdx.x + 256u * r) + ax];
+        any = any || vps[r][ax] >= 0;
+      }
+    if (!__syncthreads_or(any)) {
+      if (threadIdx.x == 0) {
+        V.surfel[2 * ptr + 0] = 0;
+        V.surfel[2 * ptr + 1] = 0;
+      }
+      continue;
+    }
     const i3 b = unpack_key(V.block_keys[ptr]);
     if (threadIdx.x < 8) {
       const int t = (int)threadIdx.x;
@@ -466,12 +554,14 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
     }
     __syncthreads();
     int inner = 0, boundary = 0;
-    for (uint32_t i = threadIdx.x; i < (uint32_t)kBlockVoxels; i += blockDim.x) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const uint32_t i = threadIdx.x + 256u * r;
       const size_t idx = (size_t)ptr * kBlockVoxels + i;
       const i3 o = devectorize(i);
       const bool is_inner = o.x >= 1 && o.y >= 1 && o.z >= 1 && o.x < 7 && o.y < 7 && o.z < 7;
       for (int ax = 0; ax < 3; ++ax) {  // uniform: wave-aggregated pushes
-        const int32_t vp = V.vptr[3 * idx + ax];
+        const int32_t vp = vps[r][ax];
         int refs = 0;
         if (vp >= 0) {
           if (is_inner) ++inner; else ++boundary;
@@ -549,27 +639,6 @@ __device__ __forceinline__ int32_t cube_edge_vertex(const View& V, const int32_t
   if (bp < 0) return -1;
   const uint32_t li = (uint32_t)((ox - 8 * dx) + 8 * (oy - 8 * dy) + 64 * (oz - 8 * dz));
   return V.vptr[((size_t)bp * kBlockVoxels + li) * 3 + edge_axis(e)];
-}
-
-// Exclusive scan of one value per lane over a 512-lane workgroup.
-__device__ __forceinline__ uint32_t block_excl_scan512(uint32_t v, uint32_t* wave_sums, uint32_t* total) {
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  uint32_t x = v;
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off);
-    if (lane >= (uint32_t)off) x += y;
-  }
-  if (lane == 63) wave_sums[wave] = x;
-  __syncthreads();
-  uint32_t before = 0, all = 0;
-  for (uint32_t w = 0; w < 8; ++w) {
-    const uint32_t ws = wave_sums[w];
-    if (w < wave) before += ws;
-    all += ws;
-  }
-  __syncthreads();
-  *total = all;
-  return before + x - v;
 }
 
 template <bool EMIT>
