@@ -234,8 +234,10 @@ __device__ __forceinline__ int cidx(int x, int y, int z) {
 // AllocateVertexWithMutex's payload (:45-72).
 constexpr uint32_t kQueue = 2048;
 template <bool PSDF>
-__device__ void write_vertex(const Shell<PSDF>& S, const View& V, const McConst& mc, i3 base,
-                             uint32_t key, int32_t vp, bool* oob) {
+// Returns true when a stencil sample fell outside the staged shell.
+__device__ bool write_vertex(const Shell<PSDF>& S, const View& V, const McConst& mc, i3 base,
+                             uint32_t key, int32_t vp) {
+  bool oob = false;
   const Geo& g = V.geo;
   const int ox = (int)(key & 15u), oy = (int)((key >> 4) & 15u), oz = (int)((key >> 8) & 15u);
   const int ax = (int)(key >> 12);
@@ -254,7 +256,7 @@ __device__ void write_vertex(const Shell<PSDF>& S, const View& V, const McConst&
     p = mk3(pA.x + mu * (pB.x - pA.x), pA.y + mu * (pB.y - pA.y), pA.z + mu * (pB.z - pA.z));
   }
   Vertex vt;
-  vertex_payload<PSDF>(S, g, base, p, &vt, oob);
+  vertex_payload<PSDF>(S, g, base, p, &vt, &oob);
   Vertex* dst = &V.vertices[vp];
   *reinterpret_cast<float4*>(&dst->px) = make_float4(vt.px, vt.py, vt.pz, vt.radius);
   dst->nx = vt.nx;
@@ -263,6 +265,7 @@ __device__ void write_vertex(const Shell<PSDF>& S, const View& V, const McConst&
   dst->cr = vt.cr;
   dst->cg = vt.cg;
   dst->cb = vt.cb;
+  return oob;
 }
 
 // SurfelExtractionKernel (marching_cubes.cu:75-170): cube index of every
@@ -442,18 +445,20 @@ __global__ __launch_bounds__(512, 4) void mc_surfel_kernel(Src src_, const Scene
       const int ax = edge_axis(e);
       const int offx = edge_owner_off(e, 0), offy = edge_owner_off(e, 1), offz = edge_owner_off(e, 2);
       const int ox = oo.x + offx, oy = oo.y + offy, oz = oo.z + offz;
-      // writer rule: offsets across the edge in the two other axes (b1 < b2)
+      // writer rule: offsets across the edge in the two other axes (b1 < b2;
+      // selected without indexing, e being a run-time value here)
       const int b1 = ax == 0 ? 1 : 0, b2 = ax == 2 ? 1 : 2;
-      const int own[3] = {offx, offy, offz};
-      const int kself = own[b1] + 2 * own[b2];
+      const int own1 = b1 == 0 ? offx : offy, own2 = b2 == 1 ? offy : offz;
+      const int kself = own1 + 2 * own2;
       // A slot claimed in this pass is also written by its claimer: the
       // designated cube may have read it while still locked and skipped it.
       bool writer = true;
       for (int k = 0; k < kself && writer && !claimed; ++k) {
-        int q[3] = {ox, oy, oz};
-        q[b1] -= k & 1;
-        q[b2] -= k >> 1;
-        if (cv[cidx(q[0], q[1], q[2])]) writer = false;
+        const int d1 = k & 1, d2 = k >> 1;
+        const int qx = ox - (b1 == 0 ? d1 : 0);
+        const int qy = oy - (b1 == 1 ? d1 : 0) - (b2 == 1 ? d2 : 0);
+        const int qz = oz - (b2 == 2 ? d2 : 0);
+        if (cv[cidx(qx, qy, qz)]) writer = false;
       }
       if (!writer) continue;
       // queue the payload; all lanes of the block work the queue below
@@ -463,14 +468,14 @@ __global__ __launch_bounds__(512, 4) void mc_surfel_kernel(Src src_, const Scene
         qptr[q] = cur;
         qkey[q] = (uint16_t)key;
       } else {
-        write_vertex<PSDF>(S, V, mc, base, key, cur, &cnt.oob);
+        cnt.oob = write_vertex<PSDF>(S, V, mc, base, key, cur) || cnt.oob;
         ++cnt.writes;
       }
     }
     __syncthreads();
     const uint32_t nq = qn < kQueue ? qn : kQueue;
     for (uint32_t j = threadIdx.x; j < nq; j += blockDim.x) {
-      write_vertex<PSDF>(S, V, mc, base, qkey[j], qptr[j], &cnt.oob);
+      cnt.oob = write_vertex<PSDF>(S, V, mc, base, qkey[j], qptr[j]) || cnt.oob;
       ++cnt.writes;
     }
     __syncthreads();

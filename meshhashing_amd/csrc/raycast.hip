@@ -16,9 +16,51 @@
 
 namespace mh {
 
+// Block lookups shared by the 64 rays of a workgroup (one wave): a
+// direct-mapped LDS table that a lane fills after a hash probe, so a block
+// is probed about once per wave rather than once per ray.  An entry is one
+// 64-bit word, read and written by single LDS instructions, so a reader
+// always gets a block and its own index whatever order the wave's divergent
+// paths run in: bits 0-23 hold the pool index + 3 (0 marks an empty slot),
+// bits 24-63 the block coordinates minus the low bits that chose the slot
+// (x, y: 3 bits, z: 2 bits), which makes (slot, entry) name one block.
+// Blocks beyond +-2^15 on an axis bypass the table.  Of the lanes that miss
+// on one slot in the same instruction, only the one whose id survives in
+// `owner` writes it.
+constexpr int kWaveTable = 256;
+struct WaveTable {
+  uint64_t e[kWaveTable];
+  uint32_t owner[kWaveTable];
+};
+__device__ __forceinline__ void wave_table_init(WaveTable* t) {
+  for (uint32_t i = threadIdx.x; i < (uint32_t)kWaveTable; i += 64u) t->e[i] = 0ull;
+  __syncthreads();
+}
+
+__device__ __forceinline__ int32_t table_find(const View& V, WaveTable* t, i3 b) {
+  const int lim = 1 << 15;
+  if (b.x < -lim || b.x >= lim || b.y < -lim || b.y >= lim || b.z < -lim || b.z >= lim)
+    return coord_ok(b) ? hash_find(V, b) : kFreePtr;
+  const uint32_t slot = (uint32_t)(b.x & 7) | ((uint32_t)(b.y & 7) << 3) | ((uint32_t)(b.z & 3) << 6);
+  const uint64_t tag = (uint64_t)((uint32_t)(b.x >> 3) & 0x1FFFu) |
+                       ((uint64_t)((uint32_t)(b.y >> 3) & 0x1FFFu) << 13) |
+                       ((uint64_t)((uint32_t)(b.z >> 2) & 0x3FFFu) << 26);
+  const uint64_t e = t->e[slot];
+  if (e != 0ull && (e >> 24) == tag) return (int32_t)(e & 0xFFFFFFu) - 3;
+  const int32_t p = coord_ok(b) ? hash_find(V, b) : kFreePtr;
+  if (p >= (1 << 24) - 3) return p;  // does not fit the entry
+  const uint32_t lane = threadIdx.x & 63u;
+  volatile uint32_t* owner = t->owner;
+  owner[slot] = lane;
+  if (owner[slot] == lane) t->e[slot] = (tag << 24) | (uint64_t)(uint32_t)(p + 3);
+  return p;
+}
+
+// The last two lookups of one ray (hits and misses), in registers.
 struct BlockCache {
   uint64_t k0, k1;
   int32_t p0, p1;
+  WaveTable* t;
 };
 
 __device__ __forceinline__ int32_t cached_find(const View& V, BlockCache& c, i3 b) {
@@ -33,7 +75,7 @@ __device__ __forceinline__ int32_t cached_find(const View& V, BlockCache& c, i3 
     c.p1 = tp;
     return c.p0;
   }
-  const int32_t p = coord_ok(b) ? hash_find(V, b) : kFreePtr;
+  const int32_t p = table_find(V, c.t, b);
   c.k1 = c.k0;
   c.p1 = c.p0;
   c.k0 = k;
@@ -111,7 +153,86 @@ __device__ __forceinline__ bool ray_sdf(const View& V, BlockCache& cache, f3 pos
   return ray_sdf_full(V, cache, pos, out);
 }
 
-// The accepted crossing of one ray: the bracketing valid samples (t_p, s_p)
+// The eight blocks around a lowest-corner block b: b + (i>>2 & 1, i>>1 & 1,
+// i & 1) for entry i, each looked up when a sample first needs it and kept
+// while the march stays with b.
+constexpr int32_t kNotLooked = INT32_MIN;
+struct Octant {
+  int bx, by, bz;
+  int32_t p[8];
+};
+
+// GetSpatialValue's SDF for a march sample (spatial_query.h:23-66).  The
+// corner voxels come from a multiply by 1/vs when the quotient is more than
+// 1e-3 voxel from a rounding tie (the same integers as the reference's
+// divisions, see fast_corner; the upper corner is then the lower + 1, the
+// sum pc + vs being within 1.3e-4 voxel of exact), else from the divisions;
+// the ratios come from the division, as the reference computes them.  A
+// missing lowest-corner block ends the sample after one compare while the
+// octant is cached, which is how empty space inside the tile range costs
+// little.
+__device__ __forceinline__ bool march_sample(const View& V, BlockCache& cache, Octant& oc, f3 pos,
+                                             float inv_vs, float* out) {
+  const float vs = V.geo.voxel_size;
+  const float hv = 0.5f * vs;
+  const float pcx = pos.x - hv, pcy = pos.y - hv, pcz = pos.z - hv;
+  int cx0, cy0, cz0, cx1, cy1, cz1;
+  if (fast_corner(pcx, inv_vs, &cx0) && fast_corner(pcy, inv_vs, &cy0) && fast_corner(pcz, inv_vs, &cz0)) {
+    cx1 = cx0 + 1;
+    cy1 = cy0 + 1;
+    cz1 = cz0 + 1;
+  } else {
+    cx0 = vox1(pcx + 0.0f, vs);
+    cx1 = vox1(pcx + vs, vs);
+    cy0 = vox1(pcy + 0.0f, vs);
+    cy1 = vox1(pcy + vs, vs);
+    cz0 = vox1(pcz + 0.0f, vs);
+    cz1 = vox1(pcz + vs, vs);
+  }
+  const int bx = voxel_to_block1(cx0), by = voxel_to_block1(cy0), bz = voxel_to_block1(cz0);
+  if (bx != oc.bx || by != oc.by || bz != oc.bz) {
+    oc.bx = bx;
+    oc.by = by;
+    oc.bz = bz;
+    oc.p[0] = cached_find(V, cache, i3{bx, by, bz});
+#pragma unroll
+    for (int i = 1; i < 8; ++i) oc.p[i] = kNotLooked;
+  }
+  if (oc.p[0] < 0) return false;
+  const bool xx = voxel_to_block1(cx1) != bx, yy = voxel_to_block1(cy1) != by, zz = voxel_to_block1(cz1) != bz;
+#pragma unroll
+  for (int i = 1; i < 8; ++i) {
+    const bool need = (!(i & 4) || xx) && (!(i & 2) || yy) && (!(i & 1) || zz);
+    if (!need) continue;
+    if (oc.p[i] == kNotLooked) oc.p[i] = cached_find(V, cache, i3{bx + ((i >> 2) & 1), by + ((i >> 1) & 1), bz + (i & 1)});
+    if (oc.p[i] < 0) return false;
+  }
+  // block of each corner (bit 2: x, bit 1: y, bit 0: z set), by selects
+  const int32_t p0 = oc.p[0];
+  const int32_t p1 = zz ? oc.p[1] : p0;
+  const int32_t p2 = yy ? oc.p[2] : p0;
+  const int32_t p3 = yy ? (zz ? oc.p[3] : oc.p[2]) : p1;
+  const int32_t p4 = xx ? oc.p[4] : p0;
+  const int32_t p5 = xx ? (zz ? oc.p[5] : oc.p[4]) : p1;
+  const int32_t p6 = xx ? (yy ? oc.p[6] : oc.p[4]) : p2;
+  const int32_t p7 = xx ? (yy ? (zz ? oc.p[7] : oc.p[6]) : (zz ? oc.p[5] : oc.p[4])) : p3;
+  const int32_t pb[8] = {p0, p1, p2, p3, p4, p5, p6, p7};
+  const int cx[2] = {cx0, cx1}, cy[2] = {cy0, cy1}, cz[2] = {cz0, cz1};
+  const float qx = pos.x / vs, qy = pos.y / vs, qz = pos.z / vs;
+  const float rx = qx - floorf(qx), ry = qy - floorf(qy), rz = qz - floorf(qz);
+  float sdf = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int mx = (i >> 2) & 1, my = (i >> 1) & 1, mz = i & 1;
+    const uint32_t li = (uint32_t)((cx[mx] & 7) + 8 * (cy[my] & 7) + 64 * (cz[mz] & 7));
+    const float w = (mx ? rx : 1.0f - rx) * (my ? ry : 1.0f - ry) * (mz ? rz : 1.0f - rz);
+    sdf += w * V.sdf[(size_t)pb[i] * kBlockVoxels + li];
+  }
+  *out = sdf;
+  return true;
+}
+
+// The accepted crossing of one ray
 // and (t_k, s_k), and the last regula-falsi point m.
 struct Hit {
   float tp, sp, tk, sk, m;
@@ -134,14 +255,17 @@ __device__ __forceinline__ bool find_surface(const View& V, const RayConst& R, f
   if (!(t_lo < t_max) || t_hi < t_lo) return false;  // no block along this ray
   float t = t_min;
   while (t < t_lo) t += R.step;
+  Octant oc{INT32_MIN, INT32_MIN, INT32_MIN, {kNotLooked, kNotLooked, kNotLooked, kNotLooked,
+                                                kNotLooked, kNotLooked, kNotLooked, kNotLooked}};
   for (; t < t_max && t <= t_hi; t += R.step) {
     const f3 p = mk3(o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z);
     float s;
-    if (!ray_sdf(V, cache, p, R.inv_vs, &s)) continue;
+    if (!march_sample(V, cache, oc, p, R.inv_vs, &s)) continue;
+    const float tg = t;
     ++*nsamp;
     if (prev_ok && prev_sdf > 0.0f && s < 0.0f) {
       // BisectionIntersection: three regula-falsi steps.
-      float l = prev_t, r = t, m = (l + r) * 0.5f;
+      float l = prev_t, r = tg, m = (l + r) * 0.5f;
       float l_sdf = prev_sdf, r_sdf = s;
       bool found = true;
       for (int k = 0; k < 3; ++k) {
@@ -161,12 +285,12 @@ __device__ __forceinline__ bool find_surface(const View& V, const RayConst& R, f
         }
       }
       if (found && fabsf(prev_sdf - s) < R.sample_sdf_threshold && fabsf(s) < R.sdf_threshold) {
-        *hit = Hit{prev_t, prev_sdf, t, s, m};
+        *hit = Hit{prev_t, prev_sdf, tg, s, m};
         return true;
       }
     }
     prev_sdf = s;
-    prev_t = t;
+    prev_t = tg;
     prev_ok = true;
   }
   return false;
@@ -189,7 +313,7 @@ __device__ __forceinline__ void ray_bounds(const RayConst& R, uint32_t x, uint32
 
 // March one pixel; returns the number of valid samples taken.
 __device__ __forceinline__ uint32_t cast_pixel(const View& V, const RayConst& R, uint32_t x, uint32_t y,
-                               float* res_d, f3* res_v, f3* res_n) {
+                               WaveTable* table, float* res_d, f3* res_v, f3* res_n) {
   uint32_t nsamp = 0;
   float out_d = -INFINITY;
   f3 out_v = mk3(-INFINITY, -INFINITY, -INFINITY);
@@ -201,7 +325,7 @@ __device__ __forceinline__ uint32_t cast_pixel(const View& V, const RayConst& R,
   const f3 o = xform(R.wTc, mk3(0.0f, 0.0f, 0.0f));
   const f3 dir = normalize(xform_dir(R.wTc, dir_c));
 
-  BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr};
+  BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr, table};
   Hit hit;
   float t_lo, t_hi;
   ray_bounds(R, x, y, dir_c.z, t_min, t_max, &t_lo, &t_hi);
@@ -386,6 +510,8 @@ __global__ __launch_bounds__(64) void raycast_backward_kernel(View V, RayConst R
                                                               float* __restrict__ grad_sdf,
                                                               double* __restrict__ partials,
                                                               float max_residual, float max_slope) {
+  __shared__ WaveTable table;
+  wave_table_init(&table);
   const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7);
   const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
   double acc[kGradSlots];
@@ -393,7 +519,7 @@ __global__ __launch_bounds__(64) void raycast_backward_kernel(View V, RayConst R
   for (int k = 0; k < kGradSlots; ++k) acc[k] = 0.0;
   if (x < R.cam.width && y < R.cam.height) {
     const float od = obs[y * R.cam.width + x];
-    BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr};
+    BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr, &table};
     float d, dd[kTan];
     f3 X[5];
     if (od > 0.0f && od < INFINITY && depth_tangent(V, R, x, y, cache, &d, dd, X) &&
@@ -455,6 +581,8 @@ __global__ void slot_reduce_kernel(const double* __restrict__ partials, uint32_t
 // triangle (21), b (6), err, count.
 __global__ __launch_bounds__(64) void point_to_psdf_kernel(View V, M4 wTc, float inv_vs, int corrected,
                                                            double* __restrict__ partials) {
+  __shared__ WaveTable table;
+  wave_table_init(&table);
   const uint32_t x = 2 * (blockIdx.x * 8 + (threadIdx.x & 7)) + 1;
   const uint32_t y = 2 * (blockIdx.y * 8 + (threadIdx.x >> 3)) + 1;
   double acc[kP2SSlots];
@@ -464,7 +592,7 @@ __global__ __launch_bounds__(64) void point_to_psdf_kernel(View V, M4 wTc, float
     const float depth = V.depth[(size_t)y * V.cam.width + x];
     if (!(depth == -INFINITY || depth == 0.0f || depth >= V.geo.sdf_upper_bound)) {
       const f3 pw = xform(wTc, reproject(V.cam, x, y, depth));
-      BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr};
+      BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr, &table};
       float d;
       if (ray_sdf(V, cache, pw, inv_vs, &d)) {
         const float h = 0.5f * V.geo.voxel_size;
@@ -531,6 +659,8 @@ __global__ __launch_bounds__(64) void raycast_kernel(Src src_, const SceneFrame*
                                                      float* __restrict__ vertex_out,
                                                      float* __restrict__ normal_out) {
   const Src src = with_table(src_, tab);
+  __shared__ WaveTable table;
+  wave_table_init(&table);
   const uint32_t scene = blockIdx.z;
   const View& V = src.view(scene);
   const RayConst& R = ray_of(src, scene, R1);
@@ -541,7 +671,7 @@ __global__ __launch_bounds__(64) void raycast_kernel(Src src_, const SceneFrame*
     const size_t pix = (size_t)scene * R.cam.width * R.cam.height + y * R.cam.width + x;
     float d;
     f3 vv, nn;
-    ns = cast_pixel(V, R, x, y, &d, &vv, &nn);
+    ns = cast_pixel(V, R, x, y, &table, &d, &vv, &nn);
     if (depth_out) depth_out[pix] = d;
     if (vertex_out) {
       vertex_out[3 * pix + 0] = vv.x;
