@@ -301,10 +301,10 @@ struct McCounts {
   }
 };
 
-// Exclusive scan of one value per lane over a 512-lane workgroup (eight
+// Exclusive scan of one value per lane over the workgroup (up to eight
 // waves); *total gets the sum.  Every lane must call it; it synchronises the
 // workgroup twice.
-__device__ __forceinline__ uint32_t block_excl_scan512(uint32_t v, uint32_t* wave_sums, uint32_t* total) {
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wave_sums, uint32_t* total) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   uint32_t x = v;
   for (int off = 1; off < 64; off <<= 1) {
@@ -314,7 +314,8 @@ __device__ __forceinline__ uint32_t block_excl_scan512(uint32_t v, uint32_t* wav
   if (lane == 63) wave_sums[wave] = x;
   __syncthreads();
   uint32_t before = 0, all = 0;
-  for (uint32_t w = 0; w < 8; ++w) {
+  const uint32_t nw = blockDim.x >> 6;
+  for (uint32_t w = 0; w < nw; ++w) {
     const uint32_t ws = wave_sums[w];
     if (w < wave) before += ws;
     all += ws;
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(512, 4) void mc_surfel_kernel(Src src_, const Scene
     {
       const uint32_t mine = (uint32_t)__popc(em);
       uint32_t total;
-      uint32_t at = block_excl_scan512(mine, scan_wave, &total);
+      uint32_t at = block_excl_scan(mine, scan_wave, &total);
       for (unsigned m = em; m; m &= m - 1u) items[at++] = (uint16_t)(i | ((uint32_t)(__ffs(m) - 1) << 9));
       if (threadIdx.x == 0) nitems = total;
     }
@@ -422,7 +423,7 @@ __global__ __launch_bounds__(512, 4) void mc_surfel_kernel(Src src_, const Scene
         }
       }
       uint32_t nclaims;
-      const uint32_t rank = block_excl_scan512(claimed ? 1u : 0u, scan_wave, &nclaims);
+      const uint32_t rank = block_excl_scan(claimed ? 1u : 0u, scan_wave, &nclaims);
       if (nclaims) {  // uniform
         if (threadIdx.x == 0) {
           const int32_t top = atomicSub(&V.ctr->vertex_top, (int32_t)nclaims);
@@ -519,6 +520,8 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
   __shared__ int32_t nb[8];  // blocks at offsets (dx, dy, dz) in {-1, 0}^3
   __shared__ uint8_t cfg[kCfg * kCfg * kCfg];
   __shared__ int counts[2];
+  __shared__ uint32_t wave_sums[4];
+  __shared__ int32_t push_top;
   const uint32_t n = src.count();
   static_assert(kBlockVoxels == 2 * 256, "two voxels per lane");
   for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
@@ -559,13 +562,14 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
     }
     __syncthreads();
     int inner = 0, boundary = 0;
+    unsigned release = 0;  // bit 3r + ax: slot to free
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const uint32_t i = threadIdx.x + 256u * r;
-      const size_t idx = (size_t)ptr * kBlockVoxels + i;
       const i3 o = devectorize(i);
       const bool is_inner = o.x >= 1 && o.y >= 1 && o.z >= 1 && o.x < 7 && o.y < 7 && o.z < 7;
-      for (int ax = 0; ax < 3; ++ax) {  // uniform: wave-aggregated pushes
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax) {
         const int32_t vp = vps[r][ax];
         int refs = 0;
         if (vp >= 0) {
@@ -581,14 +585,27 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
           }
         }
         // (the count itself is not stored: readers derive it the same way)
-        const bool release = vp >= 0 && refs == 0;
-        if (release) {
+        if (vp >= 0 && refs == 0) release |= 1u << (3 * r + ax);
+      }
+    }
+    // the block's freed slots go back to the heap with one counter update
+    uint32_t nfree;
+    uint32_t rank = block_excl_scan((uint32_t)__popc(release), wave_sums, &nfree);
+    if (nfree) {  // uniform
+      if (threadIdx.x == 0) push_top = atomicAdd(&V.ctr->vertex_top, (int32_t)nfree);
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+          if (!((release >> (3 * r + ax)) & 1u)) continue;
+          const int32_t vp = vps[r][ax];
           Vertex z{};
           V.vertices[vp] = z;
-          V.vptr[3 * idx + ax] = kFreePtr;
+          V.vptr[3 * ((size_t)ptr * kBlockVoxels + threadIdx.x + 256u * r) + ax] = kFreePtr;
+          V.vertex_heap[push_top + 1 + (int32_t)rank] = vp;
+          ++rank;
         }
-        heap_push_wave(&V.ctr->vertex_top, V.vertex_heap, release, vp);
-      }
     }
     if (inner) atomicAdd(&counts[0], inner);
     if (boundary) atomicAdd(&counts[1], boundary);
@@ -683,7 +700,7 @@ __global__ __launch_bounds__(512) void compact_tris_kernel(View V, uint32_t n, u
       ++keep;
     }
     uint32_t total;
-    const uint32_t excl = block_excl_scan512(keep, wave_sums, &total);
+    const uint32_t excl = block_excl_scan(keep, wave_sums, &total);
     if (!EMIT) {
       if (threadIdx.x == 0) blk_tris[ci] = total;
     } else {

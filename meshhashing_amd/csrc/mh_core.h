@@ -349,48 +349,6 @@ __device__ __forceinline__ int32_t alloc_block(const View& V) {
   return p;
 }
 
-// Wave-aggregated LIFO heap operations on a free-index stack whose top is
-// *top (the reference's Alloc/Free, hash_table.h:279-292, mesh.h:53-78, one
-// atomic per thread on one counter).  Here one lane per wave does a single
-// atomic for all lanes that need an index; lanes get consecutive stack
-// slots by rank.  Every lane of the wave must make the call (need = false
-// for lanes that want nothing).  Pops and pushes never share a kernel.
-__device__ __forceinline__ int32_t heap_pop_wave(int32_t* top, const int32_t* heap, bool need,
-                                                 uint32_t* flags, uint32_t bit) {
-  const unsigned long long mask = __ballot(need);
-  if (mask == 0ull) return kFreePtr;
-  const uint32_t lane = threadIdx.x & 63u;
-  const int leader = __ffsll((long long)mask) - 1;
-  const int cnt = __popcll(mask);
-  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-  int32_t old = 0;
-  if ((int)lane == leader) {
-    old = atomicSub(top, cnt);
-    const int32_t avail = old + 1 > 0 ? old + 1 : 0;
-    if (avail < cnt) {
-      atomicAdd(top, cnt - avail);
-      atomicOr(flags, bit);
-    }
-  }
-  old = __shfl(old, leader);
-  if (!need) return kFreePtr;
-  const int32_t idx = old - rank;
-  return idx >= 0 ? heap[idx] : kFreePtr;
-}
-
-__device__ __forceinline__ void heap_push_wave(int32_t* top, int32_t* heap, bool need, int32_t p) {
-  const unsigned long long mask = __ballot(need);
-  if (mask == 0ull) return;
-  const uint32_t lane = threadIdx.x & 63u;
-  const int leader = __ffsll((long long)mask) - 1;
-  const int cnt = __popcll(mask);
-  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-  int32_t old = 0;
-  if ((int)lane == leader) old = atomicAdd(top, cnt);
-  old = __shfl(old, leader);
-  if (need) heap[old + 1 + rank] = p;
-}
-
 // Local voxel index <-> offset, geometry_helper.h:111-129.
 MH_HD i3 devectorize(uint32_t i) {
   return i3{(int)(i % 8u), (int)((i % 64u) / 8u), (int)(i / 64u)};
