@@ -364,38 +364,70 @@ __device__ __forceinline__ uint32_t blend_color(uint32_t c, uint32_t d) {
   return out;
 }
 
-// Fuse the observation into voxel i of worklist block ptr; true if the
-// voxel's state changed.
-__device__ __forceinline__ bool integrate_voxel(const View& V, const i3 b, int32_t ptr, uint32_t i) {
+// Fuse the observation into the K voxels lt, lt + L, ... of worklist block
+// ptr held by one lane; returns how many changed.  Three rounds, so that the
+// lane's loads are in flight together: projections and depth reads; the
+// update tests and voxel reads; the updates and stores.
+template <int K, uint32_t L>
+__device__ __forceinline__ uint32_t integrate_voxels(const View& V, const i3 b, int32_t ptr, uint32_t lt) {
   const Geo& g = V.geo;
   const Cam& c = V.cam;
-  const i3 o = devectorize(i);
-  const i3 vp = i3{b.x * kBlockSide + o.x, b.y * kBlockSide + o.y, b.z * kBlockSide + o.z};
-  const f3 cp = xform(V.cTw, voxel_to_world(g, vp));
-  const uint32_t u = (uint32_t)f2i(proj_u(c, cp) + 0.5f);
-  const uint32_t v = (uint32_t)f2i(proj_v(c, cp) + 0.5f);
-  if (u >= c.width || v >= c.height) return false;
-  const float depth = V.depth[v * c.width + u];
-  if (depth == -INFINITY || depth == 0.0f || depth >= g.sdf_upper_bound) return false;
-  const float sdf_dec = depth - cp.z;
-  const float wgt = fmaxf(10.0f * g.weight_sample * (1.0f - normalize_depth(depth, c.min_depth, c.max_depth)), 1.0f);
-  const float tr = truncate_distance(g, depth);
-  if (sdf_dec <= -tr) return false;
-  const double vsd = (double)g.voxel_size;
-  const double X = (double)vp.x * vsd, Y = (double)vp.y * vsd, Z = (double)vp.z * vsd;
-  const double z64 = fma(V.cTw64.m[8], X, fma(V.cTw64.m[9], Y, fma(V.cTw64.m[10], Z, V.cTw64.m[11])));
-  double s = (double)depth - z64;
-  const double trd = (double)tr;
-  s = s > trd ? trd : (s < -trd ? -trd : s);
-  const float sdf = (float)s;
-  const size_t idx = (size_t)ptr * kBlockVoxels + i;
-  const float so = V.sdf[idx];
-  const float wo = V.weight[idx];
-  V.sdf[idx] = (so * wo + sdf * wgt) / (wo + wgt);
-  V.weight[idx] = wo + wgt;
-  const uint32_t dc = V.has_color ? V.color_in[v * c.width + u] : (255u << 8);  // (0, 255, 0)
-  V.color[idx] = blend_color(V.color[idx], dc);
-  return true;
+  uint32_t pix[K];
+  bool live[K];
+  float depth[K], zc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const i3 o = devectorize(lt + (uint32_t)k * L);
+    const i3 vp = i3{b.x * kBlockSide + o.x, b.y * kBlockSide + o.y, b.z * kBlockSide + o.z};
+    const f3 cp = xform(V.cTw, voxel_to_world(g, vp));
+    const uint32_t u = (uint32_t)f2i(proj_u(c, cp) + 0.5f);
+    const uint32_t v = (uint32_t)f2i(proj_v(c, cp) + 0.5f);
+    live[k] = u < c.width && v < c.height;
+    pix[k] = live[k] ? v * c.width + u : 0u;
+    zc[k] = cp.z;
+    depth[k] = live[k] ? V.depth[pix[k]] : 0.0f;
+  }
+  float so[K], wo[K], wgt[K], sdf[K];
+  uint32_t co[K], dc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float d = depth[k];
+    live[k] = live[k] && !(d == -INFINITY || d == 0.0f || d >= g.sdf_upper_bound);
+    const float sdf_dec = d - zc[k];
+    wgt[k] = fmaxf(10.0f * g.weight_sample * (1.0f - normalize_depth(d, c.min_depth, c.max_depth)), 1.0f);
+    const float tr = truncate_distance(g, d);
+    live[k] = live[k] && !(sdf_dec <= -tr);
+    sdf[k] = 0.0f;
+    so[k] = wo[k] = 0.0f;
+    co[k] = 0u;
+    dc[k] = 255u << 8;  // (0, 255, 0)
+    if (!live[k]) continue;
+    const i3 o = devectorize(lt + (uint32_t)k * L);
+    const i3 vp = i3{b.x * kBlockSide + o.x, b.y * kBlockSide + o.y, b.z * kBlockSide + o.z};
+    const double vsd = (double)g.voxel_size;
+    const double X = (double)vp.x * vsd, Y = (double)vp.y * vsd, Z = (double)vp.z * vsd;
+    const double z64 = fma(V.cTw64.m[8], X, fma(V.cTw64.m[9], Y, fma(V.cTw64.m[10], Z, V.cTw64.m[11])));
+    double sv = (double)d - z64;
+    const double trd = (double)tr;
+    sv = sv > trd ? trd : (sv < -trd ? -trd : sv);
+    sdf[k] = (float)sv;
+    const size_t idx = (size_t)ptr * kBlockVoxels + lt + (uint32_t)k * L;
+    so[k] = V.sdf[idx];
+    wo[k] = V.weight[idx];
+    co[k] = V.color[idx];
+    if (V.has_color) dc[k] = V.color_in[pix[k]];
+  }
+  uint32_t n = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (!live[k]) continue;
+    const size_t idx = (size_t)ptr * kBlockVoxels + lt + (uint32_t)k * L;
+    V.sdf[idx] = (so[k] * wo[k] + sdf[k] * wgt[k]) / (wo[k] + wgt[k]);
+    V.weight[idx] = wo[k] + wgt[k];
+    V.color[idx] = blend_color(co[k], dc[k]);
+    ++n;
+  }
+  return n;
 }
 
 // One workgroup of NT lanes fuses BPW worklist blocks at a time (NT / BPW
@@ -427,8 +459,7 @@ __global__ __launch_bounds__(NT) void integrate_tsdf_kernel(Src src_, const Scen
     const View& V = src.view(scene);
     const i3 b = unpack_key(V.block_keys[ptr]);
     nblk += lt == 0;
-#pragma unroll
-    for (uint32_t k = 0; k < kBlockVoxels / kLanes; ++k) nupd += integrate_voxel(V, b, ptr, lt + k * kLanes);
+    nupd += integrate_voxels<kBlockVoxels / kLanes, kLanes>(V, b, ptr, lt);
   }
   if (cur != ~0u) {
     const View& P = src.view(cur);

@@ -51,25 +51,44 @@ __device__ void stage_shell(const View& V, i3 b, int32_t self, Shell<PSDF>& S, i
     nb[t] = (dx | dy | dz) == 0 ? self : hash_find(V, i3{b.x + dx, b.y + dy, b.z + dz});
   }
   __syncthreads();
-  for (int s = (int)threadIdx.x; s < kShellN; s += (int)blockDim.x) {
+  // All of a lane's loads are issued before any is stored to LDS (one
+  // memory round trip for the shell), for a 512-lane workgroup.
+  constexpr int kPer = (kShellN + 511) / 512;
+  float sd[kPer], wt[kPer], va[kPer], vb[kPer];
+  bool ok[kPer];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int s = (int)threadIdx.x + 512 * r;
+    ok[r] = false;
+    sd[r] = wt[r] = va[r] = vb[r] = 0.0f;
+    if (s >= kShellN) continue;
     const int lx = s % kShell - 1, ly = (s / kShell) % kShell - 1, lz = s / (kShell * kShell) - 1;
     const int dx = lx < 0 ? -1 : (lx >= 8 ? 1 : 0);
     const int dy = ly < 0 ? -1 : (ly >= 8 ? 1 : 0);
     const int dz = lz < 0 ? -1 : (lz >= 8 ? 1 : 0);
     const int32_t p = nb[nb27(dx, dy, dz)];
-    if (p < 0) {
-      S.ok[s] = 0;
-      continue;
-    }
+    if (p < 0) continue;
     const size_t idx = (size_t)p * kBlockVoxels + (size_t)((lx - 8 * dx) + 8 * (ly - 8 * dy) +
                                                            64 * (lz - 8 * dz));
-    S.sdf[s] = V.sdf[idx];
-    S.w[s] = V.weight[idx];
+    ok[r] = true;
+    sd[r] = V.sdf[idx];
+    wt[r] = V.weight[idx];
     if (PSDF) {
-      S.a[s] = V.va[idx];
-      S.b[s] = V.vb[idx];
+      va[r] = V.va[idx];
+      vb[r] = V.vb[idx];
     }
-    S.ok[s] = 1;
+  }
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int s = (int)threadIdx.x + 512 * r;
+    if (s >= kShellN) continue;
+    S.ok[s] = ok[r] ? 1 : 0;
+    S.sdf[s] = sd[r];
+    S.w[s] = wt[r];
+    if (PSDF) {
+      S.a[s] = va[r];
+      S.b[s] = vb[r];
+    }
   }
   __syncthreads();
 }
