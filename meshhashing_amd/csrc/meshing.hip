@@ -44,7 +44,8 @@ __device__ __forceinline__ int nb27(int dx, int dy, int dz) {
 }
 
 template <bool PSDF>
-__device__ void stage_shell(const View& V, i3 b, int32_t self, Shell<PSDF>& S, int32_t* nb) {
+__device__ void stage_shell(const View& V, i3 b, int32_t self, Shell<PSDF>& S, int32_t* nb,
+                            uint8_t* meshed, uint32_t pass) {
   if (threadIdx.x < 27) {
     const int t = (int)threadIdx.x;
     const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
@@ -56,6 +57,10 @@ __device__ void stage_shell(const View& V, i3 b, int32_t self, Shell<PSDF>& S, i
   constexpr int kPer = (kShellN + 511) / 512;
   float sd[kPer], wt[kPer], va[kPer], vb[kPer];
   bool ok[kPer];
+  // whether each of the 27 blocks is meshed in this pass (loaded with the shell)
+  uint32_t mark = 0;
+  const int32_t mine = threadIdx.x < 27 ? nb[threadIdx.x] : kFreePtr;
+  if (mine >= 0) mark = V.block_mark[mine];
 #pragma unroll
   for (int r = 0; r < kPer; ++r) {
     const int s = (int)threadIdx.x + 512 * r;
@@ -78,6 +83,7 @@ __device__ void stage_shell(const View& V, i3 b, int32_t self, Shell<PSDF>& S, i
       vb[r] = V.vb[idx];
     }
   }
+  if (threadIdx.x < 27) meshed[threadIdx.x] = mine >= 0 && mark == pass;
 #pragma unroll
   for (int r = 0; r < kPer; ++r) {
     const int s = (int)threadIdx.x + 512 * r;
@@ -382,8 +388,7 @@ __global__ __launch_bounds__(512, 4) void mc_surfel_kernel(Src src_, const Scene
     const McConst& mc = mc_of(src, scene, mc1);
     cnt.blocks += threadIdx.x == 0;
     const i3 b = unpack_key(V.block_keys[ptr]);
-    stage_shell<PSDF>(V, b, ptr, S, nb);
-    if (threadIdx.x < 27) meshed[threadIdx.x] = nb[threadIdx.x] >= 0 && V.block_mark[nb[threadIdx.x]] == mc.pass;
+    stage_shell<PSDF>(V, b, ptr, S, nb, meshed, mc.pass);
     if (threadIdx.x == 0) qn = 0;
     __syncthreads();
     for (int s = (int)threadIdx.x; s < kCubeHalo * kCubeHalo * kCubeHalo; s += (int)blockDim.x) {
@@ -572,12 +577,24 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
     }
     if (threadIdx.x == 0) { counts[0] = 0; counts[1] = 0; }
     __syncthreads();
-    for (int s = (int)threadIdx.x; s < kCfg * kCfg * kCfg; s += (int)blockDim.x) {
-      const int x = s % kCfg - 1, y = (s / kCfg) % kCfg - 1, z = s / (kCfg * kCfg) - 1;
-      const int nx = x < 0, ny = y < 0, nz = z < 0;
-      const int32_t cp = nb[nx + 2 * ny + 4 * nz];
-      cfg[s] = cp < 0 ? 0 : (uint8_t)(V.cube[(size_t)cp * kBlockVoxels + (x + 8 * nx) + 8 * (y + 8 * ny) +
-                                             64 * (z + 8 * nz)] & 255);
+    {  // all of a lane's loads in flight together (256 lanes)
+      constexpr int kPer = (kCfg * kCfg * kCfg + 255) / 256;
+      uint16_t cw[kPer];
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) {
+        const int s = (int)threadIdx.x + 256 * r;
+        cw[r] = 0;
+        if (s >= kCfg * kCfg * kCfg) continue;
+        const int x = s % kCfg - 1, y = (s / kCfg) % kCfg - 1, z = s / (kCfg * kCfg) - 1;
+        const int nx = x < 0, ny = y < 0, nz = z < 0;
+        const int32_t cp = nb[nx + 2 * ny + 4 * nz];
+        if (cp >= 0) cw[r] = V.cube[(size_t)cp * kBlockVoxels + (x + 8 * nx) + 8 * (y + 8 * ny) + 64 * (z + 8 * nz)];
+      }
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) {
+        const int s = (int)threadIdx.x + 256 * r;
+        if (s < kCfg * kCfg * kCfg) cfg[s] = (uint8_t)(cw[r] & 255);
+      }
     }
     __syncthreads();
     int inner = 0, boundary = 0;
