@@ -257,43 +257,57 @@ __device__ __forceinline__ bool find_surface(const View& V, const RayConst& R, f
   while (t < t_lo) t += R.step;
   Octant oc{INT32_MIN, INT32_MIN, INT32_MIN, {kNotLooked, kNotLooked, kNotLooked, kNotLooked,
                                                 kNotLooked, kNotLooked, kNotLooked, kNotLooked}};
-  for (; t < t_max && t <= t_hi; t += R.step) {
-    const f3 p = mk3(o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z);
-    float s;
-    if (!march_sample(V, cache, oc, p, R.inv_vs, &s)) continue;
-    const float tg = t;
-    ++*nsamp;
-    if (prev_ok && prev_sdf > 0.0f && s < 0.0f) {
-      // BisectionIntersection: three regula-falsi steps.
-      float l = prev_t, r = tg, m = (l + r) * 0.5f;
-      float l_sdf = prev_sdf, r_sdf = s;
-      bool found = true;
-      for (int k = 0; k < 3; ++k) {
-        m = l + (l_sdf / (l_sdf - r_sdf)) * (r - l);
-        float m_sdf;
-        if (!ray_sdf(V, cache, mk3(o.x + m * dir.x, o.y + m * dir.y, o.z + m * dir.z), R.inv_vs,
-                     &m_sdf)) {
-          found = false;
-          break;
-        }
-        if (l_sdf * m_sdf > 0.0f) {
-          l = m;
-          l_sdf = m_sdf;
-        } else {
-          r = m;
-          r_sdf = m_sdf;
-        }
+  // The march stops at each crossing candidate and refines it outside the
+  // stepping loop, so a wave runs the refinement once for all its lanes
+  // rather than once per step at which some lane crosses.
+  while (true) {
+    float s = 0.0f;
+    bool cross = false;
+    for (; t < t_max && t <= t_hi; t += R.step) {
+      float sv;
+      if (!march_sample(V, cache, oc, mk3(o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z), R.inv_vs,
+                        &sv))
+        continue;
+      ++*nsamp;
+      if (prev_ok && prev_sdf > 0.0f && sv < 0.0f) {
+        s = sv;
+        cross = true;
+        break;
       }
-      if (found && fabsf(prev_sdf - s) < R.sample_sdf_threshold && fabsf(s) < R.sdf_threshold) {
-        *hit = Hit{prev_t, prev_sdf, tg, s, m};
-        return true;
+      prev_sdf = sv;
+      prev_t = t;
+      prev_ok = true;
+    }
+    if (!cross) return false;
+    // BisectionIntersection: three regula-falsi steps.
+    float l = prev_t, r = t, m = (l + r) * 0.5f;
+    float l_sdf = prev_sdf, r_sdf = s;
+    bool found = true;
+    for (int k = 0; k < 3; ++k) {
+      m = l + (l_sdf / (l_sdf - r_sdf)) * (r - l);
+      float m_sdf;
+      if (!ray_sdf(V, cache, mk3(o.x + m * dir.x, o.y + m * dir.y, o.z + m * dir.z), R.inv_vs, &m_sdf)) {
+        found = false;
+        break;
+      }
+      if (l_sdf * m_sdf > 0.0f) {
+        l = m;
+        l_sdf = m_sdf;
+      } else {
+        r = m;
+        r_sdf = m_sdf;
       }
     }
+    if (found && fabsf(prev_sdf - s) < R.sample_sdf_threshold && fabsf(s) < R.sdf_threshold) {
+      *hit = Hit{prev_t, prev_sdf, t, s, m};
+      return true;
+    }
+    // not accepted: the march goes on, with this sample as the previous one
     prev_sdf = s;
-    prev_t = tg;
+    prev_t = t;
     prev_ok = true;
+    t += R.step;
   }
-  return false;
 }
 
 // The ray parameters between which pixel (x, y) can meet an allocated block:
