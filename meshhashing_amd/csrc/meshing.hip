@@ -232,19 +232,22 @@ __device__ void vertex_payload(const Shell<PSDF>& S, const Geo& g, i3 base, f3 p
 template <bool PSDF>
 __device__ __forceinline__ bool cube_index(const Shell<PSDF>& S, int x, int y, int z,
                                            const McConst& mc, int* cube) {
+  // All eight corners are read and tested without early exits, so the lanes
+  // of a wave do not diverge on which corner fails first.
   int c = 0;
+  bool valid = true;
+#pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int s = sidx(x + corner_off(k, 0), y + corner_off(k, 1), z + corner_off(k, 2));
-    if (!S.ok[s]) return false;
     const float d = S.sdf[s];
-    if (fabsf(d) > mc.sdf_threshold) return false;
     const float a = PSDF ? S.a[s] : 0.0f, bb = PSDF ? S.b[s] : 0.0f;
     const float rho = a / (a + bb);
-    if (rho < mc.rho_min || S.w[s] < mc.weight_min) return false;
+    valid = valid && S.ok[s] && !(fabsf(d) > mc.sdf_threshold) && !(rho < mc.rho_min) &&
+            !(S.w[s] < mc.weight_min);
     if (d < 0.0f) c |= 1 << k;
   }
   *cube = c;
-  return true;
+  return valid;
 }
 
 // Cubes at local coordinates -1..8 (10^3), for the writer rule below.
