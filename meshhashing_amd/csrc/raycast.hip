@@ -176,16 +176,24 @@ __device__ __forceinline__ bool march_sample(const View& V, BlockCache& cache, O
   const float vs = V.geo.voxel_size;
   const float hv = 0.5f * vs;
   const float pcx = pos.x - hv, pcy = pos.y - hv, pcz = pos.z - hv;
+  // per axis, so that a wave runs the divisions only for the axes on which
+  // some lane is near a tie
   int cx0, cy0, cz0, cx1, cy1, cz1;
-  if (fast_corner(pcx, inv_vs, &cx0) && fast_corner(pcy, inv_vs, &cy0) && fast_corner(pcz, inv_vs, &cz0)) {
+  if (fast_corner(pcx, inv_vs, &cx0)) {
     cx1 = cx0 + 1;
-    cy1 = cy0 + 1;
-    cz1 = cz0 + 1;
   } else {
     cx0 = vox1(pcx + 0.0f, vs);
     cx1 = vox1(pcx + vs, vs);
+  }
+  if (fast_corner(pcy, inv_vs, &cy0)) {
+    cy1 = cy0 + 1;
+  } else {
     cy0 = vox1(pcy + 0.0f, vs);
     cy1 = vox1(pcy + vs, vs);
+  }
+  if (fast_corner(pcz, inv_vs, &cz0)) {
+    cz1 = cz0 + 1;
+  } else {
     cz0 = vox1(pcz + 0.0f, vs);
     cz1 = vox1(pcz + vs, vs);
   }
