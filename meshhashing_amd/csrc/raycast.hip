@@ -161,6 +161,10 @@ struct Octant {
   int bx, by, bz;
   int32_t p[8];
 };
+__device__ __forceinline__ Octant empty_octant() {
+  return Octant{INT32_MIN, INT32_MIN, INT32_MIN,
+                {kNotLooked, kNotLooked, kNotLooked, kNotLooked, kNotLooked, kNotLooked, kNotLooked, kNotLooked}};
+}
 
 // GetSpatialValue's SDF for a march sample (spatial_query.h:23-66).  The
 // corner voxels come from a multiply by 1/vs when the quotient is more than
@@ -257,14 +261,12 @@ struct Hit {
 // the march ends after t_hi.
 __device__ __forceinline__ bool find_surface(const View& V, const RayConst& R, f3 o, f3 dir, float t_min,
                                              float t_max, float t_lo, float t_hi, BlockCache& cache,
-                                             Hit* hit, uint32_t* nsamp) {
+                                             Octant& oc, Hit* hit, uint32_t* nsamp) {
   float prev_sdf = 0.0f, prev_t = 0.0f;
   bool prev_ok = false;
   if (!(t_lo < t_max) || t_hi < t_lo) return false;  // no block along this ray
   float t = t_min;
   while (t < t_lo) t += R.step;
-  Octant oc{INT32_MIN, INT32_MIN, INT32_MIN, {kNotLooked, kNotLooked, kNotLooked, kNotLooked,
-                                                kNotLooked, kNotLooked, kNotLooked, kNotLooked}};
   // The march stops at each crossing candidate and refines it outside the
   // stepping loop, so a wave runs the refinement once for all its lanes
   // rather than once per step at which some lane crosses.
@@ -294,7 +296,8 @@ __device__ __forceinline__ bool find_surface(const View& V, const RayConst& R, f
     for (int k = 0; k < 3; ++k) {
       m = l + (l_sdf / (l_sdf - r_sdf)) * (r - l);
       float m_sdf;
-      if (!ray_sdf(V, cache, mk3(o.x + m * dir.x, o.y + m * dir.y, o.z + m * dir.z), R.inv_vs, &m_sdf)) {
+      if (!march_sample(V, cache, oc, mk3(o.x + m * dir.x, o.y + m * dir.y, o.z + m * dir.z), R.inv_vs,
+                        &m_sdf)) {
         found = false;
         break;
       }
@@ -351,7 +354,8 @@ __device__ __forceinline__ uint32_t cast_pixel(const View& V, const RayConst& R,
   Hit hit;
   float t_lo, t_hi;
   ray_bounds(R, x, y, dir_c.z, t_min, t_max, &t_lo, &t_hi);
-  if (find_surface(V, R, o, dir, t_min, t_max, t_lo, t_hi, cache, &hit, &nsamp)) {
+  Octant oc = empty_octant();
+  if (find_surface(V, R, o, dir, t_min, t_max, t_lo, t_hi, cache, oc, &hit, &nsamp)) {
     const float m = hit.m;
     const float d = m * dir_c.z;
     out_d = d;
@@ -364,9 +368,9 @@ __device__ __forceinline__ uint32_t cast_pixel(const View& V, const RayConst& R,
 #pragma unroll
       for (int k = 0; k < 3 && valid; ++k) {
         const f3 dp = mk3(k == 0 ? h : 0.0f, k == 1 ? h : 0.0f, k == 2 ? h : 0.0f);
-        valid = ray_sdf(V, cache, iso - dp, R.inv_vs, &sn[k]);
+        valid = march_sample(V, cache, oc, iso - dp, R.inv_vs, &sn[k]);
         if (!valid) break;
-        valid = ray_sdf(V, cache, iso + dp, R.inv_vs, &sp[k]);
+        valid = march_sample(V, cache, oc, iso + dp, R.inv_vs, &sp[k]);
       }
       const float vs = V.geo.voxel_size;
       const f3 grad = mk3((sp[0] - sn[0]) / vs, (sp[1] - sn[1]) / vs, (sp[2] - sn[2]) / vs);
@@ -468,7 +472,8 @@ __device__ __forceinline__ bool depth_tangent(const View& V, const RayConst& R, 
   uint32_t ns = 0;
   float t_lo, t_hi;
   ray_bounds(R, x, y, dir_c.z, t_min, t_max, &t_lo, &t_hi);
-  if (!find_surface(V, R, o, dir, t_min, t_max, t_lo, t_hi, cache, &hit, &ns)) return false;
+  Octant oc = empty_octant();
+  if (!find_surface(V, R, o, dir, t_min, t_max, t_lo, t_hi, cache, oc, &hit, &ns)) return false;
   float l = hit.tp, r = hit.tk, ls, rs;
   float dl[kTan], dr[kTan], dls[kTan], drs[kTan], dm[kTan];
   f3 g;
