@@ -329,17 +329,31 @@ struct McCounts {
   }
 };
 
-// Exclusive scan of one value per lane over the workgroup (up to eight
-// waves); *total gets the sum.  Every lane must call it; it synchronises the
-// workgroup twice.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wave_sums, uint32_t* total) {
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  uint32_t x = v;
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off);
-    if (lane >= (uint32_t)off) x += y;
+// Exclusive scan over the wave of values v < 2^BITS, one bit plane at a time
+// (a ballot and a masked bit count per bit, no lane shuffles); *all gets the
+// wave's sum.
+template <int BITS>
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* all) {
+  uint32_t before = 0, sum = 0;
+#pragma unroll
+  for (int k = 0; k < BITS; ++k) {
+    const uint64_t m = __ballot((v >> k) & 1u);
+    before += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << k;
+    sum += (uint32_t)__popcll(m) << k;
   }
-  if (lane == 63) wave_sums[wave] = x;
+  *all = sum;
+  return before;
+}
+
+// Exclusive scan of one value v < 2^BITS per lane over the workgroup (up to
+// eight waves); *total gets the sum.  Every lane must call it; it
+// synchronises the workgroup twice.
+template <int BITS>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wave_sums, uint32_t* total) {
+  const uint32_t wave = threadIdx.x >> 6;
+  uint32_t wsum;
+  const uint32_t x = wave_excl_scan<BITS>(v, &wsum);
+  if ((threadIdx.x & 63u) == 0) wave_sums[wave] = wsum;
   __syncthreads();
   uint32_t before = 0, all = 0;
   const uint32_t nw = blockDim.x >> 6;
@@ -350,7 +364,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wave_s
   }
   __syncthreads();
   *total = all;
-  return before + x - v;
+  return before + x;
 }
 
 // Vertex slot of edge e of the cube at local voxel o: the slot of the edge's
@@ -370,6 +384,7 @@ __global__ __launch_bounds__(512, 4) void mc_surfel_kernel(Src src_, const Scene
   __shared__ int32_t nb[27];
   __shared__ uint8_t meshed[27];
   __shared__ uint8_t cv[kCubeHalo * kCubeHalo * kCubeHalo];
+  __shared__ uint8_t own_cube[kBlockVoxels];
   __shared__ int32_t qptr[kQueue];
   __shared__ uint16_t qkey[kQueue];
   __shared__ uint32_t qn;
@@ -391,24 +406,27 @@ __global__ __launch_bounds__(512, 4) void mc_surfel_kernel(Src src_, const Scene
     const McConst& mc = mc_of(src, scene, mc1);
     cnt.blocks += threadIdx.x == 0;
     const i3 b = unpack_key(V.block_keys[ptr]);
+    const uint32_t i = threadIdx.x;
+    const size_t idx = (size_t)ptr * kBlockVoxels + i;
+    const uint32_t prev = V.cube[idx] & 255u;  // in flight with the shell loads
     stage_shell<PSDF>(V, b, ptr, S, nb, meshed, mc.pass);
     if (threadIdx.x == 0) qn = 0;
     __syncthreads();
-    for (int s = (int)threadIdx.x; s < kCubeHalo * kCubeHalo * kCubeHalo; s += (int)blockDim.x) {
+    // every cube of the 10^3 halo, the block's own among them: validity for
+    // the writer rule, and the configuration of the block's own cubes
+    for (int s = (int)threadIdx.x; s < kCubeHalo * kCubeHalo * kCubeHalo; s += kBlockVoxels) {
       const int x = s % kCubeHalo - 1, y = (s / kCubeHalo) % kCubeHalo - 1, z = s / (kCubeHalo * kCubeHalo) - 1;
       const int dx = x < 0 ? -1 : (x >= 8 ? 1 : 0), dy = y < 0 ? -1 : (y >= 8 ? 1 : 0),
                 dz = z < 0 ? -1 : (z >= 8 ? 1 : 0);
       int c;
-      cv[s] = meshed[nb27(dx, dy, dz)] && cube_index(S, x, y, z, mc, &c);
+      const bool valid = cube_index(S, x, y, z, mc, &c);
+      cv[s] = meshed[nb27(dx, dy, dz)] && valid;
+      if ((dx | dy | dz) == 0) own_cube[x + 8 * y + 64 * z] = (uint8_t)(valid ? c : 0);
     }
     __syncthreads();
     const i3 base = i3{b.x * kBlockSide, b.y * kBlockSide, b.z * kBlockSide};
-    const uint32_t i = threadIdx.x;
     const i3 o = devectorize(i);
-    const size_t idx = (size_t)ptr * kBlockVoxels + i;
-    const uint32_t prev = V.cube[idx] & 255u;
-    int cube = 0;
-    if (!cube_index(S, o.x, o.y, o.z, mc, &cube)) cube = 0;
+    const int cube = own_cube[i];
     V.cube[idx] = (uint16_t)((uint32_t)cube | (prev << 8));
     // live triangles of a cube = its configuration's table row
     cnt.delta += kMc.ntri[cube] - kMc.ntri[prev];
@@ -420,13 +438,13 @@ __global__ __launch_bounds__(512, 4) void mc_surfel_kernel(Src src_, const Scene
     {
       const uint32_t mine = (uint32_t)__popc(em);
       uint32_t total;
-      uint32_t at = block_excl_scan(mine, scan_wave, &total);
+      uint32_t at = block_excl_scan<4>(mine, scan_wave, &total);  // mine <= 12
       for (unsigned m = em; m; m &= m - 1u) items[at++] = (uint16_t)(i | ((uint32_t)(__ffs(m) - 1) << 9));
       if (threadIdx.x == 0) nitems = total;
     }
     __syncthreads();
     const uint32_t n_items = nitems;
-    for (uint32_t c0 = 0; c0 < n_items; c0 += blockDim.x) {
+    for (uint32_t c0 = 0; c0 < n_items; c0 += kBlockVoxels) {
       // Vertex slot of each item: read it; claim it if free (CAS to the
       // lock value); pop fresh vertices for all claims of the workgroup
       // with one heap operation; publish them.
@@ -450,7 +468,7 @@ __global__ __launch_bounds__(512, 4) void mc_surfel_kernel(Src src_, const Scene
         }
       }
       uint32_t nclaims;
-      const uint32_t rank = block_excl_scan(claimed ? 1u : 0u, scan_wave, &nclaims);
+      const uint32_t rank = block_excl_scan<1>(claimed ? 1u : 0u, scan_wave, &nclaims);
       if (nclaims) {  // uniform
         if (threadIdx.x == 0) {
           const int32_t top = atomicSub(&V.ctr->vertex_top, (int32_t)nclaims);
@@ -502,7 +520,7 @@ __global__ __launch_bounds__(512, 4) void mc_surfel_kernel(Src src_, const Scene
     }
     __syncthreads();
     const uint32_t nq = qn < kQueue ? qn : kQueue;
-    for (uint32_t j = threadIdx.x; j < nq; j += blockDim.x) {
+    for (uint32_t j = threadIdx.x; j < nq; j += kBlockVoxels) {
       cnt.oob = write_vertex<PSDF>(S, V, mc, base, qkey[j], qptr[j]) || cnt.oob;
       ++cnt.writes;
     }
@@ -629,7 +647,7 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
     }
     // the block's freed slots go back to the heap with one counter update
     uint32_t nfree;
-    uint32_t rank = block_excl_scan((uint32_t)__popc(release), wave_sums, &nfree);
+    uint32_t rank = block_excl_scan<3>((uint32_t)__popc(release), wave_sums, &nfree);  // <= 6
     if (nfree) {  // uniform
       if (threadIdx.x == 0) push_top = atomicAdd(&V.ctr->vertex_top, (int32_t)nfree);
       __syncthreads();
@@ -739,7 +757,7 @@ __global__ __launch_bounds__(512) void compact_tris_kernel(View V, uint32_t n, u
       ++keep;
     }
     uint32_t total;
-    const uint32_t excl = block_excl_scan(keep, wave_sums, &total);
+    const uint32_t excl = block_excl_scan<3>(keep, wave_sums, &total);  // keep <= 5
     if (!EMIT) {
       if (threadIdx.x == 0) blk_tris[ci] = total;
     } else {
