@@ -583,7 +583,7 @@ int mh_marching_cubes(mh_volume* v, const mh_mc_params* p) {
   McConst c;
   const int rc = build_mc(v, p, &c);
   if (rc) return rc;
-  launch_marching_cubes(v->V, c, v->stream, v->grid_wg512());
+  launch_marching_cubes(v->V, c, v->stream, v->cus);
   HIP_TRY(hipGetLastError());
   return MH_OK;
 }
@@ -1141,7 +1141,7 @@ void batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   launch_batch_collect(B, 1, b->stream, b->cus * 4);
   if (!integrate) return;
   launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
-  if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->stream, b->cus * 4);
+  if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->stream, b->cus);
   if (flags & MH_RUN_RAYCAST) launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream);
 }
 
@@ -1397,7 +1397,7 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
     b->drop_graph();
     const BatchDev B = b->dev();
     launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
-    if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->stream, b->cus * 4);
+    if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->stream, b->cus);
     if (flags & MH_RUN_RAYCAST)
       launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream);
   } else if (flags & MH_RUN_GRAPH) {

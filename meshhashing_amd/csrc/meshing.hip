@@ -207,7 +207,9 @@ __device__ void vertex_payload(const Shell<PSDF>& S, const Geo& g, i3 base, f3 p
   const float h = 0.5f * g.voxel_size;
   bool valid = true;
   float sp[3] = {0, 0, 0}, sn[3] = {0, 0, 0};
-#pragma unroll
+  // not unrolled: the six stencils' live registers would cost mc_surfel_kernel
+  // a workgroup per CU
+#pragma unroll 1
   for (int k = 0; k < 3; ++k) {
     const f3 dp = mk3(k == 0 ? h : 0.0f, k == 1 ? h : 0.0f, k == 2 ? h : 0.0f);
     SVal q{};
@@ -263,7 +265,7 @@ __device__ __forceinline__ int cidx(int x, int y, int z) {
 constexpr uint32_t kQueue = 2048;
 template <bool PSDF>
 // Returns true when a stencil sample fell outside the staged shell.
-__device__ bool write_vertex(const Shell<PSDF>& S, const View& V, const McConst& mc, i3 base,
+__device__ __forceinline__ bool write_vertex(const Shell<PSDF>& S, const View& V, const McConst& mc, i3 base,
                              uint32_t key, int32_t vp) {
   bool oob = false;
   const Geo& g = V.geo;
@@ -875,27 +877,44 @@ void launch_compact_emit(const View& V, uint32_t n, const uint32_t* blk_off, con
   compact_vertices_kernel<<<grid, 256, 0, s>>>(V, used, remap, pos, nrm, col, V.max_vertices);
 }
 
-void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int grid) {
-  mc_mark_kernel<<<grid / 4, 256, 0, s>>>(One{V}, nullptr, p);
+// Workgroups of mc_surfel_kernel that are resident at once: cus x the
+// occupancy the runtime reports for it (queried once per instantiation).  The
+// persistent grid is exactly that, so its items are spread over one round of
+// workgroups; a larger grid would run its surplus as a second, partial round.
+template <bool PSDF, class Src>
+int surfel_grid(int cus) {
+  static const int per_cu = [] {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mc_surfel_kernel<PSDF, Src>, kBlockVoxels, 0) !=
+            hipSuccess ||
+        n < 1)
+      n = 1;
+    return n;
+  }();
+  return per_cu * cus;
+}
+
+void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int cus) {
+  mc_mark_kernel<<<cus, 256, 0, s>>>(One{V}, nullptr, p);
   if (V.va)
-    mc_surfel_kernel<true><<<grid, 512, 0, s>>>(One{V}, nullptr, p);
+    mc_surfel_kernel<true><<<surfel_grid<true, One>(cus), kBlockVoxels, 0, s>>>(One{V}, nullptr, p);
   else
-    mc_surfel_kernel<false><<<grid, 512, 0, s>>>(One{V}, nullptr, p);
-  mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(One{V}, nullptr);
+    mc_surfel_kernel<false><<<surfel_grid<false, One>(cus), kBlockVoxels, 0, s>>>(One{V}, nullptr, p);
+  mc_recycle_vertices_kernel<<<cus * 8, 256, 0, s>>>(One{V}, nullptr);
 }
 
 void launch_gc_recycle_vertices(const View& V, hipStream_t s, int grid) {
   mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(Listed{V, V.gc_list, &V.ctr->gc_count}, nullptr);
 }
 
-void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int grid) {
+void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int cus) {
   const McConst unused{};
-  mc_mark_kernel<<<grid / 4, 256, 0, s>>>(B.src(), B.frames, unused);
+  mc_mark_kernel<<<cus, 256, 0, s>>>(B.src(), B.frames, unused);
   if (B.psdf)
-    mc_surfel_kernel<true><<<grid, 512, 0, s>>>(B.src(), B.frames, unused);
+    mc_surfel_kernel<true><<<surfel_grid<true, Many>(cus), kBlockVoxels, 0, s>>>(B.src(), B.frames, unused);
   else
-    mc_surfel_kernel<false><<<grid, 512, 0, s>>>(B.src(), B.frames, unused);
-  mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(B.src(), B.frames);
+    mc_surfel_kernel<false><<<surfel_grid<false, Many>(cus), kBlockVoxels, 0, s>>>(B.src(), B.frames, unused);
+  mc_recycle_vertices_kernel<<<cus * 8, 256, 0, s>>>(B.src(), B.frames);
 }
 
 }  // namespace mh

@@ -165,8 +165,8 @@ void launch_gc_recycle(const View& V, hipStream_t s, int grid);
 // meshing.hip
 void launch_gc_release_triangles(const View& V, hipStream_t s, int grid);
 void launch_gc_recycle_vertices(const View& V, hipStream_t s, int grid);
-void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int grid);
-void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int grid);
+void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int cus);
+void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int cus);
 // mesh compaction (CompressMesh) and an exclusive scan of n uint32 (chunk_sums:
 // ceil(n / 1024) entries)
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* chunk_sums,
