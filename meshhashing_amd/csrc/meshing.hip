@@ -380,7 +380,7 @@ __device__ __forceinline__ int32_t* edge_slot(const View& V, const int32_t* nb, 
 }
 
 template <bool PSDF, class Src>
-__global__ __launch_bounds__(512, 4) void mc_surfel_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1) {
+__global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1) {
   const Src src = with_table(src_, tab);
   __shared__ Shell<PSDF> S;
   __shared__ int32_t nb[27];
