@@ -540,7 +540,7 @@ int mh_collect_garbage_blocks(mh_volume* v) {
 
 int mh_recycle_garbage_blocks(mh_volume* v, uint32_t* freed_host) {
   if (check_vol(v)) return MH_ERR_ARG;
-  launch_gc_recycle(v->V, v->stream, v->grid_wg256());
+  launch_gc_recycle(v->V, v->stream, v->cus);
   HIP_TRY(hipGetLastError());
   if (freed_host) {
     Counters c = read_counters(v);

@@ -890,11 +890,12 @@ void launch_gc_flag_garbage(const View& V, hipStream_t s, int grid) {
 }
 
 // RecycleGarbageBlockArray (recycle.cu:231-260) over the flagged worklist.
-void launch_gc_recycle(const View& V, hipStream_t s, int grid) {
+void launch_gc_recycle(const View& V, hipStream_t s, int cus) {
+  const int grid = cus * 8;
   gc_begin_kernel<<<1, 64, 0, s>>>(V);
   gc_list_kernel<<<grid, 256, 0, s>>>(V);
   launch_gc_release_triangles(V, s, grid);
-  launch_gc_recycle_vertices(V, s, grid);
+  launch_gc_recycle_vertices(V, s, cus);
   gc_count_valid_kernel<<<grid, 256, 0, s>>>(V);
   gc_free_entries_kernel<<<1, 64, 0, s>>>(V);
   gc_clear_kernel<<<grid, 256, 0, s>>>(V);

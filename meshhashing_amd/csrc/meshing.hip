@@ -394,6 +394,14 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
   __shared__ int32_t pop_top;
   __shared__ uint16_t items[12 * kBlockVoxels];
   __shared__ uint32_t nitems;
+  // per-configuration crossed edges and triangle counts, read per lane with
+  // divergent indices: from LDS rather than the constant tables
+  __shared__ uint16_t etab[256];
+  __shared__ uint8_t ntab[256];
+  if (threadIdx.x < 256) {
+    etab[threadIdx.x] = kMc.edges[threadIdx.x];
+    ntab[threadIdx.x] = (uint8_t)kMc.ntri[threadIdx.x];
+  }
   const uint32_t n = src.count();
   McCounts cnt;
   uint32_t cur = ~0u;
@@ -431,9 +439,10 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
     const int cube = own_cube[i];
     V.cube[idx] = (uint16_t)((uint32_t)cube | (prev << 8));
     // live triangles of a cube = its configuration's table row
-    cnt.delta += kMc.ntri[cube] - kMc.ntri[prev];
-    if ((uint32_t)cube != prev) cnt.assigned += (uint32_t)kMc.ntri[cube];
-    const unsigned em = (cube != 0 && cube != 255) ? kMc.edges[cube] : 0u;
+    const int nt = ntab[cube];
+    cnt.delta += nt - ntab[prev];
+    if ((uint32_t)cube != prev) cnt.assigned += (uint32_t)nt;
+    const unsigned em = etab[cube];  // 0 for configurations 0 and 255
     // The crossed edges of the block's cubes become a dense item list
     // (voxel | edge << 9), so the slot work below runs one item per lane
     // instead of twelve edge rounds on every cube.
@@ -561,16 +570,36 @@ __global__ void mc_mark_kernel(Src src_, const SceneFrame* __restrict__ tab, McC
 // before recycling, :187-205).  Cube configurations of the block and of its
 // -x/-y/-z neighbours (voxels -1..7) are staged in LDS.
 constexpr int kCfg = 9;  // local cube coordinates -1..7
+
+// The edge with owner offset (dx, dy, dz) and axis ax (a constant for
+// constant arguments).
+__host__ __device__ constexpr int edge_with(int dx, int dy, int dz, int ax) {
+  int found = -1;
+  for (int e = 0; e < 12; ++e)
+    if (edge_owner_off(e, 0) == dx && edge_owner_off(e, 1) == dy && edge_owner_off(e, 2) == dz &&
+        edge_axis(e) == ax)
+      found = e;
+  return found;
+}
+static_assert(edge_with(0, 0, 0, 0) >= 0 && edge_with(0, 1, 1, 0) >= 0 && edge_with(1, 0, 1, 1) >= 0 &&
+                  edge_with(1, 1, 0, 2) >= 0,
+              "every owner offset across an edge's axis names an edge");
 template <class Src>
 __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, const SceneFrame* __restrict__ tab) {
   const Src src = with_table(src_, tab);
   __shared__ int32_t nb[8];  // blocks at offsets (dx, dy, dz) in {-1, 0}^3
-  __shared__ uint8_t cfg[kCfg * kCfg * kCfg];
+  // edges used by the triangles of each staged cube's configuration
+  __shared__ uint16_t cfg[kCfg * kCfg * kCfg];
+  __shared__ uint16_t emask[256];
   __shared__ int counts[2];
   __shared__ uint32_t wave_sums[4];
   __shared__ int32_t push_top;
   const uint32_t n = src.count();
   static_assert(kBlockVoxels == 2 * 256, "two voxels per lane");
+  // A configuration's table row uses exactly the edges of kMc.edges (its
+  // crossed edges), so a vertex has references iff one of the four cubes
+  // around its edge has that edge's bit set (edge_refs > 0).
+  emask[threadIdx.x] = kMc.edges[threadIdx.x];
   for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
     uint32_t scene;
     const int32_t ptr = src.item(ci, &scene);
@@ -616,7 +645,7 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
 #pragma unroll
       for (int r = 0; r < kPer; ++r) {
         const int s = (int)threadIdx.x + 256 * r;
-        if (s < kCfg * kCfg * kCfg) cfg[s] = (uint8_t)(cw[r] & 255);
+        if (s < kCfg * kCfg * kCfg) cfg[s] = emask[cw[r] & 255];
       }
     }
     __syncthreads();
@@ -630,7 +659,7 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
 #pragma unroll
       for (int ax = 0; ax < 3; ++ax) {
         const int32_t vp = vps[r][ax];
-        int refs = 0;
+        bool used = false;
         if (vp >= 0) {
           if (is_inner) ++inner; else ++boundary;
           const int b1 = ax == 0 ? 1 : 0, b2 = ax == 2 ? 1 : 2;
@@ -639,12 +668,13 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
             int d[3] = {0, 0, 0};
             d[b1] = k & 1;
             d[b2] = k >> 1;
-            const int c = cfg[(o.x - d[0] + 1) + kCfg * (o.y - d[1] + 1) + kCfg * kCfg * (o.z - d[2] + 1)];
-            refs += edge_refs(kMc, c, d[0], d[1], d[2], ax);
+            const unsigned m = cfg[(o.x - d[0] + 1) + kCfg * (o.y - d[1] + 1) + kCfg * kCfg * (o.z - d[2] + 1)];
+            used |= ((m >> edge_with(d[0], d[1], d[2], ax)) & 1u) != 0;
           }
         }
-        // (the count itself is not stored: readers derive it the same way)
-        if (vp >= 0 && refs == 0) release |= 1u << (3 * r + ax);
+        // (the reference count itself is not stored: readers derive it with
+        // edge_refs)
+        if (vp >= 0 && !used) release |= 1u << (3 * r + ax);
       }
     }
     // the block's freed slots go back to the heap with one counter update
@@ -877,18 +907,15 @@ void launch_compact_emit(const View& V, uint32_t n, const uint32_t* blk_off, con
   compact_vertices_kernel<<<grid, 256, 0, s>>>(V, used, remap, pos, nrm, col, V.max_vertices);
 }
 
-// Workgroups of mc_surfel_kernel that are resident at once: cus x the
-// occupancy the runtime reports for it (queried once per instantiation).  The
-// persistent grid is exactly that, so its items are spread over one round of
-// workgroups; a larger grid would run its surplus as a second, partial round.
-template <bool PSDF, class Src>
-int surfel_grid(int cus) {
+// Workgroups of persistent kernel K (THREADS lanes) resident at once: cus x
+// the occupancy the runtime reports for it (queried once per kernel).  A
+// persistent grid of exactly that spreads its items over one round of
+// workgroups; a larger one would run its surplus as a second, partial round.
+template <auto K, int THREADS>
+int resident_grid(int cus) {
   static const int per_cu = [] {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mc_surfel_kernel<PSDF, Src>, kBlockVoxels, 0) !=
-            hipSuccess ||
-        n < 1)
-      n = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, K, THREADS, 0) != hipSuccess || n < 1) n = 1;
     return n;
   }();
   return per_cu * cus;
@@ -897,24 +924,31 @@ int surfel_grid(int cus) {
 void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int cus) {
   mc_mark_kernel<<<cus, 256, 0, s>>>(One{V}, nullptr, p);
   if (V.va)
-    mc_surfel_kernel<true><<<surfel_grid<true, One>(cus), kBlockVoxels, 0, s>>>(One{V}, nullptr, p);
+    mc_surfel_kernel<true><<<resident_grid<mc_surfel_kernel<true, One>, kBlockVoxels>(cus), kBlockVoxels, 0, s>>>(
+        One{V}, nullptr, p);
   else
-    mc_surfel_kernel<false><<<surfel_grid<false, One>(cus), kBlockVoxels, 0, s>>>(One{V}, nullptr, p);
-  mc_recycle_vertices_kernel<<<cus * 8, 256, 0, s>>>(One{V}, nullptr);
+    mc_surfel_kernel<false><<<resident_grid<mc_surfel_kernel<false, One>, kBlockVoxels>(cus), kBlockVoxels, 0,
+                              s>>>(One{V}, nullptr, p);
+  mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<One>, 256>(cus), 256, 0, s>>>(One{V},
+                                                                                                   nullptr);
 }
 
-void launch_gc_recycle_vertices(const View& V, hipStream_t s, int grid) {
-  mc_recycle_vertices_kernel<<<grid * 2, 256, 0, s>>>(Listed{V, V.gc_list, &V.ctr->gc_count}, nullptr);
+void launch_gc_recycle_vertices(const View& V, hipStream_t s, int cus) {
+  mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<Listed>, 256>(cus), 256, 0, s>>>(
+      Listed{V, V.gc_list, &V.ctr->gc_count}, nullptr);
 }
 
 void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int cus) {
   const McConst unused{};
   mc_mark_kernel<<<cus, 256, 0, s>>>(B.src(), B.frames, unused);
   if (B.psdf)
-    mc_surfel_kernel<true><<<surfel_grid<true, Many>(cus), kBlockVoxels, 0, s>>>(B.src(), B.frames, unused);
+    mc_surfel_kernel<true><<<resident_grid<mc_surfel_kernel<true, Many>, kBlockVoxels>(cus), kBlockVoxels, 0,
+                             s>>>(B.src(), B.frames, unused);
   else
-    mc_surfel_kernel<false><<<surfel_grid<false, Many>(cus), kBlockVoxels, 0, s>>>(B.src(), B.frames, unused);
-  mc_recycle_vertices_kernel<<<cus * 8, 256, 0, s>>>(B.src(), B.frames);
+    mc_surfel_kernel<false><<<resident_grid<mc_surfel_kernel<false, Many>, kBlockVoxels>(cus), kBlockVoxels, 0,
+                              s>>>(B.src(), B.frames, unused);
+  mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<Many>, 256>(cus), 256, 0, s>>>(
+      B.src(), B.frames);
 }
 
 }  // namespace mh

@@ -160,11 +160,11 @@ constexpr int32_t kBlockLife = 3;  // BLOCK_LIFE, core/block.h:13
 void launch_gc_low_surfel(const View& V, hipStream_t s, int grid);
 void launch_gc_starve(const View& V, hipStream_t s, int grid);
 void launch_gc_flag_garbage(const View& V, hipStream_t s, int grid);
-void launch_gc_recycle(const View& V, hipStream_t s, int grid);
+void launch_gc_recycle(const View& V, hipStream_t s, int cus);
 
 // meshing.hip
 void launch_gc_release_triangles(const View& V, hipStream_t s, int grid);
-void launch_gc_recycle_vertices(const View& V, hipStream_t s, int grid);
+void launch_gc_recycle_vertices(const View& V, hipStream_t s, int cus);
 void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int cus);
 void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int cus);
 // mesh compaction (CompressMesh) and an exclusive scan of n uint32 (chunk_sums:
