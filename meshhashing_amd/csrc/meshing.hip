@@ -99,11 +99,6 @@ __device__ void stage_shell(const View& V, i3 b, int32_t self, Shell<PSDF>& S, i
   __syncthreads();
 }
 
-// WorldToVoxeli along one axis.
-__device__ __forceinline__ int vox1(float w, float vs) {
-  const float p = w / vs;
-  return f2i(p + (float)sgn(p) * 0.5f);
-}
 
 struct SVal {
   float sdf, a, b, radius;

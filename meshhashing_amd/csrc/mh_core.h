@@ -102,6 +102,39 @@ MH_HD i3 world_to_voxel(const Geo& g, f3 w) {
   return i3{f2i(p.x + (float)sgn(p.x) * 0.5f), f2i(p.y + (float)sgn(p.y) * 0.5f),
             f2i(p.z + (float)sgn(p.z) * 0.5f)};
 }
+// WorldToVoxeli along one axis.
+__device__ __forceinline__ int vox1(float w, float vs) {
+  const float p = w / vs;
+  return f2i(p + (float)sgn(p) * 0.5f);
+}
+
+// vox1(pc, vs) from a multiply by 1/vs instead of the IEEE division;
+// accepted only when the quotient is more than 1e-3 voxel away from every
+// rounding tie and from 0 (and |q| < 2048, where the multiply is within 2 ulp
+// of the quotient), so it equals the exact result.
+__device__ __forceinline__ bool fast_corner(float pc, float inv_vs, int* c) {
+  const float q = pc * inv_vs;
+  const float aq = fabsf(q);
+  if (!(aq < 2048.0f) || aq < 1e-3f) return false;
+  const float f = aq - floorf(aq);
+  if (fabsf(f - 0.5f) < 1e-3f) return false;
+  *c = f2i(q + (float)sgn(q) * 0.5f);
+  return true;
+}
+
+// The two corner voxels vox1(pc, vs) and vox1(pc + vs, vs) of a trilinear
+// sample along one axis (spatial_query.h:23-66 rounds both).  Clear of a
+// tie, the upper one is the lower + 1: pc + vs is within 1.3e-4 voxel of
+// exact, so its quotient stays clear of the next tie.
+__device__ __forceinline__ void corner_pair(float pc, float vs, float inv_vs, int* c0, int* c1) {
+  if (fast_corner(pc, inv_vs, c0)) {
+    *c1 = *c0 + 1;
+  } else {
+    *c0 = vox1(pc + 0.0f, vs);
+    *c1 = vox1(pc + vs, vs);
+  }
+}
+
 MH_HD int voxel_to_block1(int v) { return (v - (v < 0) * (kBlockSide - 1)) / kBlockSide; }
 MH_HD i3 voxel_to_block(i3 v) {
   return i3{voxel_to_block1(v.x), voxel_to_block1(v.y), voxel_to_block1(v.z)};

@@ -83,24 +83,6 @@ __device__ __forceinline__ int32_t cached_find(const View& V, BlockCache& c, i3 
   return p;
 }
 
-__device__ __forceinline__ int vox1(float w, float vs) {
-  const float p = w / vs;
-  return f2i(p + (float)sgn(p) * 0.5f);
-}
-
-// Lowest-corner voxel of a sample along one axis from a multiply by 1/vs
-// instead of the IEEE division; accepted only when the quotient is more than
-// 1e-3 voxel away from every rounding tie (and |q| < 2048, where the
-// multiply is within 2 ulp of the quotient), so it equals the exact result.
-__device__ __forceinline__ bool fast_corner(float pc, float inv_vs, int* c) {
-  const float q = pc * inv_vs;
-  const float aq = fabsf(q);
-  if (!(aq < 2048.0f) || aq < 1e-3f) return false;
-  const float f = aq - floorf(aq);
-  if (fabsf(f - 0.5f) < 1e-3f) return false;
-  *c = f2i(q + (float)sgn(q) * 0.5f);
-  return true;
-}
 
 // GetSpatialValue restricted to the SDF (the only field the march uses),
 // without the empty-space test.
@@ -183,24 +165,9 @@ __device__ __forceinline__ bool march_sample(const View& V, BlockCache& cache, O
   // per axis, so that a wave runs the divisions only for the axes on which
   // some lane is near a tie
   int cx0, cy0, cz0, cx1, cy1, cz1;
-  if (fast_corner(pcx, inv_vs, &cx0)) {
-    cx1 = cx0 + 1;
-  } else {
-    cx0 = vox1(pcx + 0.0f, vs);
-    cx1 = vox1(pcx + vs, vs);
-  }
-  if (fast_corner(pcy, inv_vs, &cy0)) {
-    cy1 = cy0 + 1;
-  } else {
-    cy0 = vox1(pcy + 0.0f, vs);
-    cy1 = vox1(pcy + vs, vs);
-  }
-  if (fast_corner(pcz, inv_vs, &cz0)) {
-    cz1 = cz0 + 1;
-  } else {
-    cz0 = vox1(pcz + 0.0f, vs);
-    cz1 = vox1(pcz + vs, vs);
-  }
+  corner_pair(pcx, vs, inv_vs, &cx0, &cx1);
+  corner_pair(pcy, vs, inv_vs, &cy0, &cy1);
+  corner_pair(pcz, vs, inv_vs, &cz0, &cz1);
   const int bx = voxel_to_block1(cx0), by = voxel_to_block1(cy0), bz = voxel_to_block1(cz0);
   if (bx != oc.bx || by != oc.by || bz != oc.bz) {
     oc.bx = bx;
