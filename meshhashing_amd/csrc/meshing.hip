@@ -43,13 +43,38 @@ __device__ __forceinline__ int nb27(int dx, int dy, int dz) {
   return (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
 }
 
+// Lanes kLookupLane + t (t = 0..26) of the last wave look up neighbour t (x
+// fastest, the block itself at 13) of worklist item ci and that block's
+// meshing mark.  mc_surfel_kernel runs this for its next item while the
+// current item's payloads are written (by the first waves: the queue fills
+// from lane 0), so the dependent worklist, key and bucket reads stay off
+// the critical path.
+constexpr uint32_t kLookupLane = kBlockVoxels - 64;
+template <class Src>
+__device__ __forceinline__ void neighbour_lookup(const Src& src, uint32_t ci, int32_t* p,
+                                                 uint32_t* mark) {
+  *p = kFreePtr;
+  *mark = 0;
+  const uint32_t t = threadIdx.x - kLookupLane;
+  if (t < 27) {
+    uint32_t scene;
+    const int32_t self = src.lane_item(ci, &scene);
+    const View& V = src.view(scene);
+    const i3 b = unpack_key(V.block_keys[self]);
+    const int dx = (int)(t % 3) - 1, dy = (int)((t / 3) % 3) - 1, dz = (int)(t / 9) - 1;
+    *p = (dx | dy | dz) == 0 ? self : hash_find(V, i3{b.x + dx, b.y + dy, b.z + dz});
+    if (*p >= 0) *mark = V.block_mark[*p];
+  }
+}
+
+// nbp, markp: the lane's neighbour_lookup results for this block.
 template <bool PSDF>
-__device__ void stage_shell(const View& V, i3 b, int32_t self, Shell<PSDF>& S, int32_t* nb,
+__device__ void stage_shell(const View& V, int32_t nbp, uint32_t markp, Shell<PSDF>& S, int32_t* nb,
                             uint8_t* meshed, uint32_t pass) {
-  if (threadIdx.x < 27) {
-    const int t = (int)threadIdx.x;
-    const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-    nb[t] = (dx | dy | dz) == 0 ? self : hash_find(V, i3{b.x + dx, b.y + dy, b.z + dz});
+  const uint32_t t = threadIdx.x - kLookupLane;
+  if (t < 27) {
+    nb[t] = nbp;
+    meshed[t] = nbp >= 0 && markp == pass;
   }
   __syncthreads();
   // All of a lane's loads are issued before any is stored to LDS (one
@@ -57,10 +82,6 @@ __device__ void stage_shell(const View& V, i3 b, int32_t self, Shell<PSDF>& S, i
   constexpr int kPer = (kShellN + 511) / 512;
   float sd[kPer], wt[kPer], va[kPer], vb[kPer];
   bool ok[kPer];
-  // whether each of the 27 blocks is meshed in this pass (loaded with the shell)
-  uint32_t mark = 0;
-  const int32_t mine = threadIdx.x < 27 ? nb[threadIdx.x] : kFreePtr;
-  if (mine >= 0) mark = V.block_mark[mine];
 #pragma unroll
   for (int r = 0; r < kPer; ++r) {
     const int s = (int)threadIdx.x + 512 * r;
@@ -83,7 +104,6 @@ __device__ void stage_shell(const View& V, i3 b, int32_t self, Shell<PSDF>& S, i
       vb[r] = V.vb[idx];
     }
   }
-  if (threadIdx.x < 27) meshed[threadIdx.x] = mine >= 0 && mark == pass;
 #pragma unroll
   for (int r = 0; r < kPer; ++r) {
     const int s = (int)threadIdx.x + 512 * r;
@@ -400,6 +420,9 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
   const uint32_t n = src.count();
   McCounts cnt;
   uint32_t cur = ~0u;
+  int32_t nbp;
+  uint32_t markp;
+  if (blockIdx.x < n) neighbour_lookup(src, blockIdx.x, &nbp, &markp);
   for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
     uint32_t scene;
     const int32_t ptr = src.item(ci, &scene);
@@ -414,7 +437,7 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
     const uint32_t i = threadIdx.x;
     const size_t idx = (size_t)ptr * kBlockVoxels + i;
     const uint32_t prev = V.cube[idx] & 255u;  // in flight with the shell loads
-    stage_shell<PSDF>(V, b, ptr, S, nb, meshed, mc.pass);
+    stage_shell<PSDF>(V, nbp, markp, S, nb, meshed, mc.pass);
     if (threadIdx.x == 0) qn = 0;
     __syncthreads();
     // every cube of the 10^3 halo, the block's own among them: validity for
@@ -525,6 +548,7 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
       }
     }
     __syncthreads();
+    if (ci + gridDim.x < n) neighbour_lookup(src, ci + gridDim.x, &nbp, &markp);
     const uint32_t nq = qn < kQueue ? qn : kQueue;
     for (uint32_t j = threadIdx.x; j < nq; j += kBlockVoxels) {
       cnt.oob = write_vertex<PSDF>(S, V, mc, base, qkey[j], qptr[j]) || cnt.oob;
