@@ -648,8 +648,10 @@ __device__ __forceinline__ const RayConst& ray_of(const Many& m, uint32_t s, con
 }
 
 // Scene = blockIdx.z; a batch's outputs are scene-major (W*H per scene).
+// At most 96 registers: five waves per SIMD hold a 640x480 frame's 4800
+// tiles at once on 256 CUs (four would leave a partial second round).
 template <class Src>
-__global__ __launch_bounds__(64) void raycast_kernel(Src src_, const SceneFrame* __restrict__ tab, RayConst R1, float* __restrict__ depth_out,
+__global__ __launch_bounds__(64, 5) void raycast_kernel(Src src_, const SceneFrame* __restrict__ tab, RayConst R1, float* __restrict__ depth_out,
                                                      float* __restrict__ vertex_out,
                                                      float* __restrict__ normal_out) {
   const Src src = with_table(src_, tab);
