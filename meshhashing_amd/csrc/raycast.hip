@@ -156,7 +156,9 @@ __device__ __forceinline__ Octant empty_octant() {
 // the ratios come from the division, as the reference computes them.  A
 // missing lowest-corner block ends the sample after one compare while the
 // octant is cached, which is how empty space inside the tile range costs
-// little.
+// little.  Octant misses go to the wave's LDS table directly (the cache's
+// two-entry register cache is left out of the march: its registers cost
+// more than the lookups it saves).
 __device__ __forceinline__ bool march_sample(const View& V, BlockCache& cache, Octant& oc, f3 pos,
                                              float inv_vs, float* out) {
   const float vs = V.geo.voxel_size;
@@ -173,7 +175,7 @@ __device__ __forceinline__ bool march_sample(const View& V, BlockCache& cache, O
     oc.bx = bx;
     oc.by = by;
     oc.bz = bz;
-    oc.p[0] = cached_find(V, cache, i3{bx, by, bz});
+    oc.p[0] = table_find(V, cache.t, i3{bx, by, bz});
 #pragma unroll
     for (int i = 1; i < 8; ++i) oc.p[i] = kNotLooked;
   }
@@ -183,7 +185,7 @@ __device__ __forceinline__ bool march_sample(const View& V, BlockCache& cache, O
   for (int i = 1; i < 8; ++i) {
     const bool need = (!(i & 4) || xx) && (!(i & 2) || yy) && (!(i & 1) || zz);
     if (!need) continue;
-    if (oc.p[i] == kNotLooked) oc.p[i] = cached_find(V, cache, i3{bx + ((i >> 2) & 1), by + ((i >> 1) & 1), bz + (i & 1)});
+    if (oc.p[i] == kNotLooked) oc.p[i] = table_find(V, cache.t, i3{bx + ((i >> 2) & 1), by + ((i >> 1) & 1), bz + (i & 1)});
     if (oc.p[i] < 0) return false;
   }
   // block of each corner (bit 2: x, bit 1: y, bit 0: z set), by selects
@@ -209,6 +211,26 @@ __device__ __forceinline__ bool march_sample(const View& V, BlockCache& cache, O
   }
   *out = sdf;
   return true;
+}
+
+// A ray parameter below which every sample position o + t dir lies more
+// than 0.01 voxel inside the world box [8 b vs, 8 (b + 1) vs)^3 of block b.
+// A sample's lowest corner voxel is vox1(pos - vs/2) per axis, so its
+// lowest-corner block is b exactly on that box; the margin is 20 times the
+// FP32 error of the reference's quotient (pos - vs/2) / vs at 16 m and
+// covers the error of the position.  The box is shrunk by 0.02 voxel, so
+// the approximate divisions (a few ulp of t, well under 1e-5 m) only
+// lower the bound.
+__device__ __forceinline__ float block_exit_t(f3 o, f3 dir, int bx, int by, int bz, float vs) {
+  const float bs = (float)kBlockSide * vs, eps = 0.02f * vs;
+  const float ex = dir.x > 0.0f ? (float)(bx + 1) * bs - eps : (float)bx * bs + eps;
+  const float ey = dir.y > 0.0f ? (float)(by + 1) * bs - eps : (float)by * bs + eps;
+  const float ez = dir.z > 0.0f ? (float)(bz + 1) * bs - eps : (float)bz * bs + eps;
+  float te = INFINITY;
+  if (dir.x != 0.0f) te = fminf(te, __fdividef(ex - o.x, dir.x));
+  if (dir.y != 0.0f) te = fminf(te, __fdividef(ey - o.y, dir.y));
+  if (dir.z != 0.0f) te = fminf(te, __fdividef(ez - o.z, dir.z));
+  return te;
 }
 
 // The accepted crossing of one ray
@@ -243,8 +265,16 @@ __device__ __forceinline__ bool find_surface(const View& V, const RayConst& R, f
     for (; t < t_max && t <= t_hi; t += R.step) {
       float sv;
       if (!march_sample(V, cache, oc, mk3(o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z), R.inv_vs,
-                        &sv))
+                        &sv)) {
+        // Lowest-corner block absent: the following samples inside its box
+        // are invalid too (an invalid sample changes nothing); step over
+        // them with the same FP32 additions, so the t sequence is kept.
+        if (oc.p[0] < 0) {
+          const float t_out = block_exit_t(o, dir, oc.bx, oc.by, oc.bz, V.geo.voxel_size);
+          for (float tn = t + R.step; tn < t_out && tn > t; tn = t + R.step) t = tn;
+        }
         continue;
+      }
       ++*nsamp;
       if (prev_ok && prev_sdf > 0.0f && sv < 0.0f) {
         s = sv;
