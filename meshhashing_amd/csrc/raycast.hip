@@ -233,6 +233,40 @@ __device__ __forceinline__ float block_exit_t(f3 o, f3 dir, int bx, int by, int 
   return te;
 }
 
+// The first element >= target of the FP32 sequence t, t + step, ...
+// (t += step, t > 0, step > 0), without taking every step.  Inside a binade
+// [lo, 2 lo) every sum rounds to the grid u = lo 2^-23, so once a step has
+// been taken inside the binade the increment is constant (an exact tie
+// rounds to an even multiple of u, after which the tie always resolves the
+// same way): with t1, t2, t3 in one binade, d = t3 - t2 repeats for as long
+// as each exact sum t + step stays 2u below the binade's top.  The jump is
+// computed in FP64, where t3 + k d is exact.
+__device__ __forceinline__ float advance_steps(float t, float step, float target) {
+  while (t < target) {
+    const float t1 = t + step;
+    if (!(t1 < target)) return t1;
+    const float t2 = t1 + step;
+    if (!(t2 < target)) return t2;
+    const float t3 = t2 + step;
+    if (!(t3 < target)) return t3;
+    const float lo = __uint_as_float(__float_as_uint(t3) & 0x7F800000u);
+    if (t1 < lo) {
+      t = t3;
+      continue;
+    }
+    const double top = 2.0 * (double)lo, u = (double)lo * 0x1p-23;
+    const double d = (double)t3 - (double)t2;
+    const double room = (top - 2.0 * u - (double)step - (double)t3) / d;
+    if (!(room >= 1.0)) {
+      t = t3;
+      continue;
+    }
+    const double k = fmin(floor(room) + 1.0, ceil(((double)target - (double)t3) / d));
+    t = (float)((double)t3 + k * d);
+  }
+  return t;
+}
+
 // The accepted crossing of one ray
 // and (t_k, s_k), and the last regula-falsi point m.
 struct Hit {
@@ -254,8 +288,7 @@ __device__ __forceinline__ bool find_surface(const View& V, const RayConst& R, f
   float prev_sdf = 0.0f, prev_t = 0.0f;
   bool prev_ok = false;
   if (!(t_lo < t_max) || t_hi < t_lo) return false;  // no block along this ray
-  float t = t_min;
-  while (t < t_lo) t += R.step;
+  float t = advance_steps(t_min, R.step, t_lo);
   // The march stops at each crossing candidate and refines it outside the
   // stepping loop, so a wave runs the refinement once for all its lanes
   // rather than once per step at which some lane crosses.
