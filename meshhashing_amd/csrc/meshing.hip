@@ -124,21 +124,28 @@ struct SVal {
   float sdf, a, b, radius;
 };
 
-// GetSpatialValue (spatial_query.h:23-66) against the staged shell.  base is
-// the block's voxel origin.  Corner voxels come from rounding the corner's
-// world position, exactly as the reference, so ties and negative
-// coordinates pick the same voxels.
+// One axis of a GetSpatialValue sample (spatial_query.h:23-66) at world
+// coordinate p: the two corner voxels (rounded from p - vs/2 and p + vs/2,
+// exactly as the reference, so ties and negative coordinates pick the same
+// voxels), local to the block origin base, and the ratio frac(p / vs).  A
+// sample's axes are independent, so the gradient stencils p +- vs/2 e_k
+// share two of their three axes with the centre sample.
+struct SAxis {
+  int c0, c1;
+  float r;
+};
+__device__ __forceinline__ SAxis shell_axis(float p, float vs, int base) {
+  const float pc = p - 0.5f * vs;
+  const float q = p / vs;
+  return SAxis{vox1(pc + 0.0f, vs) - base, vox1(pc + vs, vs) - base, q - floorf(q)};
+}
+
+// GetSpatialValue against the staged shell, from the sample's three axes.
 template <bool PSDF, bool FULL = true>
-__device__ bool shell_value(const Shell<PSDF>& S, const Geo& g, i3 base, f3 pos, SVal* out,
-                            bool* oob) {
-  const float vs = g.voxel_size;
-  const float hv = 0.5f * vs;
-  const float pcx = pos.x - hv, pcy = pos.y - hv, pcz = pos.z - hv;
-  const float qx = pos.x / vs, qy = pos.y / vs, qz = pos.z / vs;
-  const float rx = qx - floorf(qx), ry = qy - floorf(qy), rz = qz - floorf(qz);
-  const int cx[2] = {vox1(pcx + 0.0f, vs) - base.x, vox1(pcx + vs, vs) - base.x};
-  const int cy[2] = {vox1(pcy + 0.0f, vs) - base.y, vox1(pcy + vs, vs) - base.y};
-  const int cz[2] = {vox1(pcz + 0.0f, vs) - base.z, vox1(pcz + vs, vs) - base.z};
+__device__ __forceinline__ bool shell_value(const Shell<PSDF>& S, const SAxis& X, const SAxis& Y,
+                                            const SAxis& Z, SVal* out, bool* oob) {
+  const int cx[2] = {X.c0, X.c1}, cy[2] = {Y.c0, Y.c1}, cz[2] = {Z.c0, Z.c1};
+  const float rx = X.r, ry = Y.r, rz = Z.r;
   // Straight-line evaluation of all 8 corners (no early exit: a missing
   // corner makes the result invalid, and the sums are then discarded).
   const bool in = cx[0] >= -1 && cx[1] <= 9 && cy[0] >= -1 && cy[1] <= 9 && cz[0] >= -1 && cz[1] <= 9 &&
@@ -200,7 +207,9 @@ template <bool PSDF>
 __device__ void vertex_payload(const Shell<PSDF>& S, const Geo& g, i3 base, f3 p, Vertex* vt,
                                bool* oob) {
   SVal sv{};
-  const bool ok = shell_value(S, g, base, p, &sv, oob);
+  const float vs = g.voxel_size;
+  const SAxis X = shell_axis(p.x, vs, base.x), Y = shell_axis(p.y, vs, base.y), Z = shell_axis(p.z, vs, base.z);
+  const bool ok = shell_value(S, X, Y, Z, &sv, oob);
   vt->px = p.x;
   vt->py = p.y;
   vt->pz = p.z;
@@ -219,6 +228,8 @@ __device__ void vertex_payload(const Shell<PSDF>& S, const Geo& g, i3 base, f3 p
   // +-vs/2, with the reference's short-circuit on the first invalid sample.
   // (all six samples are taken; any invalid one zeroes the normal, which is
   // what the reference's short-circuit produces)
+  // (p -+ h e_k differs from p on axis k only: the other two axes are the
+  // centre sample's)
   const float h = 0.5f * g.voxel_size;
   bool valid = true;
   float sp[3] = {0, 0, 0}, sn[3] = {0, 0, 0};
@@ -226,14 +237,16 @@ __device__ void vertex_payload(const Shell<PSDF>& S, const Geo& g, i3 base, f3 p
   // a workgroup per CU
 #pragma unroll 1
   for (int k = 0; k < 3; ++k) {
-    const f3 dp = mk3(k == 0 ? h : 0.0f, k == 1 ? h : 0.0f, k == 2 ? h : 0.0f);
+    const float pk = k == 0 ? p.x : (k == 1 ? p.y : p.z);
+    const int bk = k == 0 ? base.x : (k == 1 ? base.y : base.z);
     SVal q{};
-    valid = shell_value<PSDF, false>(S, g, base, p - dp, &q, oob) && valid;
+    const SAxis An = shell_axis(pk - h, vs, bk);
+    valid = shell_value<PSDF, false>(S, k == 0 ? An : X, k == 1 ? An : Y, k == 2 ? An : Z, &q, oob) && valid;
     sn[k] = q.sdf;
-    valid = shell_value<PSDF, false>(S, g, base, p + dp, &q, oob) && valid;
+    const SAxis Ap = shell_axis(pk + h, vs, bk);
+    valid = shell_value<PSDF, false>(S, k == 0 ? Ap : X, k == 1 ? Ap : Y, k == 2 ? Ap : Z, &q, oob) && valid;
     sp[k] = q.sdf;
   }
-  const float vs = g.voxel_size;
   const f3 grad = mk3((sp[0] - sn[0]) / vs, (sp[1] - sn[1]) / vs, (sp[2] - sn[2]) / vs);
   const float l = length(grad);
   const f3 n = (l > 0.0f && valid) ? grad / l : mk3(0.0f, 0.0f, 0.0f);
