@@ -232,7 +232,7 @@ __device__ void vertex_payload(const Shell<PSDF>& S, const Geo& g, i3 base, f3 p
   // centre sample's)
   const float h = 0.5f * g.voxel_size;
   bool valid = true;
-  float sp[3] = {0, 0, 0}, sn[3] = {0, 0, 0};
+  float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;  // (sp[k] - sn[k]) / vs per axis, without an indexed array
   // not unrolled: the six stencils' live registers would cost mc_surfel_kernel
   // a workgroup per CU
 #pragma unroll 1
@@ -242,12 +242,15 @@ __device__ void vertex_payload(const Shell<PSDF>& S, const Geo& g, i3 base, f3 p
     SVal q{};
     const SAxis An = shell_axis(pk - h, vs, bk);
     valid = shell_value<PSDF, false>(S, k == 0 ? An : X, k == 1 ? An : Y, k == 2 ? An : Z, &q, oob) && valid;
-    sn[k] = q.sdf;
+    const float sn = q.sdf;
     const SAxis Ap = shell_axis(pk + h, vs, bk);
     valid = shell_value<PSDF, false>(S, k == 0 ? Ap : X, k == 1 ? Ap : Y, k == 2 ? Ap : Z, &q, oob) && valid;
-    sp[k] = q.sdf;
+    const float gk = (q.sdf - sn) / vs;
+    g0 = k == 0 ? gk : g0;
+    g1 = k == 1 ? gk : g1;
+    g2 = k == 2 ? gk : g2;
   }
-  const f3 grad = mk3((sp[0] - sn[0]) / vs, (sp[1] - sn[1]) / vs, (sp[2] - sn[2]) / vs);
+  const f3 grad = mk3(g0, g1, g2);
   const float l = length(grad);
   const f3 n = (l > 0.0f && valid) ? grad / l : mk3(0.0f, 0.0f, 0.0f);
   vt->nx = n.x;
