@@ -159,17 +159,12 @@ __device__ __forceinline__ Octant empty_octant() {
 // little.  Octant misses go to the wave's LDS table directly (the cache's
 // two-entry register cache is left out of the march: its registers cost
 // more than the lookups it saves).
-__device__ __forceinline__ bool march_sample(const View& V, BlockCache& cache, Octant& oc, f3 pos,
-                                             float inv_vs, float* out) {
-  const float vs = V.geo.voxel_size;
-  const float hv = 0.5f * vs;
-  const float pcx = pos.x - hv, pcy = pos.y - hv, pcz = pos.z - hv;
-  // per axis, so that a wave runs the divisions only for the axes on which
-  // some lane is near a tie
-  int cx0, cy0, cz0, cx1, cy1, cz1;
-  corner_pair(pcx, vs, inv_vs, &cx0, &cx1);
-  corner_pair(pcy, vs, inv_vs, &cy0, &cy1);
-  corner_pair(pcz, vs, inv_vs, &cz0, &cz1);
+// The sample from its corner voxels; ratios() gives (rx, ry, rz), taken
+// only once the sample is known to be valid.
+template <class Ratios>
+__device__ __forceinline__ bool sample_corners(const View& V, BlockCache& cache, Octant& oc, int cx0, int cx1,
+                                               int cy0, int cy1, int cz0, int cz1, Ratios ratios,
+                                               float* out) {
   const int bx = voxel_to_block1(cx0), by = voxel_to_block1(cy0), bz = voxel_to_block1(cz0);
   if (bx != oc.bx || by != oc.by || bz != oc.bz) {
     oc.bx = bx;
@@ -199,8 +194,8 @@ __device__ __forceinline__ bool march_sample(const View& V, BlockCache& cache, O
   const int32_t p7 = xx ? (yy ? (zz ? oc.p[7] : oc.p[6]) : (zz ? oc.p[5] : oc.p[4])) : p3;
   const int32_t pb[8] = {p0, p1, p2, p3, p4, p5, p6, p7};
   const int cx[2] = {cx0, cx1}, cy[2] = {cy0, cy1}, cz[2] = {cz0, cz1};
-  const float qx = pos.x / vs, qy = pos.y / vs, qz = pos.z / vs;
-  const float rx = qx - floorf(qx), ry = qy - floorf(qy), rz = qz - floorf(qz);
+  const f3 r = ratios();
+  const float rx = r.x, ry = r.y, rz = r.z;
   float sdf = 0.0f;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
@@ -211,6 +206,45 @@ __device__ __forceinline__ bool march_sample(const View& V, BlockCache& cache, O
   }
   *out = sdf;
   return true;
+}
+
+__device__ __forceinline__ bool march_sample(const View& V, BlockCache& cache, Octant& oc, f3 pos,
+                                             float inv_vs, float* out) {
+  const float vs = V.geo.voxel_size;
+  const float hv = 0.5f * vs;
+  const float pcx = pos.x - hv, pcy = pos.y - hv, pcz = pos.z - hv;
+  // per axis, so that a wave runs the divisions only for the axes on which
+  // some lane is near a tie
+  int cx0, cy0, cz0, cx1, cy1, cz1;
+  corner_pair(pcx, vs, inv_vs, &cx0, &cx1);
+  corner_pair(pcy, vs, inv_vs, &cy0, &cy1);
+  corner_pair(pcz, vs, inv_vs, &cz0, &cz1);
+  return sample_corners(V, cache, oc, cx0, cx1, cy0, cy1, cz0, cz1, [&] {
+    const float qx = pos.x / vs, qy = pos.y / vs, qz = pos.z / vs;
+    return mk3(qx - floorf(qx), qy - floorf(qy), qz - floorf(qz));
+  }, out);
+}
+
+// One axis of a sample at coordinate p: its two corner voxels (as
+// march_sample rounds them) and the ratio frac(p / vs).
+struct RAxis {
+  int c0, c1;
+  float r;
+};
+__device__ __forceinline__ RAxis ray_axis(float p, float vs, float inv_vs) {
+  RAxis a;
+  corner_pair(p - 0.5f * vs, vs, inv_vs, &a.c0, &a.c1);
+  const float q = p / vs;
+  a.r = q - floorf(q);
+  return a;
+}
+
+// march_sample from the sample's three axes (the normal's stencils share two
+// of them with the crossing point).
+__device__ __forceinline__ bool axes_sample(const View& V, BlockCache& cache, Octant& oc, const RAxis& X,
+                                            const RAxis& Y, const RAxis& Z, float* out) {
+  return sample_corners(V, cache, oc, X.c0, X.c1, Y.c0, Y.c1, Z.c0, Z.c1, [&] { return mk3(X.r, Y.r, Z.r); },
+                        out);
 }
 
 // A ray parameter below which every sample position o + t dir lies more
@@ -395,14 +429,20 @@ __device__ __forceinline__ uint32_t cast_pixel(const View& V, const RayConst& R,
       const float h = 0.5f * V.geo.voxel_size;
       bool valid = true;
       float sp[3] = {0, 0, 0}, sn[3] = {0, 0, 0};
+      const float vs = V.geo.voxel_size;
+      // iso -+ h e_k equals iso off axis k (x - 0 = x), so the stencils take
+      // the crossing point's axes there
+      const RAxis X = ray_axis(iso.x, vs, R.inv_vs), Y = ray_axis(iso.y, vs, R.inv_vs),
+                  Z = ray_axis(iso.z, vs, R.inv_vs);
 #pragma unroll
       for (int k = 0; k < 3 && valid; ++k) {
-        const f3 dp = mk3(k == 0 ? h : 0.0f, k == 1 ? h : 0.0f, k == 2 ? h : 0.0f);
-        valid = march_sample(V, cache, oc, iso - dp, R.inv_vs, &sn[k]);
+        const float ik = k == 0 ? iso.x : (k == 1 ? iso.y : iso.z);
+        const RAxis An = ray_axis(ik - h, vs, R.inv_vs);
+        valid = axes_sample(V, cache, oc, k == 0 ? An : X, k == 1 ? An : Y, k == 2 ? An : Z, &sn[k]);
         if (!valid) break;
-        valid = march_sample(V, cache, oc, iso + dp, R.inv_vs, &sp[k]);
+        const RAxis Ap = ray_axis(ik + h, vs, R.inv_vs);
+        valid = axes_sample(V, cache, oc, k == 0 ? Ap : X, k == 1 ? Ap : Y, k == 2 ? Ap : Z, &sp[k]);
       }
-      const float vs = V.geo.voxel_size;
       const f3 grad = mk3((sp[0] - sn[0]) / vs, (sp[1] - sn[1]) / vs, (sp[2] - sn[2]) / vs);
       const float l2 = length(grad);
       f3 nw = (l2 > 0.0f && valid) ? grad / l2 : mk3(0.0f, 0.0f, 0.0f);
