@@ -175,7 +175,7 @@ __device__ void insert_block(const View& V, i3 p) {
 template <class Src>
 __global__ __launch_bounds__(256) void alloc_kernel(Src src_, const SceneFrame* __restrict__ tab) {
   const Src src = with_table(src_, tab);
-  const View& V = src.view(blockIdx.z);
+  const View V = src.view(blockIdx.z);  // a copy: read once, not reloaded after stores
   const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15);
   const uint32_t y = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (x >= V.cam.width || y >= V.cam.height) return;
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(NT) void integrate_tsdf_kernel(Src src_, const Scen
       nblk = 0;
       cur = scene;
     }
-    const View& V = src.view(scene);
+    const View V = src.view(scene);  // a copy: read once, not reloaded after stores
     const i3 b = unpack_key(V.block_keys[ptr]);
     nblk += lt == 0;
     nupd += integrate_voxels<kBlockVoxels / kLanes, kLanes>(V, b, ptr, lt);

@@ -638,7 +638,7 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
   for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
     uint32_t scene;
     const int32_t ptr = src.item(ci, &scene);
-    const View& V = src.view(scene);
+    const View V = src.view(scene);  // a copy: read once, not reloaded after stores
     // This lane's vertex slots (voxels threadIdx.x and threadIdx.x + 256);
     // a block without any is done at once (no slot to free, no surfels).
     int32_t vps[2][3];

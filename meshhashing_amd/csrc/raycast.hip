@@ -761,8 +761,10 @@ __global__ __launch_bounds__(64, 5) void raycast_kernel(Src src_, const SceneFra
   __shared__ WaveTable table;
   wave_table_init(&table);
   const uint32_t scene = blockIdx.z;
-  const View& V = src.view(scene);
-  const RayConst& R = ray_of(src, scene, R1);
+  // copies, not references into the frame table: read once, and not
+  // reloaded after the stores of the march (LDS table, outputs)
+  const View V = src.view(scene);
+  const RayConst R = ray_of(src, scene, R1);
   const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7);
   const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
   uint32_t ns = 0;
