@@ -469,7 +469,6 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
     }
     __syncthreads();
     const i3 base = i3{b.x * kBlockSide, b.y * kBlockSide, b.z * kBlockSide};
-    const i3 o = devectorize(i);
     const int cube = own_cube[i];
     V.cube[idx] = (uint16_t)((uint32_t)cube | (prev << 8));
     // live triangles of a cube = its configuration's table row
