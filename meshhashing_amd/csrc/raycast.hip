@@ -7,9 +7,12 @@
 // gradient), geometry/voxel_query.h:102-129 (world-point voxel lookup).
 //
 // One lane per pixel, 8x8 pixels per wave.  Samples, the crossing test and
-// the three regula-falsi steps follow the reference's FP32 expressions; each
-// lane keeps the last two block lookups (including misses) so that runs of
-// samples in one block, or in one unallocated block, cost no hash probe.
+// the three regula-falsi steps follow the reference's FP32 expressions.  The
+// march keeps the eight blocks around its current lowest-corner block
+// (Octant) and shares block lookups, misses included, through a per-wave LDS
+// table, so runs of samples in one block, or in one unallocated block, cost
+// no hash probe; the adjoint and localisation kernels keep a lane's last two
+// lookups in registers (BlockCache) instead.
 // The depth output is metric z (t * dir_c.z); the reference colour-maps it
 // (ray_caster.cu:96-98) and keeps metric depth only in the vertex image.
 #include "mh_kernels.h"
