@@ -220,9 +220,12 @@ int mh_raycast(mh_volume* v, const mh_raycaster_params* p, const float cTw[16],
                float* depth_dev, float* vertex_dev, float* normal_dev);
 
 /* ---- denoising: PrimalDualInit / PrimalDualIterate, optimize/primal_dual.h:14-29
- * Runs over the current worklist (collect_all first, as the survey'd driver
- * does).  energy_host (3 doubles, may be NULL): data term, TV term and
- * err_data*lambda/2 + err_tv of the iteration's pass 1 (primal_dual.cu:207-214). */
+ * Runs over the current worklist (collect_all first, as the reference's
+ * driver does).  energy_host (3 doubles, may be NULL): data term, TV term and
+ * err_data*lambda/2 + err_tv of the iteration's pass 1 (primal_dual.cu:207-214).
+ * init builds a face-neighbour table over the worklist; iterate returns
+ * MH_ERR_STATE if any call has rebuilt or cleared the worklist since (alloc,
+ * collect, recycle, reset, a batch run): call init again after those.     */
 int mh_primal_dual_init(mh_volume* v);
 int mh_primal_dual_iterate(mh_volume* v, float lambda, float sigma, float tau,
                            double* energy_host);
@@ -297,8 +300,11 @@ typedef struct mh_tune_result {
   int32_t candidates;           /* configurations timed                     */
   float best_ms;                /* fastest trial of the chosen tiling       */
   float default_ms;             /* fastest trial of the default tiling      */
+  int32_t early_loads;          /* 1: voxel state loaded with the depth
+                                   gathers, block keys fetched per wave    */
 } mh_tune_result;
-#define MH_NUM_TILINGS 10
+#define MH_NUM_TILINGS 18
+#define MH_NUM_TUNE_GRIDS 5 /* persistent-grid sizes timed per tiling */
 int mh_tune_integrate(mh_volume* v, uint32_t reps, mh_tune_result* out);
 int mh_set_integrate_tiling(mh_volume* v, int tiling, int workgroups_per_cu);
 

@@ -96,10 +96,11 @@ class TuneResult(C.Structure):
     _fields_ = [("tiling", C.c_int32), ("workgroup_size", C.c_int32),
                 ("blocks_per_workgroup", C.c_int32), ("voxels_per_lane", C.c_int32),
                 ("workgroups_per_cu", C.c_int32), ("candidates", C.c_int32),
-                ("best_ms", C.c_float), ("default_ms", C.c_float)]
+                ("best_ms", C.c_float), ("default_ms", C.c_float), ("early_loads", C.c_int32)]
 
 
-MH_NUM_TILINGS = 10
+MH_NUM_TILINGS = 18
+MH_NUM_TUNE_GRIDS = 5
 MH_RUN_MESH = 1 << 0
 MH_RUN_RAYCAST = 1 << 1
 MH_RUN_GRAPH = 1 << 2

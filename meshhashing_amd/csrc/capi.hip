@@ -75,6 +75,26 @@ hipError_t dalloc(T** p, size_t n) {
   return hipMalloc((void**)p, n * sizeof(T) + 16);
 }
 
+// Device allocations freed together when the holder goes out of scope,
+// unless handed over to a long-lived owner list first: an entry point that
+// needs several buffers publishes them only once every one of them exists.
+struct Scratch {
+  std::vector<void*> ptrs;
+  ~Scratch() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  hipError_t get(T** p, size_t n) {
+    const hipError_t e = dalloc(p, n);
+    if (e == hipSuccess) ptrs.push_back((void*)*p);
+    return e;
+  }
+  void release_to(std::vector<void*>& owner) {
+    owner.insert(owner.end(), ptrs.begin(), ptrs.end());
+    ptrs.clear();
+  }
+};
+
 }  // namespace
 
 struct mh_volume {
@@ -97,6 +117,10 @@ struct mh_volume {
   bool have_frame = false;
   int32_t* pd_nbr = nullptr;
   uint32_t pd_nbr_cap = 0;
+  // Bumped by every call that rebuilds or clears the worklist; the
+  // primal-dual neighbour table records the value it was built at.
+  uint64_t work_gen = 0;
+  uint64_t pd_work_gen = ~0ull;
   double* pd_partials = nullptr;
   double* pd_energy = nullptr;
   uint32_t pd_npartials = 0;
@@ -390,6 +414,7 @@ int mh_volume_reset(mh_volume* v) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(v->stream));
   v->have_frame = false;
+  ++v->work_gen;
   return MH_OK;
 }
 
@@ -415,14 +440,15 @@ int mh_get_status(mh_volume* v, mh_status* out) {
   out->block_count = (uint32_t)((int64_t)v->hp.value_capacity - 1 - c.block_top);
   out->candidate_count = c.cand_count;
   out->vertex_count = (uint32_t)((int64_t)v->mp.max_vertex_count - 2 - c.vertex_top);
-  out->triangle_count = (uint32_t)c.tri_live;
-  if (c.tri_live > v->mp.max_triangle_count) out->flags |= MH_STATUS_TRIANGLE_HEAP_EXHAUSTED;
-  out->voxel_updates = c.voxel_updates;
-  out->blocks_integrated = c.blocks_integrated;
-  out->mc_vertex_writes = c.mc_vertex_writes;
-  out->mc_triangle_writes = c.mc_triangle_writes;
-  out->mc_blocks = c.mc_blocks;
-  out->ray_samples = c.ray_samples;
+  const WorkStripe w = work_totals(c);
+  out->triangle_count = (uint32_t)w.tri_live;
+  if (w.tri_live > v->mp.max_triangle_count) out->flags |= MH_STATUS_TRIANGLE_HEAP_EXHAUSTED;
+  out->voxel_updates = w.voxel_updates;
+  out->blocks_integrated = w.blocks_integrated;
+  out->mc_vertex_writes = w.mc_vertex_writes;
+  out->mc_triangle_writes = w.mc_triangle_writes;
+  out->mc_blocks = w.mc_blocks;
+  out->ray_samples = w.ray_samples;
   return MH_OK;
 }
 
@@ -496,12 +522,14 @@ int mh_alloc_blocks(mh_volume* v) {
   v->pending_raw = nullptr;
   v->pending_color = nullptr;
   launch_alloc(Va, v->stream);
+  ++v->work_gen;  // allocation clears the worklist
   HIP_TRY(hipGetLastError());
   return MH_OK;
 }
 
 static int collect(mh_volume* v, int frustum, uint32_t* count_host) {
   launch_collect(v->V, frustum, v->stream, v->cus * 4);
+  ++v->work_gen;
   HIP_TRY(hipGetLastError());
   if (count_host) *count_host = read_counters(v).cand_count;
   return MH_OK;
@@ -541,6 +569,7 @@ int mh_collect_garbage_blocks(mh_volume* v) {
 int mh_recycle_garbage_blocks(mh_volume* v, uint32_t* freed_host) {
   if (check_vol(v)) return MH_ERR_ARG;
   launch_gc_recycle(v->V, v->stream, v->cus);
+  ++v->work_gen;  // freed blocks may sit on the worklist
   HIP_TRY(hipGetLastError());
   if (freed_host) {
     Counters c = read_counters(v);
@@ -603,31 +632,38 @@ int mh_raycast(mh_volume* v, const mh_raycaster_params* p, const float cTw[16], 
 
 static int ensure_pd(mh_volume* v) {
   View& V = v->V;
-  if (!V.pd_sdf0) {
-    const size_t nvox = (size_t)v->hp.value_capacity * kBlockVoxels;
-    HIP_TRY(dalloc(&V.pd_sdf0, nvox));
-    v->owned.push_back(V.pd_sdf0);
-    HIP_TRY(dalloc(&V.pd_sdf_bar, nvox));
-    v->owned.push_back(V.pd_sdf_bar);
-    HIP_TRY(dalloc(&V.pd_w, nvox));
-    v->owned.push_back(V.pd_w);
-    HIP_TRY(dalloc(&V.pd_p, nvox * 3));
-    v->owned.push_back(V.pd_p);
-    HIP_TRY(dalloc(&V.pd_mask, nvox));
-    v->owned.push_back(V.pd_mask);
-    HIP_TRY(hipMemsetAsync(V.pd_mask, 0, nvox, v->stream));
-    HIP_TRY(hipMemsetAsync(V.pd_sdf0, 0, nvox * 4, v->stream));
-    HIP_TRY(hipMemsetAsync(V.pd_sdf_bar, 0, nvox * 4, v->stream));
-    HIP_TRY(hipMemsetAsync(V.pd_w, 0, nvox * 4, v->stream));
-    HIP_TRY(hipMemsetAsync(V.pd_p, 0, nvox * 12, v->stream));
-    HIP_TRY(dalloc(&v->pd_nbr, (size_t)v->hp.value_capacity * 6));
-    v->owned.push_back(v->pd_nbr);
-    v->pd_npartials = (uint32_t)v->grid_wg256();
-    HIP_TRY(dalloc(&v->pd_partials, (size_t)v->pd_npartials * 2));
-    v->owned.push_back(v->pd_partials);
-    HIP_TRY(dalloc(&v->pd_energy, 4));
-    v->owned.push_back(v->pd_energy);
-  }
+  if (V.pd_sdf0) return MH_OK;
+  const size_t nvox = (size_t)v->hp.value_capacity * kBlockVoxels;
+  const uint32_t np = (uint32_t)v->grid_wg256();
+  Scratch sc;
+  float *s0 = nullptr, *sb = nullptr, *w = nullptr, *p = nullptr;
+  uint8_t* m = nullptr;
+  int32_t* nbr = nullptr;
+  double *part = nullptr, *en = nullptr;
+  HIP_TRY(sc.get(&s0, nvox));
+  HIP_TRY(sc.get(&sb, nvox));
+  HIP_TRY(sc.get(&w, nvox));
+  HIP_TRY(sc.get(&p, nvox * 3));
+  HIP_TRY(sc.get(&m, nvox));
+  HIP_TRY(sc.get(&nbr, (size_t)v->hp.value_capacity * 6));
+  HIP_TRY(sc.get(&part, (size_t)np * 2));
+  HIP_TRY(sc.get(&en, 4));
+  HIP_TRY(hipMemsetAsync(m, 0, nvox, v->stream));
+  HIP_TRY(hipMemsetAsync(s0, 0, nvox * 4, v->stream));
+  HIP_TRY(hipMemsetAsync(sb, 0, nvox * 4, v->stream));
+  HIP_TRY(hipMemsetAsync(w, 0, nvox * 4, v->stream));
+  HIP_TRY(hipMemsetAsync(p, 0, nvox * 12, v->stream));
+  // every buffer exists: publish them together
+  sc.release_to(v->owned);
+  V.pd_sdf0 = s0;
+  V.pd_sdf_bar = sb;
+  V.pd_w = w;
+  V.pd_p = p;
+  V.pd_mask = m;
+  v->pd_nbr = nbr;
+  v->pd_partials = part;
+  v->pd_energy = en;
+  v->pd_npartials = np;
   return MH_OK;
 }
 
@@ -638,12 +674,17 @@ int mh_primal_dual_init(mh_volume* v) {
   launch_pd_neighbors(v->V, v->pd_nbr, v->stream, v->grid_wg256());
   launch_pd_init(v->V, v->pd_nbr, v->stream, v->grid_wg256());
   HIP_TRY(hipGetLastError());
+  v->pd_work_gen = v->work_gen;
   return MH_OK;
 }
 
 int mh_primal_dual_iterate(mh_volume* v, float lambda, float sigma, float tau, double* energy_host) {
   if (check_vol(v)) return MH_ERR_ARG;
   if (!v->V.pd_sdf0) return fail(MH_ERR_STATE, "mh_primal_dual_init not called");
+  // The face-neighbour table indexes the worklist as it was at init; any
+  // call that has rebuilt the worklist since makes it stale.
+  if (v->pd_work_gen != v->work_gen)
+    return fail(MH_ERR_STATE, "worklist changed since mh_primal_dual_init");
   launch_pd_iterate(v->V, v->pd_nbr, lambda, sigma, tau, v->pd_partials, v->pd_npartials,
                     v->pd_energy, v->stream, v->grid_wg256());
   HIP_TRY(hipGetLastError());
@@ -953,33 +994,38 @@ int mh_load_blocks(mh_volume* v, uint32_t n, const int32_t* pos_host, const floa
     keys[i] = pack_key(p);
   }
   const size_t nv = (size_t)n * kBlockVoxels;
+  Scratch sc;  // temporaries, freed on every return path
   uint64_t* dkeys = nullptr;
   float *dsdf = nullptr, *dw = nullptr, *da = nullptr, *db = nullptr;
   uint32_t* dc = nullptr;
-  HIP_TRY(dalloc(&dkeys, n));
-  HIP_TRY(dalloc(&dsdf, nv));
-  HIP_TRY(dalloc(&dw, nv));
+  HIP_TRY(sc.get(&dkeys, n));
+  HIP_TRY(sc.get(&dsdf, nv));
+  HIP_TRY(sc.get(&dw, nv));
   HIP_TRY(hipMemcpyAsync(dkeys, keys.data(), n * 8, hipMemcpyHostToDevice, v->stream));
   HIP_TRY(hipMemcpyAsync(dsdf, sdf_host, nv * 4, hipMemcpyHostToDevice, v->stream));
   HIP_TRY(hipMemcpyAsync(dw, weight_host, nv * 4, hipMemcpyHostToDevice, v->stream));
   if (color_host) {
-    HIP_TRY(dalloc(&dc, nv));
+    HIP_TRY(sc.get(&dc, nv));
     HIP_TRY(hipMemcpyAsync(dc, color_host, nv * 4, hipMemcpyHostToDevice, v->stream));
   }
   if (a_host) {
-    HIP_TRY(dalloc(&da, nv));
+    HIP_TRY(sc.get(&da, nv));
     HIP_TRY(hipMemcpyAsync(da, a_host, nv * 4, hipMemcpyHostToDevice, v->stream));
   }
   if (b_host) {
-    HIP_TRY(dalloc(&db, nv));
+    HIP_TRY(sc.get(&db, nv));
     HIP_TRY(hipMemcpyAsync(db, b_host, nv * 4, hipMemcpyHostToDevice, v->stream));
   }
+  // the copies read pageable host memory and the kernels the device copies:
+  // the stream is drained before the temporaries go
   launch_load_insert(v->V, dkeys, n, v->stream);
   launch_load_scatter(v->V, dkeys, n, dsdf, dw, dc, da, db, v->stream);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(v->stream));
-  for (void* p : {(void*)dkeys, (void*)dsdf, (void*)dw, (void*)dc, (void*)da, (void*)db})
-    if (p) (void)hipFree(p);
+  const hipError_t e = hipStreamSynchronize(v->stream);
+  if (e != hipSuccess) {
+    sc.ptrs.clear();  // in-flight work may still use them: leak rather than free
+    return fail(MH_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+  }
   return MH_OK;
 }
 
@@ -994,20 +1040,9 @@ int mh_load_blocks(mh_volume* v, uint32_t n, const int32_t* pos_host, const floa
 namespace {
 
 constexpr int kGridPerCu[] = {2, 4, 8, 16, 32};
+static_assert(sizeof(kGridPerCu) / sizeof(kGridPerCu[0]) == MH_NUM_TUNE_GRIDS, "grid list");
+static_assert(kNumIntegrateTilings == MH_NUM_TILINGS, "tiling list");
 constexpr int kDefaultGridPerCu = 8;
-
-struct Scratch {
-  std::vector<void*> ptrs;
-  ~Scratch() {
-    for (void* p : ptrs) (void)hipFree(p);
-  }
-  template <typename T>
-  hipError_t get(T** p, size_t n) {
-    const hipError_t e = dalloc(p, n);
-    if (e == hipSuccess) ptrs.push_back((void*)*p);
-    return e;
-  }
-};
 
 struct Events {
   hipEvent_t a = nullptr, b = nullptr;
@@ -1058,6 +1093,7 @@ int tune_loop(hipStream_t s, int cus, uint32_t reps, Run run, Restore restore, m
     out->workgroup_size = tl.nt;
     out->blocks_per_workgroup = tl.bpw;
     out->voxels_per_lane = kBlockVoxels * tl.bpw / tl.nt;
+    out->early_loads = tl.early ? 1 : 0;
     out->workgroups_per_cu = *best_g;
     out->candidates = cands;
     out->best_ms = best;
@@ -1381,6 +1417,7 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
     f.V = v->V;
     f.V.raw = b->raw + (size_t)s * b->W * b->H;
     v->have_frame = true;
+    ++v->work_gen;  // the chain rebuilds every scene's worklist
   }
   const int k = b->ring;
   b->ring = (k + 1) % mh_batch::kRing;
@@ -1730,17 +1767,32 @@ int ensure_compact(mh_volume* v) {
   const size_t nv = v->mp.max_vertex_count, cap = v->hp.value_capacity;
   const size_t nt = v->mp.max_triangle_count;
   const size_t chunks = std::max((nv + 1023) / 1024, (cap + 1023) / 1024) + 1;
-  auto A = [&](auto** p, size_t n) -> int {
-    HIP_TRY(dalloc(p, n));
-    v->owned.push_back((void*)*p);
-    return MH_OK;
-  };
-  int rc = 0;
-  if ((rc = A(&v->cm_used, nv)) || (rc = A(&v->cm_remap, nv)) || (rc = A(&v->cm_blk, cap)) ||
-      (rc = A(&v->cm_blk_off, cap)) || (rc = A(&v->cm_sums, chunks)) || (rc = A(&v->cm_total, 2)) ||
-      (rc = A(&v->cm_pos, 3 * nv)) || (rc = A(&v->cm_nrm, 3 * nv)) || (rc = A(&v->cm_col, 3 * nv)) ||
-      (rc = A(&v->cm_tri, 3 * nt)))
-    return rc;
+  Scratch sc;
+  uint32_t *used, *remap, *blk, *blk_off, *sums, *total;
+  float *pos, *nrm, *col;
+  int32_t* tri;
+  HIP_TRY(sc.get(&used, nv));
+  HIP_TRY(sc.get(&remap, nv));
+  HIP_TRY(sc.get(&blk, cap));
+  HIP_TRY(sc.get(&blk_off, cap));
+  HIP_TRY(sc.get(&sums, chunks));
+  HIP_TRY(sc.get(&total, 2));
+  HIP_TRY(sc.get(&pos, 3 * nv));
+  HIP_TRY(sc.get(&nrm, 3 * nv));
+  HIP_TRY(sc.get(&col, 3 * nv));
+  HIP_TRY(sc.get(&tri, 3 * nt));
+  // every buffer exists: publish them together
+  sc.release_to(v->owned);
+  v->cm_used = used;
+  v->cm_remap = remap;
+  v->cm_blk = blk;
+  v->cm_blk_off = blk_off;
+  v->cm_sums = sums;
+  v->cm_total = total;
+  v->cm_pos = pos;
+  v->cm_nrm = nrm;
+  v->cm_col = col;
+  v->cm_tri = tri;
   return MH_OK;
 }
 

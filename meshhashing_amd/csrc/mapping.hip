@@ -60,12 +60,12 @@ __global__ void reset_voxels_kernel(View V, size_t nvox) {
 // i.e. whichever vertex happened to be allocated first; reserving the slot
 // keeps that expression while excluding no real vertex.
 __global__ void reset_counters_kernel(View V) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    Counters c{};
-    c.block_top = (int32_t)V.capacity - 1;
-    c.block_hwm = 0;
-    c.vertex_top = (int32_t)V.max_vertices - 2;
-    *V.ctr = c;
+  uint32_t* w = reinterpret_cast<uint32_t*>(V.ctr);
+  for (uint32_t i = threadIdx.x; i < sizeof(Counters) / 4; i += blockDim.x) w[i] = 0u;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    V.ctr->block_top = (int32_t)V.capacity - 1;
+    V.ctr->vertex_top = (int32_t)V.max_vertices - 2;
   }
 }
 
@@ -156,8 +156,15 @@ __device__ void insert_block(const View& V, i3 p) {
       const uint32_t ob = occ_bit(key);
       atomicOr(&V.occ[ob >> 5], 1u << (ob & 31u));
       const int32_t ptr = alloc_block(V);
+      if (ptr < 0) {
+        // Heap exhausted (flagged by alloc_block): give the slot back, so
+        // the key is not taken for allocated and a later frame retries it
+        // once recycling has returned pool slots.
+        __hip_atomic_store(&e->key, kEmptyKey, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
       e->ptr = ptr;
-      if (ptr >= 0) V.block_keys[ptr] = key;
+      V.block_keys[ptr] = key;
       return;
     }
     if (old == key) return;
@@ -249,9 +256,10 @@ __global__ __launch_bounds__(256) void alloc_kernel(Src src_, const SceneFrame* 
 // Serial list insertion for keys whose bucket was full (hash_table.h:145-177):
 // probe offsets 1..linked_list_size from the bucket's last slot, skipping
 // multiples of bucket_size, and link the first free slot in as list head.
-// Also clears the worklist counter for the collect pass that follows.
-// One workgroup per scene (blockIdx.x), one working lane.
-template <class Src>
+// After a frame's allocation (kClearWork) it also clears the worklist
+// counter for the collect pass that follows; loading blocks leaves the
+// worklist as it is.  One workgroup per scene (blockIdx.x), one working lane.
+template <class Src, bool kClearWork = true>
 __global__ void list_insert_kernel(Src src_, const SceneFrame* __restrict__ tab) {
   const Src src = with_table(src_, tab);
   const View& V = src.view(blockIdx.x);
@@ -268,10 +276,14 @@ __global__ void list_insert_kernel(Src src_, const SceneFrame* __restrict__ tab)
     for (uint32_t j = 0; j < V.bucket_size && !done; ++j) {
       Entry* e = &V.entries[first + j];
       if (e->key == kEmptyKey) {
-        e->key = key;
+        // a slot is written only once a pool index is in hand, so an
+        // exhausted heap (flagged by alloc_block) leaves no key behind
         const int32_t ptr = alloc_block(V);
-        e->ptr = ptr;
-        if (ptr >= 0) V.block_keys[ptr] = key;
+        if (ptr >= 0) {
+          e->key = key;
+          e->ptr = ptr;
+          V.block_keys[ptr] = key;
+        }
         done = true;
       }
     }
@@ -284,13 +296,15 @@ __global__ void list_insert_kernel(Src src_, const SceneFrame* __restrict__ tab)
       const uint32_t i = (last + offset) % V.entry_count;
       Entry* e = &V.entries[i];
       if (e->key == kEmptyKey) {
-        e->key = key;
-        e->offset = V.entries[last].offset;
         const int32_t ptr = alloc_block(V);
-        e->ptr = ptr;
-        if (ptr >= 0) V.block_keys[ptr] = key;
-        V.entries[last].offset = offset;
-        V.ctr->list_inserts += 1u;
+        if (ptr >= 0) {
+          e->key = key;
+          e->offset = V.entries[last].offset;
+          e->ptr = ptr;
+          V.block_keys[ptr] = key;
+          V.entries[last].offset = offset;
+          V.ctr->list_inserts += 1u;
+        }
         done = true;
         break;
       }
@@ -301,7 +315,7 @@ __global__ void list_insert_kernel(Src src_, const SceneFrame* __restrict__ tab)
     }
   }
   V.ctr->ovf_count = 0;
-  V.ctr->cand_count = 0;
+  if (kClearWork) V.ctr->cand_count = 0;
 }
 
 // --------------------------------------------------------------- compaction
@@ -449,8 +463,8 @@ __global__ __launch_bounds__(NT) void integrate_tsdf_kernel(Src src_, const Scen
     if (scene != cur) {
       if (cur != ~0u) {
         const View& P = src.view(cur);
-        wave_add_u64(&P.ctr->voxel_updates, nupd);
-        wave_add_u64(&P.ctr->blocks_integrated, nblk);
+        wave_add_u64(&work_stripe(P.ctr).voxel_updates, nupd);
+        wave_add_u64(&work_stripe(P.ctr).blocks_integrated, nblk);
       }
       nupd = 0;
       nblk = 0;
@@ -463,8 +477,135 @@ __global__ __launch_bounds__(NT) void integrate_tsdf_kernel(Src src_, const Scen
   }
   if (cur != ~0u) {
     const View& P = src.view(cur);
-    wave_add_u64(&P.ctr->voxel_updates, nupd);
-    wave_add_u64(&P.ctr->blocks_integrated, nblk);
+    wave_add_u64(&work_stripe(P.ctr).voxel_updates, nupd);
+    wave_add_u64(&work_stripe(P.ctr).blocks_integrated, nblk);
+  }
+}
+
+// The same fusion with the latency chain cut short.  A lane's voxel state
+// is loaded before the depth test, together with the projection's depth
+// gathers, so a block costs one memory round trip instead of two (the state
+// of a voxel that then fails the test is read and left alone: its cache
+// line is mostly fetched anyway, since the test splits lines).  Each wave
+// fetches the worklist items and block keys of its next 64 blocks with one
+// vector load each, so no block waits on its own key.  Same arithmetic and
+// the same stores as integrate_voxels, so the results are bit-identical.
+template <int K, uint32_t L>
+__device__ __forceinline__ uint32_t integrate_voxels_early(const View& V, const i3 b, int32_t ptr,
+                                                           uint32_t lt) {
+  const Geo& g = V.geo;
+  const Cam& c = V.cam;
+  const size_t base = (size_t)ptr * kBlockVoxels + lt;
+  float so[K], wo[K];
+  uint32_t co[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    so[k] = V.sdf[base + (uint32_t)k * L];
+    wo[k] = V.weight[base + (uint32_t)k * L];
+    co[k] = V.color[base + (uint32_t)k * L];
+  }
+  uint32_t pix[K];
+  bool live[K];
+  float depth[K], zc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const i3 o = devectorize(lt + (uint32_t)k * L);
+    const i3 vp = i3{b.x * kBlockSide + o.x, b.y * kBlockSide + o.y, b.z * kBlockSide + o.z};
+    const f3 cp = xform(V.cTw, voxel_to_world(g, vp));
+    const uint32_t u = (uint32_t)f2i(proj_u(c, cp) + 0.5f);
+    const uint32_t v = (uint32_t)f2i(proj_v(c, cp) + 0.5f);
+    live[k] = u < c.width && v < c.height;
+    pix[k] = live[k] ? v * c.width + u : 0u;
+    zc[k] = cp.z;
+    depth[k] = live[k] ? V.depth[pix[k]] : 0.0f;
+  }
+  float wgt[K], sdf[K];
+  uint32_t dc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float d = depth[k];
+    live[k] = live[k] && !(d == -INFINITY || d == 0.0f || d >= g.sdf_upper_bound);
+    const float sdf_dec = d - zc[k];
+    wgt[k] = fmaxf(10.0f * g.weight_sample * (1.0f - normalize_depth(d, c.min_depth, c.max_depth)), 1.0f);
+    const float tr = truncate_distance(g, d);
+    live[k] = live[k] && !(sdf_dec <= -tr);
+    sdf[k] = 0.0f;
+    dc[k] = 255u << 8;  // (0, 255, 0)
+    if (!live[k]) continue;
+    const i3 o = devectorize(lt + (uint32_t)k * L);
+    const i3 vp = i3{b.x * kBlockSide + o.x, b.y * kBlockSide + o.y, b.z * kBlockSide + o.z};
+    const double vsd = (double)g.voxel_size;
+    const double X = (double)vp.x * vsd, Y = (double)vp.y * vsd, Z = (double)vp.z * vsd;
+    const double z64 = fma(V.cTw64.m[8], X, fma(V.cTw64.m[9], Y, fma(V.cTw64.m[10], Z, V.cTw64.m[11])));
+    double sv = (double)d - z64;
+    const double trd = (double)tr;
+    sv = sv > trd ? trd : (sv < -trd ? -trd : sv);
+    sdf[k] = (float)sv;
+    if (V.has_color) dc[k] = V.color_in[pix[k]];
+  }
+  uint32_t n = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (!live[k]) continue;
+    const size_t idx = base + (uint32_t)k * L;
+    V.sdf[idx] = (so[k] * wo[k] + sdf[k] * wgt[k]) / (wo[k] + wgt[k]);
+    V.weight[idx] = wo[k] + wgt[k];
+    V.color[idx] = blend_color(co[k], dc[k]);
+    ++n;
+  }
+  return n;
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t x, uint32_t j) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, j);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), j);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+template <class Src, int NT, int BPW>
+__global__ __launch_bounds__(NT) void integrate_tsdf_early_kernel(Src src_, const SceneFrame* __restrict__ tab) {
+  const Src src = with_table(src_, tab);
+  constexpr uint32_t kLanes = NT / BPW;
+  static_assert(kLanes % 64 == 0 && kBlockVoxels % kLanes == 0, "tiling");
+  const uint32_t n = src.count();
+  const uint32_t sub = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes), lt = threadIdx.x % kLanes;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t stride = gridDim.x * BPW;
+  uint32_t nupd = 0, nblk = 0, cur = ~0u;
+  for (uint32_t c0 = blockIdx.x * BPW + sub; c0 < n; c0 += 64u * stride) {
+    // lane j holds this wave's j-th next item: scene, pool index and key
+    const uint32_t cj = c0 + lane * stride;
+    uint32_t sj = 0;
+    int32_t pj = 0;
+    uint64_t kj = 0;
+    if (cj < n) {
+      pj = src.lane_item(cj, &sj);
+      kj = src.view(sj).block_keys[pj];
+    }
+    const uint32_t m = min(64u, (n - c0 + stride - 1u) / stride);
+    for (uint32_t j = 0; j < m; ++j) {
+      const uint32_t scene = __builtin_amdgcn_readlane(sj, j);
+      const int32_t ptr = __builtin_amdgcn_readlane(pj, j);
+      const i3 b = unpack_key(readlane_u64(kj, j));
+      if (scene != cur) {
+        if (cur != ~0u) {
+          const View& P = src.view(cur);
+          wave_add_u64(&work_stripe(P.ctr).voxel_updates, nupd);
+          wave_add_u64(&work_stripe(P.ctr).blocks_integrated, nblk);
+        }
+        nupd = 0;
+        nblk = 0;
+        cur = scene;
+      }
+      const View V = src.view(scene);
+      nblk += lt == 0;
+      nupd += integrate_voxels_early<kBlockVoxels / kLanes, kLanes>(V, b, ptr, lt);
+    }
+  }
+  if (cur != ~0u) {
+    const View& P = src.view(cur);
+    wave_add_u64(&work_stripe(P.ctr).voxel_updates, nupd);
+    wave_add_u64(&work_stripe(P.ctr).blocks_integrated, nblk);
   }
 }
 
@@ -554,7 +695,7 @@ __global__ __launch_bounds__(256) void integrate_psdf_kernel(View V) {
   const Cam c = V.cam;
   const double vsd = (double)g.voxel_size;
   uint32_t nupd = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&V.ctr->blocks_integrated, (unsigned long long)n);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&work_stripe(V.ctr).blocks_integrated, (unsigned long long)n);
   for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
     const int32_t ptr = V.cand[ci];
     const i3 b = unpack_key(V.block_keys[ptr]);
@@ -612,7 +753,7 @@ __global__ __launch_bounds__(256) void integrate_psdf_kernel(View V) {
       V.vb[idx] = na * (1.0f - f) / f;
     }
   }
-  wave_add_u64(&V.ctr->voxel_updates, nupd);
+  wave_add_u64(&work_stripe(V.ctr).voxel_updates, nupd);
 }
 
 // ------------------------------------------------------------ block restore
@@ -644,7 +785,7 @@ __global__ __launch_bounds__(256) void load_scatter_kernel(View V, const uint64_
 
 void launch_load_insert(const View& V, const uint64_t* keys, uint32_t n, hipStream_t s) {
   load_insert_kernel<<<(n + 255) / 256, 256, 0, s>>>(V, keys, n);
-  list_insert_kernel<<<1, 64, 0, s>>>(One{V}, nullptr);
+  list_insert_kernel<One, false><<<1, 64, 0, s>>>(One{V}, nullptr);
 }
 
 void launch_load_scatter(const View& V, const uint64_t* keys, uint32_t n, const float* sdf,
@@ -906,7 +1047,7 @@ void launch_reset(const View& V, hipStream_t s, int grid) {
   reset_entries_kernel<<<grid, 256, 0, s>>>(V.entries, V.entry_count);
   reset_pool_kernel<<<grid, 256, 0, s>>>(V);
   reset_voxels_kernel<<<grid * 4, 256, 0, s>>>(V, (size_t)V.capacity * kBlockVoxels);
-  reset_counters_kernel<<<1, 64, 0, s>>>(V);
+  reset_counters_kernel<<<1, 256, 0, s>>>(V);
 }
 
 void launch_convert_depth(const View& V, const uint16_t* raw, hipStream_t s) {
@@ -937,9 +1078,12 @@ template <class Src, int T = 0>
 void integrate_dispatch(const Src& src, hipStream_t s, int grid, int tiling) {
   if constexpr (T < kNumIntegrateTilings) {
     constexpr int NT = kIntegrateTilings[T].nt, BPW = kIntegrateTilings[T].bpw;
-    if (tiling == T)
-      integrate_tsdf_kernel<Src, NT, BPW><<<grid, NT, 0, s>>>(src, table_of(src));
-    else
+    if (tiling == T) {
+      if constexpr (kIntegrateTilings[T].early)
+        integrate_tsdf_early_kernel<Src, NT, BPW><<<grid, NT, 0, s>>>(src, table_of(src));
+      else
+        integrate_tsdf_kernel<Src, NT, BPW><<<grid, NT, 0, s>>>(src, table_of(src));
+    } else
       integrate_dispatch<Src, T + 1>(src, s, grid, tiling);
   }
 }

@@ -352,12 +352,12 @@ struct McCounts {
     if (__any(oob)) {
       if ((threadIdx.x & 63u) == 0) flag(V, MH_STATUS_STENCIL_RANGE);
     }
-    wave_add_u64(&V.ctr->mc_vertex_writes, writes);
-    wave_add_u64(&V.ctr->mc_triangle_writes, assigned);
-    wave_add_u64(&V.ctr->mc_blocks, blocks);
+    wave_add_u64(&work_stripe(V.ctr).mc_vertex_writes, writes);
+    wave_add_u64(&work_stripe(V.ctr).mc_triangle_writes, assigned);
+    wave_add_u64(&work_stripe(V.ctr).mc_blocks, blocks);
     for (int off = 32; off > 0; off >>= 1) delta += __shfl_xor(delta, off);
     if ((threadIdx.x & 63u) == 0 && delta)
-      atomicAdd(&V.ctr->tri_live, (unsigned long long)(long long)delta);
+      atomicAdd(&work_stripe(V.ctr).tri_live, (unsigned long long)(long long)delta);
     *this = McCounts{};
   }
 };
@@ -759,7 +759,7 @@ __global__ __launch_bounds__(512) void gc_release_kernel(View V) {
   }
   for (int off = 32; off > 0; off >>= 1) delta += __shfl_xor(delta, off);
   if ((threadIdx.x & 63u) == 0 && delta)
-    atomicAdd(&V.ctr->tri_live, (unsigned long long)(long long)delta);
+    atomicAdd(&work_stripe(V.ctr).tri_live, (unsigned long long)(long long)delta);
 }
 
 void launch_gc_release_triangles(const View& V, hipStream_t s, int grid) {

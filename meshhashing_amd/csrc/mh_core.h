@@ -210,6 +210,23 @@ MH_HD uint32_t hash_bucket(i3 p, uint32_t bucket_count) {
   return h % bucket_count;
 }
 
+// Work counters (accumulated since reset), for throughput and the roofline.
+// They are summed by one atomic per wave; every wave of a launch adding to
+// one address serialises those atomics device-wide (they cost more than a
+// whole fusion pass at high occupancy), so the counters are striped over
+// kWorkStripes 256-byte lines, picked by workgroup, and the host adds the
+// stripes up.
+constexpr uint32_t kWorkStripes = 64;
+struct alignas(256) WorkStripe {
+  unsigned long long voxel_updates;      // voxels whose fused state changed
+  unsigned long long blocks_integrated;  // worklist blocks visited by fusion
+  unsigned long long mc_vertex_writes;   // vertex records written by MC pass 1
+  unsigned long long mc_triangle_writes; // triangles of cubes whose configuration changed
+  unsigned long long mc_blocks;          // worklist blocks meshed
+  unsigned long long ray_samples;        // valid trilinear samples of the raycaster
+  unsigned long long tri_live;           // live triangles (sum of table rows; deltas wrap)
+};
+
 // Device-wide counters and status, one instance per volume.
 struct Counters {
   int32_t block_top;       // heap counter (hash_table.h:279-286)
@@ -223,15 +240,28 @@ struct Counters {
   uint32_t dda_cap;
   uint32_t list_inserts;
   uint32_t gc_free_count;  // garbage blocks left with no vertex (to be freed)
-  // work counters (accumulated since reset), for throughput and roofline
-  unsigned long long voxel_updates;      // voxels whose fused state changed
-  unsigned long long blocks_integrated;  // worklist blocks visited by fusion
-  unsigned long long mc_vertex_writes;   // vertex records written by MC pass 1
-  unsigned long long mc_triangle_writes; // triangles of cubes whose configuration changed
-  unsigned long long mc_blocks;          // worklist blocks meshed
-  unsigned long long ray_samples;        // valid trilinear samples of the raycaster
-  unsigned long long tri_live;           // live triangles (sum of table rows)
+  WorkStripe work[kWorkStripes];
 };
+
+// The stripe this workgroup adds its work counts to.
+__device__ __forceinline__ WorkStripe& work_stripe(Counters* c) {
+  return c->work[blockIdx.x % kWorkStripes];
+}
+
+// Sum of the stripes (host side).
+inline WorkStripe work_totals(const Counters& c) {
+  WorkStripe t{};
+  for (uint32_t i = 0; i < kWorkStripes; ++i) {
+    t.voxel_updates += c.work[i].voxel_updates;
+    t.blocks_integrated += c.work[i].blocks_integrated;
+    t.mc_vertex_writes += c.work[i].mc_vertex_writes;
+    t.mc_triangle_writes += c.work[i].mc_triangle_writes;
+    t.mc_blocks += c.work[i].mc_blocks;
+    t.ray_samples += c.work[i].ray_samples;
+    t.tri_live += c.work[i].tri_live;
+  }
+  return t;
+}
 
 // Sum v over the wave and add it to *dst once.  Every lane of the wave must
 // reach the call.

@@ -125,9 +125,13 @@ struct BatchDev {
 // gets NT / BPW lanes, a multiple of 64, and 512 * BPW / NT voxels per lane).
 struct Tiling {
   int nt, bpw;
+  bool early;  // state loaded with the depth gathers, keys fetched per wave
 };
-constexpr Tiling kIntegrateTilings[] = {{64, 1},  {128, 1}, {256, 1}, {512, 1}, {128, 2},
-                                        {256, 2}, {256, 4}, {512, 2}, {512, 4}, {512, 8}};
+constexpr Tiling kIntegrateTilings[] = {
+    {64, 1, false},  {128, 1, false}, {256, 1, false}, {512, 1, false}, {128, 2, false},
+    {256, 2, false}, {256, 4, false}, {512, 2, false}, {512, 4, false}, {512, 8, false},
+    {64, 1, true},   {128, 1, true},  {256, 1, true},  {512, 1, true},  {128, 2, true},
+    {256, 2, true},  {256, 4, true},  {512, 4, true}};
 constexpr int kNumIntegrateTilings = sizeof(kIntegrateTilings) / sizeof(kIntegrateTilings[0]);
 constexpr int kDefaultIntegrateTiling = 2;  // 256 lanes, one block, 2 voxels per lane
 

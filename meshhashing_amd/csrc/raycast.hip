@@ -788,7 +788,7 @@ __global__ __launch_bounds__(64, 5) void raycast_kernel(Src src_, const SceneFra
       normal_out[3 * pix + 2] = nn.z;
     }
   }
-  wave_add_u64(&V.ctr->ray_samples, ns);
+  wave_add_u64(&work_stripe(V.ctr).ray_samples, ns);
 }
 
 // ------------------------------------------------------------ block ranges

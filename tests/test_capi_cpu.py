@@ -73,3 +73,11 @@ def test_synth_spline_speed_limits():
     assert max(v) <= 0.5 + 1e-3 and max(w) <= 30.0 + 0.5
     holes = synth_depth(cfg, 10) == 0
     assert 0.01 < holes.mean() < 0.05
+
+
+def test_header_constants_match_binding():
+    text = (ROOT / "include" / "meshhashing.h").read_text()
+    for name in ("MH_NUM_TILINGS", "MH_NUM_TUNE_GRIDS"):
+        m = re.search(rf"#define {name} (\d+)", text)
+        assert m and int(m.group(1)) == getattr(A, name), name
+    assert C.sizeof(A.TuneResult) == 36

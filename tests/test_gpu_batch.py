@@ -149,7 +149,7 @@ def test_batch_tuning_changes_nothing(hip_device):
     print(f"[tune] batch: tiling {tune.tiling} ({tune.workgroup_size} lanes, "
           f"{tune.blocks_per_workgroup} blocks/wg, {tune.workgroups_per_cu} wg/CU) "
           f"{tune.best_ms:.4f} ms vs default {tune.default_ms:.4f} ms")
-    assert tune.candidates == 50 and 0 <= tune.tiling < A.MH_NUM_TILINGS
+    assert tune.candidates == A.MH_NUM_TILINGS * A.MH_NUM_TUNE_GRIDS and 0 <= tune.tiling < A.MH_NUM_TILINGS
     assert tune.best_ms <= tune.default_ms
     for f in frames:
         for s in range(len(cfgs)):
@@ -194,5 +194,29 @@ def test_tune_integrate_changes_nothing(hip_device):
     ref, _ = run_tsdf(cfg, frames)
     vol, res = run_tsdf(cfg, frames, tune_at=2)
     print(f"[tune] volume: tiling {res.tiling} {res.best_ms:.4f} ms vs default {res.default_ms:.4f} ms")
-    assert res.candidates == 50
+    assert res.candidates == A.MH_NUM_TILINGS * A.MH_NUM_TUNE_GRIDS
     assert_same(scene_state(vol, mesh=False), scene_state(ref, mesh=False), "tuned")
+
+
+def test_c4_all_scenes_batched_equal_sequential(hip_device):
+    """The whole batched workload: 64 scenes x 4 frames of 320x240, voxel
+    sizes 1-4 cm, replayed as one HIP graph per frame (integration tuned on
+    the first frame), against each scene run alone: every scene's state and
+    every frame's raycast images bit-identical."""
+    cfgs = c4_scenes(64, 4)
+    frames = range(4)
+    b, bimg, tune = batched(cfgs, frames, hip_device, graph=True, tune_first=True)
+    counts = []
+    for s, cfg in enumerate(cfgs):
+        vol, simg = sequential(cfg, frames, hip_device)
+        assert_same(scene_state(b.volumes[s]), scene_state(vol), f"scene {s}")
+        for f in frames:
+            assert np.array_equal(bimg[f][s][0], simg[f][0]), f"scene {s} frame {f}: depth"
+            assert np.array_equal(bimg[f][s][1], simg[f][1]), f"scene {s} frame {f}: normal"
+        st = vol.status()
+        assert st["flags"] == 0, st
+        counts.append(st["block_count"])
+        vol.close()
+    print(f"[c4] 64 scenes equal; blocks per scene {min(counts)}-{max(counts)}, tiling {tune.tiling}")
+    assert len(set(c.voxel_size for c in cfgs)) == 4
+    b.close()
