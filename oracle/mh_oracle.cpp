@@ -1730,6 +1730,24 @@ int orc_perturb_sdf(orc_volume* v, uint32_t block, uint32_t li, double delta) {
   return 0;
 }
 
+// Test hooks for closed-form pins: ValToRGB itself, and a hand-placed
+// vertex record in a voxel's x/y/z slot (the predict step reads a voxel's
+// own three slots, update_bayesian.cu:76-95).
+void orc_val_to_rgb(double v, double vmin, double vmax, double* out) { val_to_rgb(v, vmin, vmax, out); }
+int orc_set_vertex(orc_volume* v, const int32_t* key, const float* pos, const double* normal,
+                   double radius) {
+  if (key[3] < 0 || key[3] > 2) return -1;
+  Vtx vt;
+  for (int k = 0; k < 3; ++k) {
+    vt.pos[k] = pos[k];
+    vt.normal[k] = normal[k];
+    vt.color[k] = 0.0;
+  }
+  vt.radius = radius;
+  v->verts[VKey{{key[0], key[1], key[2], key[3]}}] = vt;
+  return 0;
+}
+
 int orc_mc_ntri(int cube) { return ntri(cube); }
 int orc_mc_tri_edge(int cube, int k) { return tri_edge(cube, k); }
 uint32_t orc_mc_edge_mask(int cube) { return edge_mask(cube); }

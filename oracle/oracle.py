@@ -86,6 +86,9 @@ def lib() -> C.CDLL:
             "orc_collect_garbage": (C.c_int, [vp]),
             "orc_recycle_garbage": (C.c_uint32, [vp]),
             "orc_dump_cand_flags": (C.c_uint32, [vp, P(C.c_int32)]),
+            "orc_val_to_rgb": (None, [C.c_double, C.c_double, C.c_double, P(C.c_double)]),
+            "orc_set_vertex": (C.c_int, [vp, P(C.c_int32), P(C.c_float), P(C.c_double),
+                                         C.c_double]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -167,6 +170,14 @@ class Oracle:
                            _p(nrm, C.c_float))
         return (d.reshape(p.height, p.width), vtx.reshape(p.height, p.width, 3),
                 nrm.reshape(p.height, p.width, 3))
+
+    def set_vertex(self, key, pos, normal, radius):
+        """Place a vertex record in voxel key[:3]'s slot key[3] (test hook)."""
+        k = np.ascontiguousarray(key, np.int32)
+        p = np.ascontiguousarray(pos, np.float32)
+        n = np.ascontiguousarray(normal, np.float64)
+        assert self.L.orc_set_vertex(self.h, _p(k, C.c_int32), _p(p, C.c_float),
+                                     _p(n, C.c_double), float(radius)) == 0
 
     def pd_init(self):
         self.L.orc_pd_init(self.h)
@@ -339,6 +350,12 @@ class Oracle:
 
     def perturb_sdf(self, block: int, voxel: int, delta: float):
         assert self.L.orc_perturb_sdf(self.h, block, voxel, delta) == 0
+
+
+def val_to_rgb(v: float, vmin: float, vmax: float) -> np.ndarray:
+    out = np.zeros(3, np.float64)
+    lib().orc_val_to_rgb(float(v), float(vmin), float(vmax), _p(out, C.c_double))
+    return out
 
 
 def hash_bucket(x: int, y: int, z: int, n: int) -> int:

@@ -108,3 +108,40 @@ def test_pose_refinement_converges(hip_device):
         assert ratio < 1 / 3 and r1 < 0.3
         assert cond <= 1e-2 or t1 < 0.005
     assert sum(cond > 1e-2 for *_, cond in rows) >= 2
+
+
+def test_raycast_backward_matches_oracle_at_5mm_640x480(hip_device, oracle_mod):
+    """The C5 volume (the room of the 320x240 sequence fused at 5 mm from its
+    20 frames at 640x480) and a 640x480 view perturbed by ~1 deg / 2 cm:
+    pixel set, loss, dL/dxi and J^T J within 1e-4 relative of the oracle's
+    FP32-march adjoint, dL/dsdf within 1e-4 of its largest entry."""
+    import torch
+    from meshhashing_amd.configs import C5
+    cfg = C5
+    vol = fused_volume(cfg, range(cfg.frames))
+    g = vol.dump_blocks()
+    print(f"[adjoint c5] {len(g['pos'])} blocks", flush=True)
+    orc = oracle_mod.Oracle(cfg)
+    orc.load_blocks(g["pos"], g["sdf"], g["weight"], g["color"])
+    del g
+    for f, xi in ((5, [0.010, -0.012, 0.006, 0.012, -0.015, 0.004]),
+                  (13, [-0.014, 0.004, 0.011, -0.008, 0.01, 0.016])):
+        wTc = se3_exp(xi) @ synth_pose(cfg, f).astype(np.float64)
+        cTw = np.linalg.inv(wTc).astype(np.float32)
+        obs = (synth_depth(cfg, f) * np.float32(0.001)).astype(np.float32)
+        obs_dev = torch.from_numpy(obs).to(hip_device)
+        gs = torch.zeros(cfg.value_capacity * 512, dtype=torch.float32, device=hip_device)
+        out = vol.raycast_backward(cTw, obs_dev.data_ptr(), gs.data_ptr())
+        ref = orc.raycast_adjoint(False, cTw, obs, grad_sdf=True)
+        gsdf = vol.dump_voxel_field(gs.data_ptr())
+        del gs
+        ge = np.abs(out["grad"] - ref["grad"]).max() / np.abs(ref["grad"]).max()
+        he = np.abs(out["hessian"] - ref["hessian"]).max() / np.abs(ref["hessian"]).max()
+        se = np.abs(gsdf - ref["grad_sdf"]).max() / np.abs(ref["grad_sdf"]).max()
+        print(f"[adjoint c5] frame {f}: pixels {out['pixels']} vs {ref['pixels']}, loss "
+              f"{out['loss']:.6e} vs {ref['loss']:.6e}; rel err grad {ge:.2e}, J^T J {he:.2e}, "
+              f"dL/dsdf {se:.2e}", flush=True)
+        assert out["pixels"] == ref["pixels"] and out["pixels"] > 150_000
+        assert abs(out["loss"] - ref["loss"]) <= 1e-4 * ref["loss"]
+        assert ge < 1e-4 and he < 1e-4 and se < 1e-4
+    vol.close()

@@ -1,7 +1,8 @@
 """The bench workload at its full size (640x480 depth, 5 mm voxels, 2 cm
-truncation, 2^20 x 10 hash entries, 2^19 blocks), checked through properties
-that hold at any size; the oracle takes ~9 s per frame of it, so the
-per-function parity tests run at smaller sizes.
+truncation, 2^20 x 10 hash entries, 2^19 blocks).  Its first two frames are
+compared with the FP64 oracle stage by stage (the oracle takes ~20 s per
+frame of it); a longer stretch of the sequence is checked through
+properties that hold at any size:
 
 * The per-frame chain replayed as one HIP graph (a one-scene batch, kernels
   over the device frame table, integration tiling tuned on the first frame)
@@ -98,4 +99,87 @@ def test_c2_graph_equals_eager_and_invariants(hip_device, oracle_mod):
           f"p95 {1000 * np.percentile(err, 95):.2f} mm over {both.sum()} px", flush=True)
     assert np.median(err) < cfg.voxel_size
     assert (err < 0.01).mean() > 0.8  # the rest: silhouettes seen from other frames
+    vol.close()
+
+
+def test_c2_two_frames_match_the_oracle(hip_device, oracle_mod):
+    """The first two frames of the bench sequence at full size through the
+    HIP path and the FP64 oracle: block set, canonical hash layout and
+    worklist bit-exact; SDF / weight within 1e-5 relative (colour exact);
+    marching cubes in lockstep from identical SDF (cube indices, triangles,
+    vertex keys, reference counts and positions bit-exact, normals 1e-4);
+    the 640 x 480 raycast from identical SDF (hit mask equal, depth within
+    1e-4 m, normals 1e-4).  The oracle spends ~45 s here."""
+    import torch
+    from meshhashing_amd.engine import Volume, synth_pose
+
+    cfg = C2
+    vol = Volume(cfg)
+    orc = oracle_mod.Oracle(cfg)
+
+    def canon(rec):
+        out = {}
+        for r in rec:
+            out.setdefault(int(r[1]), []).append((tuple(r[3:]), int(r[2])))
+        return {k: (sorted(p for p, _ in v), sum(f for _, f in v)) for k, v in out.items()}
+
+    for f in (0, 1):
+        print(f"[c2-oracle] frame {f}", flush=True)
+        depth, pose = synth_depth(cfg, f), synth_pose(cfg, f)
+        vol.upload_depth(depth, pose)
+        vol.alloc()
+        n = vol.collect_visible(sync=True)
+        vol.integrate_tsdf()
+        orc.set_frame(depth, pose)
+        orc.alloc()
+        m = orc.collect_visible()
+        orc.integrate_tsdf()
+        g, o = vol.dump_blocks(), orc.dump_blocks()
+        assert np.array_equal(g["pos"], o["pos"]), f"frame {f}: block sets differ"
+        assert n == m and np.array_equal(vol.dump_candidates(), orc.dump_candidates())
+        assert canon(vol.dump_hash()) == canon(orc.dump_hash())
+        obs = o["weight"] > 0
+        assert np.array_equal(g["weight"] > 0, obs)
+        e_sdf = np.abs(g["sdf"][obs] - o["sdf"][obs]) / np.maximum(np.abs(o["sdf"][obs]), cfg.truncation)
+        e_w = np.abs(g["weight"][obs] - o["weight"][obs]) / np.maximum(np.abs(o["weight"][obs]), 1.0)
+        print(f"[c2-oracle] frame {f}: {len(g['pos'])} blocks, {obs.sum()} observed voxels, "
+              f"sdf {e_sdf.max():.2e} weight {e_w.max():.2e}", flush=True)
+        assert e_sdf.max() < 1e-5 and e_w.max() < 1e-5
+        assert np.array_equal(g["color"], o["color"])
+        # marching cubes in lockstep from the GPU's field
+        orc.load_blocks(g["pos"], g["sdf"], g["weight"], g["color"])
+        vol.marching_cubes()
+        orc.marching_cubes()
+        assert np.array_equal(vol.dump_cube_indices(), orc.dump_blocks()["cube"]), f"frame {f}: cubes"
+        mg, mo = vol.dump_mesh(), orc.dump_mesh()
+        assert len(mg["tri"]) > 100_000
+        for k in ("tri_cube", "tri", "vkey", "vref"):
+            assert np.array_equal(mg[k], mo[k]), f"frame {f}: {k}"
+        assert np.array_equal(mg["vval"][:, :3], mo["vval"][:, :3].astype(np.float32))
+        dn = np.abs(mg["vval"][:, 3:6] - mo["vval"][:, 3:6]).max(axis=1)
+        bad = np.nonzero(dn >= 1e-4)[0]
+        for i in bad:  # only where the SDF gradient is at the FP32 floor
+            gr, _ = orc.spatial_gradient(mg["vval"][i, :3])
+            assert np.linalg.norm(gr) < 1e-3, (mg["vkey"][i], gr)
+        assert len(bad) <= 1e-4 * len(dn) + 2
+        print(f"[c2-oracle] frame {f}: {len(mg['tri'])} triangles, normal err "
+              f"{np.delete(dn, bad).max():.2e}, ill-conditioned {len(bad)}", flush=True)
+    # raycast of the full image at the second frame's pose, identical SDF
+    cTw = np.linalg.inv(synth_pose(cfg, 1).astype(np.float64)).astype(np.float32)
+    npx = cfg.width * cfg.height
+    d = torch.empty(npx, dtype=torch.float32, device=hip_device)
+    nn = torch.empty(3 * npx, dtype=torch.float32, device=hip_device)
+    vol.raycast(cTw, d.data_ptr(), 0, nn.data_ptr())
+    vol.synchronize()
+    dg = d.cpu().numpy().reshape(cfg.height, cfg.width)
+    ng = nn.cpu().numpy().reshape(cfg.height, cfg.width, 3)
+    do, _, no = orc.raycast(cTw)
+    hit = np.isfinite(do)
+    assert np.array_equal(np.isfinite(dg), hit)
+    assert hit.mean() > 0.5
+    err = np.abs(dg[hit] - do[hit])
+    print(f"[c2-oracle] raycast: {hit.sum()} hits, max |depth err| {err.max():.2e}, "
+          f"bit-exact {np.mean(err == 0):.4f}", flush=True)
+    assert err.max() <= 1e-4
+    assert np.abs(ng[hit] - no[hit]).max() < 1e-4
     vol.close()

@@ -276,16 +276,30 @@ def test_psdf_and_primal_dual(hip_device, oracle_mod):
         assert np.array_equal(g["weight"] > 0, obs)
         for k, s in (("sdf", cfg.truncation), ("weight", 1.0), ("a", 1.0), ("b", 1.0)):
             e = rel_err(g[k][obs], o[k][obs], s)
-            errs[k] = max(errs.get(k, 0), float(np.quantile(e, 0.999)))
-            errs[k + "_max"] = max(errs.get(k + "_max", 0), float(e.max()))
+            errs[k] = max(errs.get(k, 0), float(e.max()))
         vol.marching_cubes()
         orc.load_blocks(g["pos"], g["sdf"], g["weight"], g["color"], g["a"], g["b"])
         orc.marching_cubes()
-        assert np.array_equal(vol.dump_mesh()["tri"], orc.dump_mesh()["tri"])
-    print("[psdf] errors", errs)
+        mg, mo = vol.dump_mesh(), orc.dump_mesh()
+        assert np.array_equal(mg["tri"], mo["tri"])
+        assert np.array_equal(mg["vkey"], mo["vkey"])
+        # vertex payload (marching_cubes.cu:59-69): radius = sqrt(1 / inv_sigma2)
+        # of the interpolated voxel, colour = ValToRGB(a / (a + b), 0.4, 1)
+        rg, ro = mg["vval"][:, 9].astype(np.float64), mo["vval"][:, 9]
+        fin = np.isfinite(ro)
+        assert np.array_equal(np.isfinite(rg), fin)
+        cg, co = mg["vval"][:, 6:9].astype(np.float64), mo["vval"][:, 6:9]
+        assert np.isfinite(co).all() and np.isfinite(cg).all()
+        errs["vertices"] = errs.get("vertices", 0) + len(ro)
+        if fin.any():
+            errs["radius"] = max(errs.get("radius", 0), float(rel_err(rg[fin], ro[fin], 1e-3).max()))
+            errs["colour"] = max(errs.get("colour", 0), float(np.abs(cg - co).max()))
+    print("[psdf] errors (max over frames)", errs)
     assert errs["inlier"] < 1e-5
-    for k in ("sdf", "weight", "a", "b"):
+    for k in ("sdf", "weight", "a", "b", "radius"):
         assert errs[k] < 1e-5, (k, errs)
+    assert errs["colour"] < 1e-5, errs
+    assert errs["vertices"] > 1000, errs
     # primal-dual over all blocks from identical state
     g = vol.dump_blocks()
     orc.load_blocks(g["pos"], g["sdf"], g["weight"], g["color"], g["a"], g["b"])
