@@ -22,6 +22,9 @@
 #include <map>
 #include <unordered_map>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "../include/meshhashing.h"
 
@@ -234,6 +237,11 @@ struct TKey {
 };
 
 }  // namespace
+
+// Threads of the stages that are parallel over blocks or pixels (integrate,
+// marching cubes pass 1, raycast); results do not depend on it.  Allocation,
+// triangle bookkeeping and vertex recycling stay serial.
+static int g_threads = 1;
 
 struct orc_volume {
   mh_hash_params hp;
@@ -656,8 +664,10 @@ uint32_t orc_collect(orc_volume* v, int frustum) {
 int orc_integrate_tsdf(orc_volume* v) {
   const Geo& g = v->g;
   const Cam& c = v->cam;
-  for (int ptr : v->cand) {
-    Block& B = v->blocks[ptr];
+  uint64_t updates = 0;
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : updates) num_threads(g_threads)
+  for (size_t ci = 0; ci < v->cand.size(); ++ci) {
+    Block& B = v->blocks[v->cand[ci]];
     for (int i = 0; i < 512; ++i) {
       const I3 vp = voxel_of(B, i);
       uint32_t u, vv;
@@ -676,7 +686,7 @@ int orc_integrate_tsdf(orc_volume* v) {
       const double w = std::max(10.0 * (double)g.ws * (1.0 - nd), 1.0);
       B.sdf[i] = (B.sdf[i] * B.w[i] + s * w) / (B.w[i] + w);
       B.w[i] += w;
-      ++v->voxel_updates;
+      ++updates;
       uint8_t dcol[3] = {0, 255, 0};  // no colour image (update_simple.cu:71-73)
       if (v->has_color) {
         const uint32_t cin = v->color_in[vv * c.w + u];
@@ -690,6 +700,7 @@ int orc_integrate_tsdf(orc_volume* v) {
       }
     }
   }
+  v->voxel_updates += updates;
   return 0;
 }
 
@@ -837,8 +848,18 @@ int orc_marching_cubes(orc_volume* v, const mh_mc_params* mp) {
     *b = (float)B.b[li];
     return true;
   };
-  // pass 1
-  for (int ptr : v->cand) {
+  // pass 1, parallel over the worklist: each block writes only its own cube
+  // words; vertices go to per-thread lists and are merged afterwards (every
+  // writer of a vertex computes the same values, so the merge order is free)
+  std::vector<std::vector<std::pair<VKey, Vtx>>> found((size_t)std::max(g_threads, 1));
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads)
+  for (size_t ci = 0; ci < v->cand.size(); ++ci) {
+    const int ptr = v->cand[ci];
+#ifdef _OPENMP
+    auto& mine = found[(size_t)omp_get_thread_num()];
+#else
+    auto& mine = found[0];
+#endif
     for (int i = 0; i < 512; ++i) {
       Block& B = v->blocks[ptr];
       const I3 vp = voxel_of(B, i);
@@ -883,7 +904,7 @@ int orc_marching_cubes(orc_volume* v, const mh_mc_params* mp) {
           const float mu = (0.0f - vA) / (vB - vA);
           p = F3{pA.x + mu * (pB.x - pA.x), pA.y + mu * (pB.y - pA.y), pA.z + mu * (pB.z - pA.z)};
         }
-        Vtx& vt = v->verts[VKey{{o.x, o.y, o.z, ax}}];
+        Vtx vt;
         vt.pos[0] = p.x;
         vt.pos[1] = p.y;
         vt.pos[2] = p.z;
@@ -914,9 +935,18 @@ int orc_marching_cubes(orc_volume* v, const mh_mc_params* mp) {
         const double gr[3] = {(sp[0] - sn[0]) / g.vs, (sp[1] - sn[1]) / g.vs, (sp[2] - sn[2]) / g.vs};
         const double l = std::sqrt(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
         for (int k = 0; k < 3; ++k) vt.normal[k] = (l > 0.0 && ok) ? gr[k] / l : 0.0;
+        mine.emplace_back(VKey{{o.x, o.y, o.z, ax}}, vt);
       }
     }
   }
+  for (const auto& list : found)
+    for (const auto& kv : list) {
+      Vtx& vt = v->verts[kv.first];  // keeps the reference count of an existing vertex
+      std::memcpy(vt.pos, kv.second.pos, sizeof(vt.pos));
+      std::memcpy(vt.normal, kv.second.normal, sizeof(vt.normal));
+      std::memcpy(vt.color, kv.second.color, sizeof(vt.color));
+      vt.radius = kv.second.radius;
+    }
   // pass 2 (TriangleExtractionKernel, :180-275)
   for (int ptr : v->cand) {
     for (int i = 0; i < 512; ++i) {
@@ -1064,6 +1094,7 @@ int orc_raycast(orc_volume* v, const mh_raycaster_params* rp, const float* cTw, 
   inv4<float>(cTw, wTc);
   const Cam c{rp->fx, rp->fy, rp->cx, rp->cy, rp->width, rp->height, rp->min_raycast_depth,
               rp->max_raycast_depth};
+#pragma omp parallel for schedule(dynamic, 2) num_threads(g_threads)
   for (uint32_t y = 0; y < rp->height; ++y) {
     for (uint32_t x = 0; x < rp->width; ++x) {
       const uint32_t pix = y * rp->width + x;
@@ -1690,6 +1721,8 @@ int orc_spatial_gradient(const orc_volume* v, const float* pos, double* grad) {
 
 uint32_t orc_dropped(const orc_volume* v) { return v->dropped; }
 uint64_t orc_voxel_updates(const orc_volume* v) { return v->voxel_updates; }
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+int orc_get_threads(void) { return g_threads; }
 uint32_t orc_list_inserts(const orc_volume* v) { return v->list_inserts; }
 
 // MC table access for the table tests.

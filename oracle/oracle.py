@@ -87,6 +87,8 @@ def lib() -> C.CDLL:
             "orc_recycle_garbage": (C.c_uint32, [vp]),
             "orc_dump_cand_flags": (C.c_uint32, [vp, P(C.c_int32)]),
             "orc_val_to_rgb": (None, [C.c_double, C.c_double, C.c_double, P(C.c_double)]),
+            "orc_set_threads": (None, [C.c_int]),
+            "orc_get_threads": (C.c_int, []),
             "orc_set_vertex": (C.c_int, [vp, P(C.c_int32), P(C.c_float), P(C.c_double),
                                          C.c_double]),
         }
@@ -350,6 +352,12 @@ class Oracle:
 
     def perturb_sdf(self, block: int, voxel: int, delta: float):
         assert self.L.orc_perturb_sdf(self.h, block, voxel, delta) == 0
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's block- and pixel-parallel stages (results do
+    not depend on it)."""
+    lib().orc_set_threads(int(n))
 
 
 def val_to_rgb(v: float, vmin: float, vmax: float) -> np.ndarray:
