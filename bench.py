@@ -465,8 +465,10 @@ def main():
         rows = {}
         for s in ("integrate", "mesh", "raycast"):
             b = algorithmic_bytes(s, cnt, K, W, H)
-            rows[s] = {"ms_per_frame": stage_ms[s] / K, "GB/s": b / (stage_ms[s] * 1e-3) / 1e9,
-                       "frac": b / (stage_ms[s] * 1e-3) / HBM_PEAK}
+            rows[s] = {"ms_per_frame": stage_ms[s] / K,
+                       "GB/s": None if b is None else b / (stage_ms[s] * 1e-3) / 1e9,
+                       "frac": None if b is None else b / (stage_ms[s] * 1e-3) / HBM_PEAK}
+        dom = max((s for s in rows if rows[s]["GB/s"] is not None), key=lambda s: stage_ms[s])
         b = algorithmic_bytes(dom, cnt, K, W, H)
         ach = b / (stage_ms[dom] * 1e-3)
         tr = measured_traffic(dom, None if tune is None else (tune.workgroup_size,
