@@ -377,6 +377,20 @@ typedef struct mh_raycast_grad {
 int mh_raycast_backward(mh_volume* v, const mh_raycaster_params* p,
                         const float cTw[16], const float* obs_depth_dev,
                         float* grad_sdf_dev, mh_raycast_grad* out);
+/* Allocation-free form of mh_raycast_backward (no counterpart in the
+ * reference).  For callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc:
+ * it calls no hipMalloc, hipFree or allocating library routine.  Its
+ * scratch (tile ranges, per-workgroup partial sums, the result row) is one
+ * static device array of ~5.1 MB per GPU, held until the process ends, so
+ * images are limited to 1280 x 1024 (larger: MH_ERR_CAPACITY).  The
+ * volume's device must be current.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race;
+ * serialise them.  Same arguments, results and synchronisation as
+ * mh_raycast_backward.                                                     */
+int mh_raycast_backward_noalloc(mh_volume* v, const mh_raycaster_params* p,
+                                const float cTw[16], const float* obs_depth_dev,
+                                float* grad_sdf_dev, mh_raycast_grad* out);
 /* Pose refinement: Levenberg-damped Gauss-Newton over xi, iterations steps
  * from wTc_init, on a truncated quadratic: pixels with |depth - obs| above
  * a gate cost gate^2 / 2 each (no gate for the first 40 % of the steps,

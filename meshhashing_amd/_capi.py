@@ -174,6 +174,8 @@ SIGNATURES = {
     "mh_localize": (C.c_int, [VP, C.c_uint32, C.c_float, C.c_int, P_F32, P_F64, P_U32, P_U32]),
     "mh_raycast_backward": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, VP, VP,
                                       C.POINTER(RaycastGrad)]),
+    "mh_raycast_backward_noalloc": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, VP, VP,
+                                              C.POINTER(RaycastGrad)]),
     "mh_refine_pose": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, VP, C.c_uint32, P_F32,
                                  P_F64]),
     "mh_dump_voxel_field": (C.c_int, [VP, VP, C.c_uint32, P_F32, P_U32]),

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -19,6 +20,7 @@ static constexpr McTables kHostMc = build_mc_tables();
 namespace {
 
 thread_local std::string g_err;
+int scratch_address(int dev, unsigned char** out);
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -325,6 +327,15 @@ int mh_volume_create(const mh_hash_params* hash, const mh_volume_params* vol,
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, v->device) == hipSuccess &&
       cus > 0)
     v->cus = cus;
+  {
+    // set-up of the allocation-free adjoint: this device's scratch address
+    unsigned char* sa = nullptr;
+    const int rc = scratch_address(v->device, &sa);
+    if (rc) {
+      delete v;
+      return rc;
+    }
+  }
 
   View& V = v->V;
   const size_t cap = hash->value_capacity;
@@ -1484,21 +1495,15 @@ hipError_t ensure_partials(mh_volume* v, uint32_t rows) {
   return hipSuccess;
 }
 
-int raycast_grad(mh_volume* v, const mh_raycaster_params* p, const float cTw[16], const float* obs,
-                 float* grad_sdf, mh_raycast_grad* out, float max_residual = INFINITY,
-                 float max_slope = INFINITY) {
-  RayConst R;
-  int rc = build_ray(v, p, cTw, &R);
-  if (rc) return rc;
-  rc = attach_range(v, &R);
-  if (rc) return rc;
-  const uint32_t n = raycast_grad_partials(R);
-  HIP_TRY(ensure_partials(v, n));
-  launch_raycast_backward(v->V, R, obs, grad_sdf, v->rgrad_partials, v->rgrad_out, max_residual,
-                          max_slope, v->stream);
+// The adjoint from a built RayConst (its range buffer attached) into the
+// given partial-sum rows and result row; synchronises the stream.
+int raycast_grad_run(mh_volume* v, const RayConst& R, const float* obs, float* grad_sdf,
+                     double* partials, double* res, mh_raycast_grad* out, float max_residual,
+                     float max_slope) {
+  launch_raycast_backward(v->V, R, obs, grad_sdf, partials, res, max_residual, max_slope, v->stream);
   HIP_TRY(hipGetLastError());
   double h[30];
-  HIP_TRY(hipMemcpyAsync(h, v->rgrad_out, sizeof(h), hipMemcpyDeviceToHost, v->stream));
+  HIP_TRY(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, v->stream));
   HIP_TRY(hipStreamSynchronize(v->stream));
   out->loss = h[0];
   for (int k = 0; k < 6; ++k) out->grad[k] = h[1 + k];
@@ -1511,6 +1516,64 @@ int raycast_grad(mh_volume* v, const mh_raycaster_params* p, const float cTw[16]
     }
   out->pixels = (uint32_t)h[28];
   out->outliers = (uint32_t)h[29];
+  return MH_OK;
+}
+
+int raycast_grad(mh_volume* v, const mh_raycaster_params* p, const float cTw[16], const float* obs,
+                 float* grad_sdf, mh_raycast_grad* out, float max_residual = INFINITY,
+                 float max_slope = INFINITY) {
+  RayConst R;
+  int rc = build_ray(v, p, cTw, &R);
+  if (rc) return rc;
+  rc = attach_range(v, &R);
+  if (rc) return rc;
+  const uint32_t n = raycast_grad_partials(R);
+  HIP_TRY(ensure_partials(v, n));
+  return raycast_grad_run(v, R, obs, grad_sdf, v->rgrad_partials, v->rgrad_out, out, max_residual,
+                          max_slope);
+}
+
+// ---------------------------------------------------- allocation-free adjoint
+// mh_raycast_backward_noalloc keeps its scratch -- the tile-range words of
+// the range pass, one FP64 partial-sum row per 8 x 8-pixel workgroup and the
+// result row -- in one file-scope device array instead of buffers allocated
+// on first use.  Each GPU holds its own copy of the array for the life of
+// the process; its device address is looked up once per device
+// (hipGetSymbolAddress) and only that address is handed to kernels and
+// copies.  Images up to kNoAllocMaxWidth x kNoAllocMaxHeight fit.
+constexpr uint32_t kNoAllocMaxWidth = 1280, kNoAllocMaxHeight = 1024;
+constexpr uint32_t kNoAllocTiles = ((kNoAllocMaxWidth + 7) / 8) * ((kNoAllocMaxHeight + 7) / 8);
+constexpr size_t kNoAllocRound = 256;
+constexpr size_t round_up(size_t x) { return (x + kNoAllocRound - 1) / kNoAllocRound * kNoAllocRound; }
+constexpr size_t kNoAllocPartialOff = 0;
+constexpr size_t kNoAllocPartialBytes = round_up((size_t)kNoAllocTiles * 30 * sizeof(double));
+constexpr size_t kNoAllocResultOff = kNoAllocPartialOff + kNoAllocPartialBytes;
+constexpr size_t kNoAllocResultBytes = round_up(30 * sizeof(double));
+constexpr size_t kNoAllocRangeOff = kNoAllocResultOff + kNoAllocResultBytes;
+constexpr size_t kNoAllocRangeBytes = round_up((2 * (size_t)kNoAllocTiles + 1) * sizeof(uint32_t));
+constexpr size_t kNoAllocScratchBytes = kNoAllocRangeOff + kNoAllocRangeBytes;  // ~5.1 MB
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_addr[kMaxDevices] = {};
+
+// The device address of `scratch` on device `dev`, which must be current.
+int scratch_address(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(MH_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(MH_ERR_HIP, "hipGetSymbolAddress returned no address");
+    g_scratch_addr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch_addr[dev];
   return MH_OK;
 }
 
@@ -1626,6 +1689,28 @@ int mh_raycast_backward(mh_volume* v, const mh_raycaster_params* p, const float 
                         const float* obs_depth_dev, float* grad_sdf_dev, mh_raycast_grad* out) {
   if (check_vol(v) || !p || !cTw || !obs_depth_dev || !out) return fail(MH_ERR_ARG, "null argument");
   return raycast_grad(v, p, cTw, obs_depth_dev, grad_sdf_dev, out);
+}
+
+int mh_raycast_backward_noalloc(mh_volume* v, const mh_raycaster_params* p, const float cTw[16],
+                                const float* obs_depth_dev, float* grad_sdf_dev,
+                                mh_raycast_grad* out) {
+  if (check_vol(v) || !p || !cTw || !obs_depth_dev || !out) return fail(MH_ERR_ARG, "null argument");
+  if (p->width > kNoAllocMaxWidth || p->height > kNoAllocMaxHeight)
+    return fail(MH_ERR_CAPACITY, "image larger than the allocation-free scratch (1280 x 1024)");
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != v->device) return fail(MH_ERR_STATE, "the volume's device must be current");
+  unsigned char* base = nullptr;
+  int rc = scratch_address(dev, &base);
+  if (rc) return rc;
+  RayConst R;
+  rc = build_ray(v, p, cTw, &R);
+  if (rc) return rc;
+  R.range = reinterpret_cast<uint32_t*>(base + kNoAllocRangeOff);  // cleared by the range pass
+  return raycast_grad_run(v, R, obs_depth_dev, grad_sdf_dev,
+                          reinterpret_cast<double*>(base + kNoAllocPartialOff),
+                          reinterpret_cast<double*>(base + kNoAllocResultOff), out, INFINITY,
+                          INFINITY);
 }
 
 int mh_refine_pose(mh_volume* v, const mh_raycaster_params* p, const float wTc_init[16],

@@ -175,14 +175,16 @@ class Volume:
                 "mh_raycast")
 
     def raycast_backward(self, cTw: np.ndarray, obs_ptr: int, grad_sdf_ptr: int = 0,
-                         params: A.RayCasterParams | None = None) -> dict:
+                         params: A.RayCasterParams | None = None, noalloc: bool = False) -> dict:
         """Adjoint of the raycast depth (mh_raycast_backward): loss, dL/dxi,
-        Gauss-Newton J^T J and pixel count; dL/dsdf added to grad_sdf_ptr."""
+        Gauss-Newton J^T J and pixel count; dL/dsdf added to grad_sdf_ptr.
+        noalloc: the allocation-free entry point (static scratch)."""
         p = params if params is not None else self.cfg.raycaster_params()
         g = A.RaycastGrad()
-        A.check(self.lib.mh_raycast_backward(self.h, C.byref(p), A.f32x16(cTw), C.c_void_p(obs_ptr),
-                                             C.c_void_p(grad_sdf_ptr or None), C.byref(g)),
-                "mh_raycast_backward")
+        fn = self.lib.mh_raycast_backward_noalloc if noalloc else self.lib.mh_raycast_backward
+        A.check(fn(self.h, C.byref(p), A.f32x16(cTw), C.c_void_p(obs_ptr),
+                   C.c_void_p(grad_sdf_ptr or None), C.byref(g)),
+                "mh_raycast_backward_noalloc" if noalloc else "mh_raycast_backward")
         return {"loss": g.loss, "grad": np.array(g.grad[:]),
                 "hessian": np.array(g.hessian[:]).reshape(6, 6), "pixels": g.pixels,
                 "outliers": g.outliers}
