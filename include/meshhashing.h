@@ -122,6 +122,8 @@ typedef struct mh_status {
   uint64_t mc_triangle_writes; /* triangles (re)assigned by MC              */
   uint64_t mc_blocks;          /* worklist blocks meshed                    */
   uint64_t ray_samples;        /* valid trilinear samples of the raycaster  */
+  uint64_t ray_blocks;         /* distinct blocks sampled, summed over
+                                  raycasts run with MH_DIAG_RAY_BLOCKS      */
 } mh_status;
 
 typedef struct mh_volume mh_volume;
@@ -141,6 +143,24 @@ int mh_volume_reset(mh_volume* v);
 int mh_volume_set_stream(mh_volume* v, void* hip_stream);
 int mh_synchronize(mh_volume* v);
 int mh_get_status(mh_volume* v, mh_status* out);
+/* Diagnostics, off by default; neither changes a result.  The reference
+ * times its stages with host timers (util/timer.h, written out by
+ * engine/logging_engine.cc:61-93); here:
+ * MH_DIAG_KERNEL_TIMES: each kernel the volume's stage entry points launch
+ *   is bracketed by HIP events on the volume's stream; mh_kernel_times sums
+ *   them per kernel name (launches, total ms) and synchronises.
+ * MH_DIAG_RAY_BLOCKS: mh_raycast records which blocks its lookups find and
+ *   adds their number to mh_status.ray_blocks (one more kernel per call). */
+#define MH_DIAG_KERNEL_TIMES 1u
+#define MH_DIAG_RAY_BLOCKS 2u
+int mh_volume_set_diagnostics(mh_volume* v, uint32_t flags);
+typedef struct mh_kernel_time {
+  char name[48];
+  uint64_t launches;
+  double total_ms;
+} mh_kernel_time;
+int mh_kernel_times(mh_volume* v, uint32_t capacity, mh_kernel_time* out,
+                    uint32_t* count, int reset);
 void mh_mc_params_default(mh_mc_params* out);
 const char* mh_last_error(void);
 
@@ -185,6 +205,18 @@ int mh_predict_inlier(mh_volume* v);
 int mh_integrate_psdf(mh_volume* v);
 /* Inlier ratio image of the current frame (device, W*H floats). */
 int mh_inlier_ratio_device_ptr(mh_volume* v, float** inlier_dev);
+/* Device storage of a voxel field: value_capacity * 512 values, block-pool
+ * index major, voxel x + 8 y + 64 z within a block (the BlockArray's device
+ * memory, core/block_array.h:14-28).  For interop, e.g. viewing the SDF as a
+ * framework tensor; writes through it change the volume.  count (may be
+ * NULL) receives the number of values.  PSDF fields need enable_psdf.     */
+#define MH_FIELD_SDF 0     /* float */
+#define MH_FIELD_WEIGHT 1  /* float: weight (TSDF) or inv_sigma2 (PSDF) */
+#define MH_FIELD_COLOR 2   /* uint32: R | G << 8 | B << 16 */
+#define MH_FIELD_BETA_A 3  /* float */
+#define MH_FIELD_BETA_B 4  /* float */
+int mh_volume_field_device_ptr(mh_volume* v, int field, void** field_dev,
+                               uint64_t* count);
 
 /* ---- block recycling: mapping/recycle.h:11-35 ---------------------------
  * Each works on the current worklist, as the reference (MainEngine::Meshing
@@ -373,6 +405,8 @@ typedef struct mh_raycast_grad {
   double hessian[36];   /* sum J^T J (Gauss-Newton), J = d depth / d xi      */
   uint32_t pixels;      /* pixels in the sum                                 */
   uint32_t outliers;    /* hit + observation, left out by a residual gate   */
+  uint32_t observed;    /* pixels with a finite positive observation, hit or
+                           not (the pose refinement charges the lost ones)   */
 } mh_raycast_grad;
 int mh_raycast_backward(mh_volume* v, const mh_raycaster_params* p,
                         const float cTw[16], const float* obs_depth_dev,
@@ -382,7 +416,7 @@ int mh_raycast_backward(mh_volume* v, const mh_raycaster_params* p,
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc:
  * it calls no hipMalloc, hipFree or allocating library routine.  Its
  * scratch (tile ranges, per-workgroup partial sums, the result row) is one
- * static device array of ~5.1 MB per GPU, held until the process ends, so
+ * static device array of ~5.2 MB per GPU, held until the process ends, so
  * images are limited to 1280 x 1024 (larger: MH_ERR_CAPACITY).  The
  * volume's device must be current.  Every call on a GPU shares that GPU's
  * array: calls that overlap on one GPU (other streams, other threads) race;
@@ -392,13 +426,15 @@ int mh_raycast_backward_noalloc(mh_volume* v, const mh_raycaster_params* p,
                                 const float cTw[16], const float* obs_depth_dev,
                                 float* grad_sdf_dev, mh_raycast_grad* out);
 /* Pose refinement: Levenberg-damped Gauss-Newton over xi, iterations steps
- * from wTc_init, on a truncated quadratic: pixels with |depth - obs| above
- * a gate cost gate^2 / 2 each (no gate for the first 40 % of the steps,
- * then 0.10 m, then 0.05 m for the last 30 %); a step is kept only if it
- * lowers the cost per pixel.  Rays close to grazing the surface (depth
- * moving more than 3x a camera translation along some axis) are skipped.  cost_trace (may be NULL) receives the cost
- * per pixel before the first and after every step (iterations + 1 values,
- * each at that step's gate).                                               */
+ * from wTc_init, on a truncated quadratic over every observed pixel: r^2 / 2
+ * for a pixel within the residual gate, gate^2 / 2 for one gated out or
+ * lost (no hit; a ray close to grazing the surface, its depth moving more
+ * than 3x a camera translation along some axis).  The gate is 0.20 m for the
+ * first 40 % of the steps, then 0.10 m, then 0.05 m for the last 30 %; a
+ * step is kept only if it lowers the cost.  The result is wTc_init if that
+ * costs less at the final gate.  cost_trace (may be NULL) receives the cost
+ * per observed pixel before the first and after every step (iterations + 1
+ * values, each at that step's gate).                                       */
 int mh_refine_pose(mh_volume* v, const mh_raycaster_params* p,
                    const float wTc_init[16], const float* obs_depth_dev,
                    uint32_t iterations, float wTc_out[16], double* cost_trace);

@@ -75,12 +75,12 @@ class Status(C.Structure):
                 ("vertex_count", C.c_uint32), ("triangle_count", C.c_uint32),
                 ("voxel_updates", C.c_uint64), ("blocks_integrated", C.c_uint64),
                 ("mc_vertex_writes", C.c_uint64), ("mc_triangle_writes", C.c_uint64),
-                ("mc_blocks", C.c_uint64), ("ray_samples", C.c_uint64)]
+                ("mc_blocks", C.c_uint64), ("ray_samples", C.c_uint64), ("ray_blocks", C.c_uint64)]
 
 
 class RaycastGrad(C.Structure):
     _fields_ = [("loss", C.c_double), ("grad", C.c_double * 6), ("hessian", C.c_double * 36),
-                ("pixels", C.c_uint32), ("outliers", C.c_uint32)]
+                ("pixels", C.c_uint32), ("outliers", C.c_uint32), ("observed", C.c_uint32)]
 
 
 class PointToPsdf(C.Structure):
@@ -101,6 +101,15 @@ class TuneResult(C.Structure):
 
 MH_NUM_TILINGS = 18
 MH_NUM_TUNE_GRIDS = 5
+MH_FIELD_SDF, MH_FIELD_WEIGHT, MH_FIELD_COLOR, MH_FIELD_BETA_A, MH_FIELD_BETA_B = range(5)
+MH_DIAG_KERNEL_TIMES = 1
+MH_DIAG_RAY_BLOCKS = 2
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64), ("total_ms", C.c_double)]
+
+
 MH_RUN_MESH = 1 << 0
 MH_RUN_RAYCAST = 1 << 1
 MH_RUN_GRAPH = 1 << 2
@@ -172,6 +181,10 @@ SIGNATURES = {
     "mh_save_ply": (C.c_int, [VP, C.c_char_p]),
     "mh_point_to_psdf": (C.c_int, [VP, P_F32, C.c_int, C.POINTER(PointToPsdf)]),
     "mh_localize": (C.c_int, [VP, C.c_uint32, C.c_float, C.c_int, P_F32, P_F64, P_U32, P_U32]),
+    "mh_volume_set_diagnostics": (C.c_int, [VP, C.c_uint32]),
+    "mh_volume_field_device_ptr": (C.c_int, [VP, C.c_int, C.POINTER(VP), C.POINTER(C.c_uint64)]),
+    "mh_kernel_times": (C.c_int, [VP, C.c_uint32, C.POINTER(KernelTime), C.POINTER(C.c_uint32),
+                                  C.c_int]),
     "mh_raycast_backward": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, VP, VP,
                                       C.POINTER(RaycastGrad)]),
     "mh_raycast_backward_noalloc": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, VP, VP,

@@ -100,6 +100,15 @@ C5 = replace(C1, name="c5", width=640, height=480, fx=525.0, fy=525.0, cx=319.5,
              voxel_size=0.005, truncation=0.02, bucket_count=1 << 20, value_capacity=1 << 19,
              max_vertices=1 << 20, max_triangles=1 << 21)
 
+# Coarse copies of C5's scene (same frames, 4 cm and 2 cm voxels, truncation
+# four voxels) for coarse-to-fine pose refinement, and the Gauss-Newton steps
+# per level (4 cm, 2 cm, 5 mm): ten in all.
+C5_PYRAMID = (replace(C5, name="c5_4cm", voxel_size=0.04, truncation=0.16, bucket_count=1 << 16,
+                      value_capacity=1 << 13),
+              replace(C5, name="c5_2cm", voxel_size=0.02, truncation=0.08, bucket_count=1 << 17,
+                      value_capacity=1 << 15))
+C5_PYRAMID_STEPS = (3, 3, 4)
+
 CONFIGS = {c.name: c for c in (C1, C2, C3, C5)}
 
 # Batched form: independent rooms of 320x240 frames, voxel sizes drawn from

@@ -147,15 +147,80 @@ struct mh_volume {
   int tiling = kDefaultIntegrateTiling;  // integration tiling (kIntegrateTilings)
   int grid_per_cu = 8;                   // integration workgroups per CU
   std::vector<void*> owned;
+  // diagnostics (mh_volume_set_diagnostics)
+  uint32_t diag = 0;
+  struct Interval {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> ev_pool, ev_used;
+  std::vector<Interval> intervals;
+  const char* open_name = nullptr;
+  hipEvent_t open_ev = nullptr;
+  std::vector<std::string> kt_names;
+  std::vector<uint64_t> kt_launches;
+  std::vector<double> kt_ms;
+  KernelClock clock{};
+  uint32_t* ray_mark = nullptr;  // per block (MH_DIAG_RAY_BLOCKS)
+  int32_t* mc_nbt = nullptr;     // marching cubes: 27 neighbours per worklist item
+  uint32_t ray_stamp = 0;
 
   int grid_wg256() const { return cus * 8; }
   int grid_wg512() const { return cus * 4; }
+  const KernelClock* kc() const { return (diag & MH_DIAG_KERNEL_TIMES) ? &clock : nullptr; }
 };
 
 namespace {
 
 int check_vol(mh_volume* v) {
   if (!v) return fail(MH_ERR_ARG, "null volume");
+  return MH_OK;
+}
+
+// Per-kernel timing: an event at every tick; the interval from one tick to
+// the next is the named kernel's.  Event creation failures drop the
+// interval (timing is best effort and changes no result).
+void kernel_tick(void* ctx, hipStream_t s, const char* next) {
+  mh_volume* v = static_cast<mh_volume*>(ctx);
+  hipEvent_t e = nullptr;
+  if (!v->ev_pool.empty()) {
+    e = v->ev_pool.back();
+    v->ev_pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    e = nullptr;
+  }
+  if (e && hipEventRecord(e, s) == hipSuccess) {
+    v->ev_used.push_back(e);
+  } else {
+    if (e) v->ev_pool.push_back(e);
+    e = nullptr;
+  }
+  if (v->open_name && v->open_ev && e) v->intervals.push_back({v->open_name, v->open_ev, e});
+  v->open_name = next;
+  v->open_ev = e;
+}
+
+// Waits for the stream and adds the recorded intervals to the totals.
+int harvest_times(mh_volume* v) {
+  HIP_TRY(hipStreamSynchronize(v->stream));
+  for (const auto& iv : v->intervals) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, iv.a, iv.b) != hipSuccess) continue;
+    size_t k = 0;
+    while (k < v->kt_names.size() && v->kt_names[k] != iv.name) ++k;
+    if (k == v->kt_names.size()) {
+      v->kt_names.push_back(iv.name);
+      v->kt_launches.push_back(0);
+      v->kt_ms.push_back(0.0);
+    }
+    v->kt_launches[k] += 1;
+    v->kt_ms[k] += ms;
+  }
+  v->intervals.clear();
+  if (!v->open_name) {  // no interval open: every used event can be reused
+    v->ev_pool.insert(v->ev_pool.end(), v->ev_used.begin(), v->ev_used.end());
+    v->ev_used.clear();
+  }
   return MH_OK;
 }
 
@@ -337,6 +402,7 @@ int mh_volume_create(const mh_hash_params* hash, const mh_volume_params* vol,
     }
   }
 
+  v->clock = KernelClock{kernel_tick, v};
   View& V = v->V;
   const size_t cap = hash->value_capacity;
   const size_t nvox = cap * kBlockVoxels;
@@ -370,6 +436,7 @@ int mh_volume_create(const mh_hash_params* hash, const mh_volume_params* vol,
   A(&V.vertices, mesh->max_vertex_count);
   A(&V.vertex_heap, mesh->max_vertex_count);
   A(&V.cand, cap);
+  A(&v->mc_nbt, cap * 27);
   A(&V.cand_flag, cap);
   A(&V.life, cap);
   A(&V.gc_list, cap);
@@ -415,6 +482,9 @@ int mh_volume_destroy(mh_volume* v) {
   if (v->rgrad_partials) (void)hipFree(v->rgrad_partials);
   if (v->rgrad_out) (void)hipFree(v->rgrad_out);
   if (v->ray_range) (void)hipFree(v->ray_range);
+  if (v->ray_mark) (void)hipFree(v->ray_mark);
+  for (hipEvent_t e : v->ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : v->ev_used) (void)hipEventDestroy(e);
   delete v;
   return MH_OK;
 }
@@ -426,6 +496,49 @@ int mh_volume_reset(mh_volume* v) {
   HIP_TRY(hipStreamSynchronize(v->stream));
   v->have_frame = false;
   ++v->work_gen;
+  return MH_OK;
+}
+
+int mh_volume_set_diagnostics(mh_volume* v, uint32_t flags) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  if (flags & ~(uint32_t)(MH_DIAG_KERNEL_TIMES | MH_DIAG_RAY_BLOCKS))
+    return fail(MH_ERR_ARG, "unknown diagnostics flag");
+  if ((flags & MH_DIAG_RAY_BLOCKS) && !v->ray_mark) {
+    uint32_t* m = nullptr;
+    HIP_TRY(dalloc(&m, v->hp.value_capacity));
+    const hipError_t e = hipMemsetAsync(m, 0, (size_t)v->hp.value_capacity * 4, v->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(m);
+      return fail(MH_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    }
+    v->ray_mark = m;
+  }
+  if (!(flags & MH_DIAG_KERNEL_TIMES) && v->open_name) kernel_tick(v, v->stream, nullptr);
+  v->diag = flags;
+  return MH_OK;
+}
+
+int mh_kernel_times(mh_volume* v, uint32_t capacity, mh_kernel_time* out, uint32_t* count,
+                    int reset) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  const int rc = harvest_times(v);
+  if (rc) return rc;
+  const uint32_t n = (uint32_t)v->kt_names.size();
+  if (count) *count = n;
+  if (out) {
+    if (capacity < n) return fail(MH_ERR_CAPACITY, "kernel time buffer too small");
+    for (uint32_t i = 0; i < n; ++i) {
+      std::memset(out[i].name, 0, sizeof(out[i].name));
+      std::strncpy(out[i].name, v->kt_names[i].c_str(), sizeof(out[i].name) - 1);
+      out[i].launches = v->kt_launches[i];
+      out[i].total_ms = v->kt_ms[i];
+    }
+  }
+  if (reset) {
+    v->kt_names.clear();
+    v->kt_launches.clear();
+    v->kt_ms.clear();
+  }
   return MH_OK;
 }
 
@@ -460,6 +573,7 @@ int mh_get_status(mh_volume* v, mh_status* out) {
   out->mc_triangle_writes = w.mc_triangle_writes;
   out->mc_blocks = w.mc_blocks;
   out->ray_samples = w.ray_samples;
+  out->ray_blocks = w.ray_blocks;
   return MH_OK;
 }
 
@@ -524,6 +638,23 @@ int mh_inlier_ratio_device_ptr(mh_volume* v, float** inlier_dev) {
   return MH_OK;
 }
 
+int mh_volume_field_device_ptr(mh_volume* v, int field, void** field_dev, uint64_t* count) {
+  if (check_vol(v) || !field_dev) return fail(MH_ERR_ARG, "null argument");
+  void* p = nullptr;
+  switch (field) {
+    case MH_FIELD_SDF: p = v->V.sdf; break;
+    case MH_FIELD_WEIGHT: p = v->V.weight; break;
+    case MH_FIELD_COLOR: p = v->V.color; break;
+    case MH_FIELD_BETA_A: p = v->V.va; break;
+    case MH_FIELD_BETA_B: p = v->V.vb; break;
+    default: return fail(MH_ERR_ARG, "unknown field");
+  }
+  if (!p) return fail(MH_ERR_STATE, "volume has no PSDF fields");
+  *field_dev = p;
+  if (count) *count = (uint64_t)v->hp.value_capacity * kBlockVoxels;
+  return MH_OK;
+}
+
 int mh_alloc_blocks(mh_volume* v) {
   if (check_vol(v)) return MH_ERR_ARG;
   if (!v->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
@@ -532,14 +663,14 @@ int mh_alloc_blocks(mh_volume* v) {
   Va.raw_color = v->pending_color;
   v->pending_raw = nullptr;
   v->pending_color = nullptr;
-  launch_alloc(Va, v->stream);
+  launch_alloc(Va, v->stream, v->kc());
   ++v->work_gen;  // allocation clears the worklist
   HIP_TRY(hipGetLastError());
   return MH_OK;
 }
 
 static int collect(mh_volume* v, int frustum, uint32_t* count_host) {
-  launch_collect(v->V, frustum, v->stream, v->cus * 4);
+  launch_collect(v->V, frustum, v->stream, v->cus * 4, v->kc());
   ++v->work_gen;
   HIP_TRY(hipGetLastError());
   if (count_host) *count_host = read_counters(v).cand_count;
@@ -593,7 +724,7 @@ int mh_integrate_tsdf(mh_volume* v) {
   if (check_vol(v)) return MH_ERR_ARG;
   if (!v->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
   flush_frame(v);
-  launch_integrate_tsdf(v->V, v->stream, v->cus * v->grid_per_cu, v->tiling);
+  launch_integrate_tsdf(v->V, v->stream, v->cus * v->grid_per_cu, v->tiling, v->kc());
   HIP_TRY(hipGetLastError());
   return MH_OK;
 }
@@ -623,7 +754,7 @@ int mh_marching_cubes(mh_volume* v, const mh_mc_params* p) {
   McConst c;
   const int rc = build_mc(v, p, &c);
   if (rc) return rc;
-  launch_marching_cubes(v->V, c, v->stream, v->cus);
+  launch_marching_cubes(v->V, c, v->mc_nbt, v->stream, v->cus, v->kc());
   HIP_TRY(hipGetLastError());
   return MH_OK;
 }
@@ -636,7 +767,15 @@ int mh_raycast(mh_volume* v, const mh_raycaster_params* p, const float cTw[16], 
   if (rc) return rc;
   rc = attach_range(v, &R);
   if (rc) return rc;
-  launch_raycast(v->V, R, depth_dev, vertex_dev, normal_dev, v->stream);
+  if (v->diag & MH_DIAG_RAY_BLOCKS) {
+    View Vr = v->V;  // lookups record the blocks they find; then they are counted
+    Vr.ray_mark = v->ray_mark;
+    Vr.ray_stamp = ++v->ray_stamp;
+    launch_raycast(Vr, R, depth_dev, vertex_dev, normal_dev, v->stream, v->kc());
+    launch_ray_mark_count(Vr, v->stream, v->cus * 2);
+  } else {
+    launch_raycast(v->V, R, depth_dev, vertex_dev, normal_dev, v->stream, v->kc());
+  }
   HIP_TRY(hipGetLastError());
   return MH_OK;
 }
@@ -1143,6 +1282,7 @@ struct mh_batch {
   uint32_t* d_work_count = nullptr;
   uint16_t* d_raw = nullptr;       // staging for host depth uploads
   uint32_t* d_range = nullptr;     // raycast tile ranges, ray_range_words(W, H) per scene
+  int32_t* d_nbt = nullptr;        // marching cubes: 27 neighbours per concatenated-worklist item
   const uint16_t* raw = nullptr;   // depth of the current frame (n*W*H)
   bool have_frame = false;
   std::vector<mh_raycaster_params> rp;
@@ -1173,7 +1313,7 @@ void batch_free(mh_batch* b) {
     if (b->pinned[k]) (void)hipHostFree(b->pinned[k]);
   }
   for (void* p : {(void*)b->d_frames, (void*)b->d_work, (void*)b->d_work_count, (void*)b->d_raw,
-                  (void*)b->d_range})
+                  (void*)b->d_range, (void*)b->d_nbt})
     if (p) (void)hipFree(p);
   for (mh_volume* v : b->vols) mh_volume_destroy(v);
   if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
@@ -1188,7 +1328,7 @@ void batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   launch_batch_collect(B, 1, b->stream, b->cus * 4);
   if (!integrate) return;
   launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
-  if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->stream, b->cus);
+  if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus);
   if (flags & MH_RUN_RAYCAST) launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream);
 }
 
@@ -1321,7 +1461,8 @@ int mh_batch_create(uint32_t n_scenes, const mh_hash_params* hash, const mh_volu
   }
   const size_t npix = (size_t)b->W * b->H * n_scenes;
   if (dalloc(&b->d_frames, n_scenes) != hipSuccess || dalloc(&b->d_work, work_cap) != hipSuccess ||
-      dalloc(&b->d_work_count, 1) != hipSuccess || dalloc(&b->d_raw, npix) != hipSuccess)
+      dalloc(&b->d_work_count, 1) != hipSuccess || dalloc(&b->d_raw, npix) != hipSuccess ||
+      dalloc(&b->d_nbt, work_cap * 27) != hipSuccess)
     return bail(fail(MH_ERR_HIP, "device allocation failed"));
   for (int k = 0; k < mh_batch::kRing; ++k) {
     if (hipHostMalloc((void**)&b->pinned[k], sizeof(SceneFrame) * n_scenes, hipHostMallocDefault) !=
@@ -1445,7 +1586,7 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
     b->drop_graph();
     const BatchDev B = b->dev();
     launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
-    if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->stream, b->cus);
+    if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus);
     if (flags & MH_RUN_RAYCAST)
       launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream);
   } else if (flags & MH_RUN_GRAPH) {
@@ -1497,12 +1638,14 @@ hipError_t ensure_partials(mh_volume* v, uint32_t rows) {
 
 // The adjoint from a built RayConst (its range buffer attached) into the
 // given partial-sum rows and result row; synchronises the stream.
+static_assert(sizeof(mh_raycast_grad) == 360, "mh_raycast_grad layout");
+
 int raycast_grad_run(mh_volume* v, const RayConst& R, const float* obs, float* grad_sdf,
                      double* partials, double* res, mh_raycast_grad* out, float max_residual,
                      float max_slope) {
   launch_raycast_backward(v->V, R, obs, grad_sdf, partials, res, max_residual, max_slope, v->stream);
   HIP_TRY(hipGetLastError());
-  double h[30];
+  double h[kGradSlots];
   HIP_TRY(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, v->stream));
   HIP_TRY(hipStreamSynchronize(v->stream));
   out->loss = h[0];
@@ -1516,6 +1659,7 @@ int raycast_grad_run(mh_volume* v, const RayConst& R, const float* obs, float* g
     }
   out->pixels = (uint32_t)h[28];
   out->outliers = (uint32_t)h[29];
+  out->observed = (uint32_t)h[30];
   return MH_OK;
 }
 
@@ -1546,12 +1690,12 @@ constexpr uint32_t kNoAllocTiles = ((kNoAllocMaxWidth + 7) / 8) * ((kNoAllocMaxH
 constexpr size_t kNoAllocRound = 256;
 constexpr size_t round_up(size_t x) { return (x + kNoAllocRound - 1) / kNoAllocRound * kNoAllocRound; }
 constexpr size_t kNoAllocPartialOff = 0;
-constexpr size_t kNoAllocPartialBytes = round_up((size_t)kNoAllocTiles * 30 * sizeof(double));
+constexpr size_t kNoAllocPartialBytes = round_up((size_t)kNoAllocTiles * kGradSlots * sizeof(double));
 constexpr size_t kNoAllocResultOff = kNoAllocPartialOff + kNoAllocPartialBytes;
-constexpr size_t kNoAllocResultBytes = round_up(30 * sizeof(double));
+constexpr size_t kNoAllocResultBytes = round_up(kGradSlots * sizeof(double));
 constexpr size_t kNoAllocRangeOff = kNoAllocResultOff + kNoAllocResultBytes;
 constexpr size_t kNoAllocRangeBytes = round_up((2 * (size_t)kNoAllocTiles + 1) * sizeof(uint32_t));
-constexpr size_t kNoAllocScratchBytes = kNoAllocRangeOff + kNoAllocRangeBytes;  // ~5.1 MB
+constexpr size_t kNoAllocScratchBytes = kNoAllocRangeOff + kNoAllocRangeBytes;  // ~5.2 MB
 
 }  // namespace
 
@@ -1664,20 +1808,23 @@ int eval_pose(mh_volume* v, const mh_raycaster_params* p, const double T[16], co
 
 // Residual gate of the pose refinement: pixels with |r| above it (occlusion
 // boundaries, surfaces the volume does not hold) leave the least-squares
-// problem and cost gate^2 / 2 each (a truncated quadratic), so leaving
-// pixels out is never free.  The first 40 % of the steps are ungated (the
-// start may be centimetres off), then 0.10 m for 30 %, then 0.05 m.
+// problem.  The gate narrows as the pose settles: 0.20 m for the first 40 %
+// of the steps (the start may be centimetres and degrees off), 0.10 m for
+// the next 30 %, then 0.05 m.
 float gate_at(uint32_t step, uint32_t steps) {
-  if (10 * step < 4 * steps) return INFINITY;
+  if (10 * step < 4 * steps) return 0.20f;
   if (10 * step < 7 * steps) return 0.10f;
   return 0.05f;
 }
 
+// A truncated quadratic over every observed pixel: r^2 / 2 for a pixel in
+// the sum, gate^2 / 2 for one that is gated out or that the pose loses (no
+// hit, or a grazing ray).  The count of observed pixels does not depend on
+// the pose, so a step cannot lower the cost by dropping pixels.
 double truncated_cost(const mh_raycast_grad& g, float gate) {
-  const double n = (double)g.pixels + (double)g.outliers;
-  if (n == 0.0) return INFINITY;
-  const double out = g.outliers ? 0.5 * (double)gate * (double)gate * g.outliers : 0.0;
-  return (g.loss + out) / n;
+  if (g.observed == 0) return INFINITY;
+  const double left = (double)g.observed - (double)g.pixels;
+  return (g.loss + 0.5 * (double)gate * (double)gate * left) / (double)g.observed;
 }
 
 
@@ -1725,7 +1872,7 @@ int mh_refine_pose(mh_volume* v, const mh_raycaster_params* p, const float wTc_i
   int rc = eval_pose(v, p, T, obs_depth_dev, gate, &cur);
   if (rc) return rc;
   if (cost_trace) cost_trace[0] = truncated_cost(cur, gate);
-  double lambda = 1e-4;
+  double lambda = 1e-2;  // damped first steps: the start may be degrees off
   for (uint32_t it = 0; it < iterations; ++it) {
     double xi[6];
     bool improved = false;
@@ -1752,7 +1899,15 @@ int mh_refine_pose(mh_volume* v, const mh_raycaster_params* p, const float wTc_i
       if (rc) return rc;
     }
   }
-  for (int i = 0; i < 16; ++i) wTc_out[i] = (float)T[i];
+  // Each gate's steps only lower that gate's cost; the start is kept if the
+  // result costs more than it at the final gate.
+  double S[16];
+  for (int i = 0; i < 16; ++i) S[i] = wTc_init[i];
+  mh_raycast_grad at_start;
+  rc = eval_pose(v, p, S, obs_depth_dev, gate, &at_start);
+  if (rc) return rc;
+  const bool keep_start = truncated_cost(at_start, gate) < truncated_cost(cur, gate);
+  for (int i = 0; i < 16; ++i) wTc_out[i] = keep_start ? wTc_init[i] : (float)T[i];
   return MH_OK;
 }
 

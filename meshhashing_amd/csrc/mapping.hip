@@ -438,7 +438,8 @@ __device__ __forceinline__ uint32_t integrate_voxels(const View& V, const i3 b, 
     const size_t idx = (size_t)ptr * kBlockVoxels + lt + (uint32_t)k * L;
     V.sdf[idx] = (so[k] * wo[k] + sdf[k] * wgt[k]) / (wo[k] + wgt[k]);
     V.weight[idx] = wo[k] + wgt[k];
-    V.color[idx] = blend_color(co[k], dc[k]);
+    const uint32_t nc = blend_color(co[k], dc[k]);
+    if (nc != co[k]) V.color[idx] = nc;  // a converged colour is left alone (same bits)
     ++n;
   }
   return n;
@@ -550,7 +551,8 @@ __device__ __forceinline__ uint32_t integrate_voxels_early(const View& V, const 
     const size_t idx = base + (uint32_t)k * L;
     V.sdf[idx] = (so[k] * wo[k] + sdf[k] * wgt[k]) / (wo[k] + wgt[k]);
     V.weight[idx] = wo[k] + wgt[k];
-    V.color[idx] = blend_color(co[k], dc[k]);
+    const uint32_t nc = blend_color(co[k], dc[k]);
+    if (nc != co[k]) V.color[idx] = nc;  // a converged colour is left alone (same bits)
     ++n;
   }
   return n;
@@ -1062,15 +1064,21 @@ void launch_convert_color(const View& V, const uint8_t* bgra, hipStream_t s) {
                                                        V.color_in, n);
 }
 
-void launch_alloc(const View& V, hipStream_t s) {
+void launch_alloc(const View& V, hipStream_t s, const KernelClock* kc) {
   dim3 grid((V.cam.width + 15) / 16, (V.cam.height + 15) / 16);
+  tick(kc, s, "alloc_kernel");
   alloc_kernel<<<grid, 256, 0, s>>>(One{V}, nullptr);
+  tick(kc, s, "list_insert_kernel");
   list_insert_kernel<<<1, 64, 0, s>>>(One{V}, nullptr);
+  tick(kc, s, nullptr);
 }
 
-void launch_collect(const View& V, int frustum, hipStream_t s, int grid) {
+void launch_collect(const View& V, int frustum, hipStream_t s, int grid, const KernelClock* kc) {
+  tick(kc, s, "set_u32_kernel");
   set_u32_kernel<<<1, 64, 0, s>>>(&V.ctr->cand_count, 0u);
+  tick(kc, s, "collect_kernel");
   collect_kernel<<<grid, 256, 0, s>>>(One{V}, nullptr, frustum, nullptr, nullptr);
+  tick(kc, s, nullptr);
 }
 
 namespace {
@@ -1089,8 +1097,10 @@ void integrate_dispatch(const Src& src, hipStream_t s, int grid, int tiling) {
 }
 }  // namespace
 
-void launch_integrate_tsdf(const View& V, hipStream_t s, int grid, int tiling) {
+void launch_integrate_tsdf(const View& V, hipStream_t s, int grid, int tiling, const KernelClock* kc) {
+  tick(kc, s, "integrate_tsdf_kernel");
   integrate_dispatch(One{V}, s, grid, tiling);
+  tick(kc, s, nullptr);
 }
 
 void launch_predict_inlier(const View& V, hipStream_t s, int grid) {

@@ -43,38 +43,46 @@ __device__ __forceinline__ int nb27(int dx, int dy, int dz) {
   return (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
 }
 
-// Lanes kLookupLane + t (t = 0..26) of the last wave look up neighbour t (x
-// fastest, the block itself at 13) of worklist item ci and that block's
-// meshing mark.  mc_surfel_kernel runs this for its next item while the
-// current item's payloads are written (by the first waves: the queue fills
-// from lane 0), so the dependent worklist, key and bucket reads stay off
-// the critical path.
-constexpr uint32_t kLookupLane = kBlockVoxels - 64;
+// The 27 neighbours of every worklist block (x fastest, the block itself at
+// 13) with their meshed-in-this-pass marks, one lane per (item, neighbour),
+// before the surfel pass: the dependent worklist -> key -> occupancy ->
+// bucket -> mark reads of a lookup then run with the whole grid's
+// parallelism instead of on one workgroup's critical path.  Entry: the pool
+// index, with bit 30 set when the neighbour is meshed in this pass; a
+// negative value for an absent block.
+__device__ __forceinline__ const McConst& mc_of(const One&, uint32_t, const McConst& mc) { return mc; }
+__device__ __forceinline__ const McConst& mc_of(const Many& m, uint32_t s, const McConst&) {
+  return m.f[s].mc;
+}
+constexpr uint32_t kLookupLane = kBlockVoxels - 64;  // the surfel pass's last wave
+constexpr int32_t kMeshedBit = 1 << 30;
 template <class Src>
-__device__ __forceinline__ void neighbour_lookup(const Src& src, uint32_t ci, int32_t* p,
-                                                 uint32_t* mark) {
-  *p = kFreePtr;
-  *mark = 0;
-  const uint32_t t = threadIdx.x - kLookupLane;
-  if (t < 27) {
+__global__ __launch_bounds__(256) void mc_neighbours_kernel(Src src_, const SceneFrame* __restrict__ tab,
+                                                             McConst mc1, int32_t* __restrict__ nbt) {
+  const Src src = with_table(src_, tab);
+  const uint32_t total = src.count() * 27u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const uint32_t ci = i / 27u, t = i % 27u;
     uint32_t scene;
     const int32_t self = src.lane_item(ci, &scene);
     const View& V = src.view(scene);
     const i3 b = unpack_key(V.block_keys[self]);
     const int dx = (int)(t % 3) - 1, dy = (int)((t / 3) % 3) - 1, dz = (int)(t / 9) - 1;
-    *p = (dx | dy | dz) == 0 ? self : hash_find(V, i3{b.x + dx, b.y + dy, b.z + dz});
-    if (*p >= 0) *mark = V.block_mark[*p];
+    const int32_t p = (dx | dy | dz) == 0 ? self : hash_find(V, i3{b.x + dx, b.y + dy, b.z + dz});
+    int32_t e = kFreePtr;
+    if (p >= 0) e = p | (V.block_mark[p] == mc_of(src, scene, mc1).pass ? kMeshedBit : 0);
+    nbt[i] = e;
   }
 }
 
-// nbp, markp: the lane's neighbour_lookup results for this block.
+// nbe: the lane's neighbour-table entry for this block (lanes
+// kLookupLane + t, t < 27).
 template <bool PSDF>
-__device__ void stage_shell(const View& V, int32_t nbp, uint32_t markp, Shell<PSDF>& S, int32_t* nb,
-                            uint8_t* meshed, uint32_t pass) {
+__device__ void stage_shell(const View& V, int32_t nbe, Shell<PSDF>& S, int32_t* nb, uint8_t* meshed) {
   const uint32_t t = threadIdx.x - kLookupLane;
   if (t < 27) {
-    nb[t] = nbp;
-    meshed[t] = nbp >= 0 && markp == pass;
+    nb[t] = nbe < 0 ? nbe : (nbe & (kMeshedBit - 1));
+    meshed[t] = nbe >= 0 && (nbe & kMeshedBit) != 0;
   }
   __syncthreads();
   // All of a lane's loads are issued before any is stored to LDS (one
@@ -329,6 +337,85 @@ __device__ __forceinline__ bool write_vertex(const Shell<PSDF>& S, const View& V
   return oob;
 }
 
+// The same payload spread over eight lanes per vertex, so that a vertex
+// costs the latency of one trilinear sample instead of seven in a row: lanes
+// 0-5 take the gradient stencils p -+ h e_k (k = sub / 2; odd: +h), lane 6
+// the full sample at p (radius, colour) and stores position, radius and
+// colour; lane 7 is idle.  Stencil values and validities meet in LDS
+// (pay_s, pay_ok, row jl), from which lane 0 finishes the normal after the
+// workgroup barrier (finish_normal).  Every sample is write_vertex's, with
+// the same axes and arithmetic, so the stored bits are the same.
+template <bool PSDF>
+__device__ __forceinline__ bool payload_part(const Shell<PSDF>& S, const View& V, const McConst& mc,
+                                             i3 base, uint32_t key, int32_t vp, uint32_t sub,
+                                             float* pay_s, uint8_t* pay_ok) {
+  bool oob = false;
+  const Geo& g = V.geo;
+  const int ox = (int)(key & 15u), oy = (int)((key >> 4) & 15u), oz = (int)((key >> 8) & 15u);
+  const int ax = (int)(key >> 12);
+  const float vA = S.sdf[sidx(ox, oy, oz)];
+  const float vB = S.sdf[sidx(ox + (ax == 0), oy + (ax == 1), oz + (ax == 2))];
+  const f3 pA = voxel_to_world(g, i3{base.x + ox, base.y + oy, base.z + oz});
+  const f3 pB = mk3(pA.x + (ax == 0 ? g.voxel_size : 0.0f), pA.y + (ax == 1 ? g.voxel_size : 0.0f),
+                    pA.z + (ax == 2 ? g.voxel_size : 0.0f));
+  f3 p;
+  if ((double)fabsf(vA) < mc.snap) {
+    p = pA;
+  } else if ((double)fabsf(vB) < mc.snap) {
+    p = pB;
+  } else {
+    const float mu = (0.0f - vA) / (vB - vA);
+    p = mk3(pA.x + mu * (pB.x - pA.x), pA.y + mu * (pB.y - pA.y), pA.z + mu * (pB.z - pA.z));
+  }
+  const float vs = g.voxel_size;
+  const SAxis X = shell_axis(p.x, vs, base.x), Y = shell_axis(p.y, vs, base.y), Z = shell_axis(p.z, vs, base.z);
+  if (sub < 6) {
+    const int k = (int)(sub >> 1);
+    const float pk = k == 0 ? p.x : (k == 1 ? p.y : p.z);
+    const int bk = k == 0 ? base.x : (k == 1 ? base.y : base.z);
+    const float h = 0.5f * g.voxel_size;
+    const SAxis An = shell_axis((sub & 1u) ? pk + h : pk - h, vs, bk);
+    SVal q{};
+    const bool ok = shell_value<PSDF, false>(S, k == 0 ? An : X, k == 1 ? An : Y, k == 2 ? An : Z, &q, &oob);
+    pay_s[sub] = q.sdf;
+    pay_ok[sub] = ok ? 1 : 0;
+  } else if (sub == 6) {
+    SVal sv{};
+    const bool ok = shell_value(S, X, Y, Z, &sv, &oob);
+    Vertex* dst = &V.vertices[vp];
+    float radius = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;  // the reference reads an uninitialised query
+    if (ok) {
+      const float inv_sigma2 = 1.0f / (sv.radius * sv.radius);
+      radius = sqrtf(1.0f / inv_sigma2);
+      const f3 c = val_to_rgb(sv.a / (sv.a + sv.b), 0.4f, 1.0f);
+      cr = c.x;
+      cg = c.y;
+      cb = c.z;
+    }
+    *reinterpret_cast<float4*>(&dst->px) = make_float4(p.x, p.y, p.z, radius);
+    dst->cr = cr;
+    dst->cg = cg;
+    dst->cb = cb;
+  }
+  return oob;
+}
+
+// GetSpatialSDFGradient's normal from the six stencil values of a vertex
+// (pay_s: -x, +x, -y, +y, -z, +z); any invalid sample zeroes it, which is
+// what the reference's short-circuit produces.
+__device__ __forceinline__ void finish_normal(const View& V, int32_t vp, const float* pay_s,
+                                              const uint8_t* pay_ok) {
+  const float vs = V.geo.voxel_size;
+  const bool valid = pay_ok[0] && pay_ok[1] && pay_ok[2] && pay_ok[3] && pay_ok[4] && pay_ok[5];
+  const f3 grad = mk3((pay_s[1] - pay_s[0]) / vs, (pay_s[3] - pay_s[2]) / vs, (pay_s[5] - pay_s[4]) / vs);
+  const float l = length(grad);
+  const f3 n = (l > 0.0f && valid) ? grad / l : mk3(0.0f, 0.0f, 0.0f);
+  Vertex* dst = &V.vertices[vp];
+  dst->nx = n.x;
+  dst->ny = n.y;
+  dst->nz = n.z;
+}
+
 // SurfelExtractionKernel (marching_cubes.cu:75-170): cube index of every
 // cube of the block, and the vertex of every active edge.  A vertex is
 // shared by up to four cubes.  Its payload is written once per pass, by a
@@ -338,10 +425,6 @@ __device__ __forceinline__ bool write_vertex(const Shell<PSDF>& S, const View& V
 // 10^3 halo of cube validities and the meshed flags of the 27 blocks), so
 // no atomics are needed; the other cubes only make sure the vertex slot is
 // allocated.  Any writer would store the same bits (canonical vertices).
-__device__ __forceinline__ const McConst& mc_of(const One&, uint32_t, const McConst& mc) { return mc; }
-__device__ __forceinline__ const McConst& mc_of(const Many& m, uint32_t s, const McConst&) {
-  return m.f[s].mc;
-}
 
 // Work counters of one scene, flushed per wave.
 struct McCounts {
@@ -411,7 +494,8 @@ __device__ __forceinline__ int32_t* edge_slot(const View& V, const int32_t* nb, 
 }
 
 template <bool PSDF, class Src>
-__global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1) {
+__global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1,
+                                                           const int32_t* __restrict__ nbt) {
   const Src src = with_table(src_, tab);
   __shared__ Shell<PSDF> S;
   __shared__ int32_t nb[27];
@@ -425,6 +509,10 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
   __shared__ int32_t pop_top;
   __shared__ uint16_t items[12 * kBlockVoxels];
   __shared__ uint32_t nitems;
+  // payload exchange: eight lanes per vertex, kLookupLane / 8 vertices a round
+  constexpr uint32_t kPayRound = kLookupLane / 8;
+  __shared__ float pay_s[kPayRound * 8];
+  __shared__ uint8_t pay_ok[kPayRound * 8];
   // per-configuration crossed edges and triangle counts, read per lane with
   // divergent indices: from LDS rather than the constant tables
   __shared__ uint16_t etab[256];
@@ -435,16 +523,17 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
   }
   const uint32_t n = src.count();
   McCounts cnt;
-  uint32_t cur = ~0u;
-  int32_t nbp;
-  uint32_t markp;
-  if (blockIdx.x < n) neighbour_lookup(src, blockIdx.x, &nbp, &markp);
+  uint32_t cur_scene = ~0u;
+  // lanes kLookupLane + t hold neighbour t's table entry of the next block
+  const uint32_t nb_lane = threadIdx.x - kLookupLane;
+  int32_t nbe = kFreePtr;
+  if (nb_lane < 27 && blockIdx.x < n) nbe = nbt[(size_t)blockIdx.x * 27u + nb_lane];
   for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
     uint32_t scene;
     const int32_t ptr = src.item(ci, &scene);
-    if (scene != cur) {
-      if (cur != ~0u) cnt.flush(src.view(cur));
-      cur = scene;
+    if (scene != cur_scene) {
+      if (cur_scene != ~0u) cnt.flush(src.view(cur_scene));
+      cur_scene = scene;
     }
     const View& V = src.view(scene);
     const McConst& mc = mc_of(src, scene, mc1);
@@ -453,7 +542,7 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
     const uint32_t i = threadIdx.x;
     const size_t idx = (size_t)ptr * kBlockVoxels + i;
     const uint32_t prev = V.cube[idx] & 255u;  // in flight with the shell loads
-    stage_shell<PSDF>(V, nbp, markp, S, nb, meshed, mc.pass);
+    stage_shell<PSDF>(V, nbe, S, nb, meshed);
     if (threadIdx.x == 0) qn = 0;
     __syncthreads();
     // every cube of the 10^3 halo, the block's own among them: validity for
@@ -563,15 +652,26 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
       }
     }
     __syncthreads();
-    if (ci + gridDim.x < n) neighbour_lookup(src, ci + gridDim.x, &nbp, &markp);
+    if (nb_lane < 27 && ci + gridDim.x < n) nbe = nbt[(size_t)(ci + gridDim.x) * 27u + nb_lane];
     const uint32_t nq = qn < kQueue ? qn : kQueue;
-    for (uint32_t j = threadIdx.x; j < nq; j += kBlockVoxels) {
-      cnt.oob = write_vertex<PSDF>(S, V, mc, base, qkey[j], qptr[j]) || cnt.oob;
-      ++cnt.writes;
+    // payloads, eight lanes per vertex, on the waves before the last one
+    // (which looks up the next block's neighbours meanwhile)
+    for (uint32_t j0 = 0; j0 < nq; j0 += kPayRound) {  // uniform
+      const uint32_t jl = threadIdx.x >> 3, sub = threadIdx.x & 7u;
+      const uint32_t j = j0 + jl;
+      const bool act = threadIdx.x < kLookupLane && j < nq;
+      if (act)
+        cnt.oob = payload_part<PSDF>(S, V, mc, base, qkey[j], qptr[j], sub, &pay_s[8 * jl], &pay_ok[8 * jl]) ||
+                  cnt.oob;
+      __syncthreads();
+      if (act && sub == 0) {
+        finish_normal(V, qptr[j], &pay_s[8 * jl], &pay_ok[8 * jl]);
+        ++cnt.writes;
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
-  if (cur != ~0u) cnt.flush(src.view(cur));
+  if (cur_scene != ~0u) cnt.flush(src.view(cur_scene));
 }
 
 // Meshed-in-this-pass marks for the writer rule.
@@ -955,16 +1055,23 @@ int resident_grid(int cus) {
   return per_cu * cus;
 }
 
-void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int cus) {
+void launch_marching_cubes(const View& V, const McConst& p, int32_t* nbt, hipStream_t s, int cus,
+                           const KernelClock* kc) {
+  tick(kc, s, "mc_mark_kernel");
   mc_mark_kernel<<<cus, 256, 0, s>>>(One{V}, nullptr, p);
+  tick(kc, s, "mc_neighbours_kernel");
+  mc_neighbours_kernel<<<cus * 8, 256, 0, s>>>(One{V}, nullptr, p, nbt);
+  tick(kc, s, "mc_surfel_kernel");
   if (V.va)
     mc_surfel_kernel<true><<<resident_grid<mc_surfel_kernel<true, One>, kBlockVoxels>(cus), kBlockVoxels, 0, s>>>(
-        One{V}, nullptr, p);
+        One{V}, nullptr, p, nbt);
   else
     mc_surfel_kernel<false><<<resident_grid<mc_surfel_kernel<false, One>, kBlockVoxels>(cus), kBlockVoxels, 0,
-                              s>>>(One{V}, nullptr, p);
+                              s>>>(One{V}, nullptr, p, nbt);
+  tick(kc, s, "mc_recycle_vertices_kernel");
   mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<One>, 256>(cus), 256, 0, s>>>(One{V},
                                                                                                    nullptr);
+  tick(kc, s, nullptr);
 }
 
 void launch_gc_recycle_vertices(const View& V, hipStream_t s, int cus) {
@@ -972,15 +1079,16 @@ void launch_gc_recycle_vertices(const View& V, hipStream_t s, int cus) {
       Listed{V, V.gc_list, &V.ctr->gc_count}, nullptr);
 }
 
-void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int cus) {
+void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus) {
   const McConst unused{};
   mc_mark_kernel<<<cus, 256, 0, s>>>(B.src(), B.frames, unused);
+  mc_neighbours_kernel<<<cus * 8, 256, 0, s>>>(B.src(), B.frames, unused, nbt);
   if (B.psdf)
     mc_surfel_kernel<true><<<resident_grid<mc_surfel_kernel<true, Many>, kBlockVoxels>(cus), kBlockVoxels, 0,
-                             s>>>(B.src(), B.frames, unused);
+                             s>>>(B.src(), B.frames, unused, nbt);
   else
     mc_surfel_kernel<false><<<resident_grid<mc_surfel_kernel<false, Many>, kBlockVoxels>(cus), kBlockVoxels, 0,
-                              s>>>(B.src(), B.frames, unused);
+                              s>>>(B.src(), B.frames, unused, nbt);
   mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<Many>, 256>(cus), 256, 0, s>>>(
       B.src(), B.frames);
 }

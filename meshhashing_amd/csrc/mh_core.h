@@ -225,6 +225,7 @@ struct alignas(256) WorkStripe {
   unsigned long long mc_blocks;          // worklist blocks meshed
   unsigned long long ray_samples;        // valid trilinear samples of the raycaster
   unsigned long long tri_live;           // live triangles (sum of table rows; deltas wrap)
+  unsigned long long ray_blocks;         // distinct blocks each diagnostic raycast sampled
 };
 
 // Device-wide counters and status, one instance per volume.
@@ -259,6 +260,7 @@ inline WorkStripe work_totals(const Counters& c) {
     t.mc_blocks += c.work[i].mc_blocks;
     t.ray_samples += c.work[i].ray_samples;
     t.tri_live += c.work[i].tri_live;
+    t.ray_blocks += c.work[i].ray_blocks;
   }
   return t;
 }
@@ -329,6 +331,10 @@ struct View {
   float* inlier;         // W*H PSDF inlier ratio
   M4 wTc, cTw;           // FP32 pose (decisions)
   D34 cTw64;             // FP64 world->camera (values)
+  // raycast diagnostics: per block, the stamp of the last raycast whose
+  // block lookups found it (null: not recorded)
+  uint32_t* ray_mark;
+  uint32_t ray_stamp;
 };
 
 // Occupancy filter: one bit per hashed block key (2^kOccBits bits, 2 MiB,

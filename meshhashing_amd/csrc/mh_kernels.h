@@ -135,13 +135,28 @@ constexpr Tiling kIntegrateTilings[] = {
 constexpr int kNumIntegrateTilings = sizeof(kIntegrateTilings) / sizeof(kIntegrateTilings[0]);
 constexpr int kDefaultIntegrateTiling = 2;  // 256 lanes, one block, 2 voxels per lane
 
+// Optional per-kernel timing (the reference's per-stage timers, util/timer.h).
+// A launcher that runs several kernels calls tick(s, name) before each of
+// them and tick(s, nullptr) after the last; the host side records HIP events
+// at those points.  Null: no timing.
+struct KernelClock {
+  void (*fn)(void* ctx, hipStream_t s, const char* next);
+  void* ctx;
+  void tick(hipStream_t s, const char* next) const { fn(ctx, s, next); }
+};
+inline void tick(const KernelClock* c, hipStream_t s, const char* next) {
+  if (c) c->tick(s, next);
+}
+
 // mapping.hip
 void launch_reset(const View& V, hipStream_t s, int grid);
 void launch_convert_depth(const View& V, const uint16_t* raw, hipStream_t s);
 void launch_convert_color(const View& V, const uint8_t* bgra, hipStream_t s);
-void launch_alloc(const View& V, hipStream_t s);
-void launch_collect(const View& V, int frustum, hipStream_t s, int grid);
-void launch_integrate_tsdf(const View& V, hipStream_t s, int grid, int tiling);
+void launch_alloc(const View& V, hipStream_t s, const KernelClock* kc = nullptr);
+void launch_collect(const View& V, int frustum, hipStream_t s, int grid,
+                    const KernelClock* kc = nullptr);
+void launch_integrate_tsdf(const View& V, hipStream_t s, int grid, int tiling,
+                           const KernelClock* kc = nullptr);
 void launch_predict_inlier(const View& V, hipStream_t s, int grid);
 void launch_integrate_psdf(const View& V, hipStream_t s, int grid);
 void launch_load_insert(const View& V, const uint64_t* keys, uint32_t n, hipStream_t s);
@@ -169,8 +184,11 @@ void launch_gc_recycle(const View& V, hipStream_t s, int cus);
 // meshing.hip
 void launch_gc_release_triangles(const View& V, hipStream_t s, int grid);
 void launch_gc_recycle_vertices(const View& V, hipStream_t s, int cus);
-void launch_marching_cubes(const View& V, const McConst& p, hipStream_t s, int cus);
-void launch_batch_marching_cubes(const BatchDev& B, hipStream_t s, int cus);
+// nbt: the 27-neighbour table, 27 int32 per worklist item (for a batch, per
+// item of the concatenated worklist)
+void launch_marching_cubes(const View& V, const McConst& p, int32_t* nbt, hipStream_t s, int cus,
+                           const KernelClock* kc = nullptr);
+void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus);
 // mesh compaction (CompressMesh) and an exclusive scan of n uint32 (chunk_sums:
 // ceil(n / 1024) entries)
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* chunk_sums,
@@ -183,11 +201,15 @@ void launch_compact_emit(const View& V, uint32_t n, const uint32_t* blk_off, con
 
 // raycast.hip
 void launch_raycast(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,
-                    hipStream_t s);
+                    hipStream_t s, const KernelClock* kc = nullptr);
+// diagnostics: add the distinct blocks the last raycast sampled (V.ray_mark
+// == V.ray_stamp) to the work counters
+void launch_ray_mark_count(const View& V, hipStream_t s, int grid);
 void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,
                           hipStream_t s);
-// adjoint: 30 FP64 sums (loss, dL/dxi[6], J^T J upper triangle[21], pixels,
-// gated-out pixels)
+// adjoint: 31 FP64 sums (loss, dL/dxi[6], J^T J upper triangle[21], pixels,
+// gated-out pixels, observed pixels)
+constexpr int kGradSlots = 31;
 uint32_t raycast_grad_partials(const RayConst& p);
 void launch_raycast_backward(const View& V, const RayConst& p, const float* obs, float* grad_sdf,
                              double* partials, double* out, float max_residual, float max_slope,

@@ -51,6 +51,7 @@ __device__ __forceinline__ int32_t table_find(const View& V, WaveTable* t, i3 b)
   const uint64_t e = t->e[slot];
   if (e != 0ull && (e >> 24) == tag) return (int32_t)(e & 0xFFFFFFu) - 3;
   const int32_t p = coord_ok(b) ? hash_find(V, b) : kFreePtr;
+  if (V.ray_mark && p >= 0) V.ray_mark[p] = V.ray_stamp;  // diagnostics: block sampled
   if (p >= (1 << 24) - 3) return p;  // does not fit the entry
   const uint32_t lane = threadIdx.x & 63u;
   volatile uint32_t* owner = t->owner;
@@ -87,6 +88,48 @@ __device__ __forceinline__ int32_t cached_find(const View& V, BlockCache& c, i3 
 }
 
 
+// Pool indices of the blocks holding a trilinear cell's eight corners
+// (corner i = (cx[i>>2 & 1], cy[i>>1 & 1], cz[i & 1])).  Each distinct
+// block is looked up once, lowest corner first, from a single call site (a
+// rolled loop): the probe is emitted once per caller instead of once per
+// corner.  kStop: end at the first missing block (false); otherwise every
+// block is looked up and missing ones are left negative.
+template <bool kStop>
+__device__ __forceinline__ bool corner_blocks(const View& V, BlockCache& cache, const int cx[2],
+                                              const int cy[2], const int cz[2], int32_t pb[8]) {
+  const int bx0 = voxel_to_block1(cx[0]), by0 = voxel_to_block1(cy[0]), bz0 = voxel_to_block1(cz[0]);
+  const bool xx = voxel_to_block1(cx[1]) != bx0, yy = voxel_to_block1(cy[1]) != by0,
+             zz = voxel_to_block1(cz[1]) != bz0;
+  // distinct blocks: entry e = (dx << 2 | dy << 1 | dz) for block b0 + d
+  uint32_t todo = 1;
+#pragma unroll
+  for (int e = 1; e < 8; ++e)
+    if ((!(e & 4) || xx) && (!(e & 2) || yy) && (!(e & 1) || zz)) todo |= 1u << e;
+  int32_t be[8];
+  bool ok = true;
+  while (todo) {
+    const int e = __ffs(todo) - 1;
+    todo &= todo - 1u;
+    const int32_t q = cached_find(V, cache, i3{bx0 + ((e >> 2) & 1), by0 + ((e >> 1) & 1), bz0 + (e & 1)});
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (j == e) be[j] = q;
+    if (q < 0) {
+      ok = false;
+      if (kStop) return false;
+    }
+  }
+  // corner i lies in entry (i & 4 if xx) | (i & 2 if yy) | (i & 1 if zz),
+  // chosen by selects (constant indices keep be[] in registers)
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const bool sx = (i & 4) && xx, sy = (i & 2) && yy, sz = (i & 1) && zz;
+    pb[i] = sx ? (sy ? (sz ? be[7] : be[6]) : (sz ? be[5] : be[4]))
+               : (sy ? (sz ? be[3] : be[2]) : (sz ? be[1] : be[0]));
+  }
+  return ok;
+}
+
 // GetSpatialValue restricted to the SDF (the only field the march uses),
 // without the empty-space test.
 __device__ __forceinline__ bool ray_sdf_full(const View& V, BlockCache& cache, f3 pos, float* out) {
@@ -98,27 +141,15 @@ __device__ __forceinline__ bool ray_sdf_full(const View& V, BlockCache& cache, f
   const int cx[2] = {vox1(pcx + 0.0f, vs), vox1(pcx + vs, vs)};
   const int cy[2] = {vox1(pcy + 0.0f, vs), vox1(pcy + vs, vs)};
   const int cz[2] = {vox1(pcz + 0.0f, vs), vox1(pcz + vs, vs)};
-  const int bx[2] = {voxel_to_block1(cx[0]), voxel_to_block1(cx[1])};
-  const int by[2] = {voxel_to_block1(cy[0]), voxel_to_block1(cy[1])};
-  const int bz[2] = {voxel_to_block1(cz[0]), voxel_to_block1(cz[1])};
   // The sample is invalid if any corner's block is missing; in empty space
   // the lowest corner's block already says so (one cached probe).
-  const int32_t p000 = cached_find(V, cache, i3{bx[0], by[0], bz[0]});
-  if (p000 < 0) return false;
   int32_t pb[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int mx = (i >> 2) & 1, my = (i >> 1) & 1, mz = i & 1;
-    const bool same = bx[mx] == bx[0] && by[my] == by[0] && bz[mz] == bz[0];
-    pb[i] = same ? p000 : cached_find(V, cache, i3{bx[mx], by[my], bz[mz]});
-    if (pb[i] < 0) return false;
-  }
+  if (!corner_blocks<true>(V, cache, cx, cy, cz, pb)) return false;
   float sdf = 0.0f;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int mx = (i >> 2) & 1, my = (i >> 1) & 1, mz = i & 1;
-    const uint32_t li = (uint32_t)((cx[mx] - bx[mx] * 8) + 8 * (cy[my] - by[my] * 8) +
-                                   64 * (cz[mz] - bz[mz] * 8));
+    const uint32_t li = (uint32_t)((cx[mx] & 7) + 8 * (cy[my] & 7) + 64 * (cz[mz] & 7));
     const float w = (mx ? rx : 1.0f - rx) * (my ? ry : 1.0f - ry) * (mz ? rz : 1.0f - rz);
     sdf += w * V.sdf[(size_t)pb[i] * kBlockVoxels + li];
   }
@@ -473,15 +504,14 @@ __device__ __forceinline__ bool ray_sdf_grad(const View& V, BlockCache& cache, f
   const int cx[2] = {vox1(pcx + 0.0f, vs), vox1(pcx + vs, vs)};
   const int cy[2] = {vox1(pcy + 0.0f, vs), vox1(pcy + vs, vs)};
   const int cz[2] = {vox1(pcz + 0.0f, vs), vox1(pcz + vs, vs)};
+  int32_t pb[8];
+  if (!corner_blocks<true>(V, cache, cx, cy, cz, pb)) return false;
   float sdf = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int mx = (i >> 2) & 1, my = (i >> 1) & 1, mz = i & 1;
-    const i3 b = i3{voxel_to_block1(cx[mx]), voxel_to_block1(cy[my]), voxel_to_block1(cz[mz])};
-    const int32_t p = cached_find(V, cache, b);
-    if (p < 0) return false;
-    const uint32_t li = (uint32_t)((cx[mx] - b.x * 8) + 8 * (cy[my] - b.y * 8) + 64 * (cz[mz] - b.z * 8));
-    const float c = V.sdf[(size_t)p * kBlockVoxels + li];
+    const uint32_t li = (uint32_t)((cx[mx] & 7) + 8 * (cy[my] & 7) + 64 * (cz[mz] & 7));
+    const float c = V.sdf[(size_t)pb[i] * kBlockVoxels + li];
     const float wx = mx ? rx : 1.0f - rx, wy = my ? ry : 1.0f - ry, wz = mz ? rz : 1.0f - rz;
     sdf += wx * wy * wz * c;
     gx += (mx ? c : -c) * wy * wz;
@@ -504,14 +534,14 @@ __device__ __forceinline__ void scatter_sdf(const View& V, BlockCache& cache, f3
   const int cx[2] = {vox1(pcx + 0.0f, vs), vox1(pcx + vs, vs)};
   const int cy[2] = {vox1(pcy + 0.0f, vs), vox1(pcy + vs, vs)};
   const int cz[2] = {vox1(pcz + 0.0f, vs), vox1(pcz + vs, vs)};
+  int32_t pb[8];
+  corner_blocks<false>(V, cache, cx, cy, cz, pb);
   for (int i = 0; i < 8; ++i) {
     const int mx = (i >> 2) & 1, my = (i >> 1) & 1, mz = i & 1;
-    const i3 b = i3{voxel_to_block1(cx[mx]), voxel_to_block1(cy[my]), voxel_to_block1(cz[mz])};
-    const int32_t p = cached_find(V, cache, b);
-    if (p < 0) continue;  // cannot happen for a sample the forward pass accepted
-    const uint32_t li = (uint32_t)((cx[mx] - b.x * 8) + 8 * (cy[my] - b.y * 8) + 64 * (cz[mz] - b.z * 8));
+    if (pb[i] < 0) continue;  // cannot happen for a sample the forward pass accepted
+    const uint32_t li = (uint32_t)((cx[mx] & 7) + 8 * (cy[my] & 7) + 64 * (cz[mz] & 7));
     const float w = (mx ? rx : 1.0f - rx) * (my ? ry : 1.0f - ry) * (mz ? rz : 1.0f - rz);
-    atomicAdd(&grad_sdf[(size_t)p * kBlockVoxels + li], coef * w);
+    atomicAdd(&grad_sdf[(size_t)pb[i] * kBlockVoxels + li], coef * w);
   }
 }
 
@@ -521,7 +551,6 @@ __device__ __forceinline__ void scatter_sdf(const View& V, BlockCache& cache, f3
 // evaluations the depth depends on (the two bracketing samples and the
 // three regula-falsi points).
 constexpr int kTan = 11;
-constexpr int kGradSlots = 30;  // loss, g[6], H upper triangle [21], pixels, gated out
 
 __device__ __forceinline__ void sample_tangent(f3 x, f3 g, int slot, float* t) {
   const f3 c = mk3(x.y * g.z - x.z * g.y, x.z * g.x - x.x * g.z, x.x * g.y - x.y * g.x);  // x cross g
@@ -559,7 +588,7 @@ __device__ __forceinline__ bool depth_tangent(const View& V, const RayConst& R, 
 #pragma unroll
   for (int k = 0; k < kTan; ++k) dl[k] = dr[k] = 0.0f;
   float m = 0.0f;
-#pragma unroll
+#pragma unroll 1
   for (int j = 0; j < 3; ++j) {
     const float D = ls - rs;
     const float q = ls / D;
@@ -570,11 +599,14 @@ __device__ __forceinline__ bool depth_tangent(const View& V, const RayConst& R, 
       const float dq = (ls * drs[k] - rs * dls[k]) * inv_d2;
       dm[k] = dl[k] + dq * (r - l) + q * (dr[k] - dl[k]);
     }
-    X[2 + j] = mk3(o.x + m * dir.x, o.y + m * dir.y, o.z + m * dir.z);
+    const f3 xm = mk3(o.x + m * dir.x, o.y + m * dir.y, o.z + m * dir.z);
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (j == q) X[2 + q] = xm;
     float ms;
-    ray_sdf_grad(V, cache, X[2 + j], &ms, &g);
+    ray_sdf_grad(V, cache, xm, &ms, &g);
     float dms[kTan];
-    sample_tangent(X[2 + j], g, 8 + j, dms);
+    sample_tangent(xm, g, 8 + j, dms);
     const float gd = g.x * dir.x + g.y * dir.y + g.z * dir.z;
 #pragma unroll
     for (int k = 0; k < kTan; ++k) dms[k] += gd * dm[k];
@@ -599,6 +631,8 @@ __device__ __forceinline__ bool depth_tangent(const View& V, const RayConst& R, 
 // Per pixel: residual r = depth - observed depth (pixels with a hit, a
 // finite positive observation and |r| <= max_residual; the others with a
 // hit and an observation are counted as gated out), J = d depth / d xi.
+// Every pixel with a finite positive observation is counted as observed,
+// hit or not, so a caller can charge the pixels a pose loses.
 // Pixels whose depth moves faster than max_slope times a camera translation
 // (|d depth / d upsilon| on some axis; rays close to grazing the surface)
 // are left out altogether.  Per workgroup (one
@@ -622,6 +656,7 @@ __global__ __launch_bounds__(64) void raycast_backward_kernel(View V, RayConst R
     BlockCache cache{kEmptyKey, kEmptyKey, kFreePtr, kFreePtr, &table};
     float d, dd[kTan];
     f3 X[5];
+    acc[30] = od > 0.0f && od < INFINITY ? 1.0 : 0.0;
     if (od > 0.0f && od < INFINITY && depth_tangent(V, R, x, y, cache, &d, dd, X) &&
         fmaxf(fabsf(dd[3]), fmaxf(fabsf(dd[4]), fabsf(dd[5]))) <= max_slope) {
       const float res = d - od;
@@ -698,11 +733,18 @@ __global__ __launch_bounds__(64) void point_to_psdf_kernel(View V, M4 wTc, float
         const float h = 0.5f * V.geo.voxel_size;
         bool ok = true;
         float sp[3] = {0, 0, 0}, sn[3] = {0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < 3 && ok; ++k) {
+        // x-, x+, y-, y+, z-, z+ from one (rolled) sample site
+#pragma unroll 1
+        for (int j = 0; j < 6 && ok; ++j) {
+          const int k = j >> 1;
           const f3 dp = mk3(k == 0 ? h : 0.0f, k == 1 ? h : 0.0f, k == 2 ? h : 0.0f);
-          ok = ray_sdf(V, cache, pw - dp, inv_vs, &sn[k]);
-          if (ok) ok = ray_sdf(V, cache, pw + dp, inv_vs, &sp[k]);
+          float v;
+          ok = ray_sdf(V, cache, (j & 1) ? pw + dp : pw - dp, inv_vs, &v);
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            if (j == 2 * q) sn[q] = v;
+            if (j == 2 * q + 1) sp[q] = v;
+          }
         }
         if (ok) {
           const float vs = V.geo.voxel_size;
@@ -864,10 +906,29 @@ void launch_ray_range(Src src, const SceneFrame* tab, const RayConst& R, uint32_
 }
 
 void launch_raycast(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,
-                    hipStream_t s) {
+                    hipStream_t s, const KernelClock* kc) {
   dim3 grid((p.cam.width + 7) / 8, (p.cam.height + 7) / 8);
-  if (p.range) launch_ray_range(One{V}, nullptr, p, 1, s);
+  if (p.range) {
+    tick(kc, s, "ray_range_kernel");
+    launch_ray_range(One{V}, nullptr, p, 1, s);
+  }
+  tick(kc, s, "raycast_kernel");
   raycast_kernel<<<grid, 64, 0, s>>>(One{V}, nullptr, p, depth, vertex, normal);
+  tick(kc, s, nullptr);
+}
+
+// Diagnostics: the distinct blocks the last raycast's lookups found (their
+// ray_mark equals its stamp), added to the work counters.
+__global__ __launch_bounds__(256) void ray_mark_count_kernel(View V) {
+  const uint32_t n = V.ctr->block_hwm;
+  uint32_t c = 0;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+    c += V.ray_mark[p] == V.ray_stamp;
+  wave_add_u64(&work_stripe(V.ctr).ray_blocks, c);
+}
+
+void launch_ray_mark_count(const View& V, hipStream_t s, int grid) {
+  ray_mark_count_kernel<<<grid, 256, 0, s>>>(V);
 }
 
 uint32_t raycast_grad_partials(const RayConst& p) {

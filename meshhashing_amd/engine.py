@@ -100,6 +100,26 @@ class Volume:
         A.check(self.lib.mh_frame_depth_u16_device(self.h, C.c_void_p(dev_ptr), A.f32x16(wTc)),
                 "mh_frame_depth_u16_device")
 
+    def field_ptr(self, field: int = A.MH_FIELD_SDF) -> tuple[int, int]:
+        """(device pointer, number of values) of a voxel field."""
+        p, n = C.c_void_p(), C.c_uint64(0)
+        A.check(self.lib.mh_volume_field_device_ptr(self.h, field, C.byref(p), C.byref(n)),
+                "mh_volume_field_device_ptr")
+        return p.value, n.value
+
+    def set_diagnostics(self, kernel_times: bool = False, ray_blocks: bool = False):
+        flags = (A.MH_DIAG_KERNEL_TIMES if kernel_times else 0) | (A.MH_DIAG_RAY_BLOCKS if ray_blocks else 0)
+        A.check(self.lib.mh_volume_set_diagnostics(self.h, flags), "mh_volume_set_diagnostics")
+
+    def kernel_times(self, reset: bool = True) -> dict:
+        """{kernel: (launches, total ms)} since the last reset (synchronises)."""
+        n = C.c_uint32(0)
+        A.check(self.lib.mh_kernel_times(self.h, 0, None, C.byref(n), 0), "mh_kernel_times")
+        buf = (A.KernelTime * max(1, n.value))()
+        A.check(self.lib.mh_kernel_times(self.h, n.value, buf, C.byref(n), 1 if reset else 0),
+                "mh_kernel_times")
+        return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms) for i in range(n.value)}
+
     def set_stream(self, stream_ptr: int):
         A.check(self.lib.mh_volume_set_stream(self.h, C.c_void_p(stream_ptr)), "set_stream")
 
@@ -187,7 +207,7 @@ class Volume:
                 "mh_raycast_backward_noalloc" if noalloc else "mh_raycast_backward")
         return {"loss": g.loss, "grad": np.array(g.grad[:]),
                 "hessian": np.array(g.hessian[:]).reshape(6, 6), "pixels": g.pixels,
-                "outliers": g.outliers}
+                "outliers": g.outliers, "observed": g.observed}
 
     def refine_pose(self, wTc: np.ndarray, obs_ptr: int, iterations: int = 10,
                     params: A.RayCasterParams | None = None):
@@ -371,6 +391,18 @@ class Volume:
                 "mh_dump_primal_dual")
         return {"sdf0": pdv[:, 0], "sdf_bar": pdv[:, 1], "w_pd": pdv[:, 2], "p": pdv[:, 3:6],
                 "mask": mask}
+
+
+def refine_pose_pyramid(levels, wTc: np.ndarray, obs_ptr: int) -> np.ndarray:
+    """Coarse-to-fine pose refinement: mh_refine_pose on each level in turn,
+    each starting from the previous level's result.  levels: (Volume,
+    RayCasterParams or None, iterations) from coarsest to finest, volumes of
+    the same scene at growing resolution; their wider truncation bands keep a
+    pose that is centimetres off inside the coarse SDFs' valid range."""
+    T = np.asarray(wTc, np.float32)
+    for vol, params, iterations in levels:
+        T, _ = vol.refine_pose(T, obs_ptr, iterations, params=params)
+    return T
 
 
 class MainEngine:
