@@ -710,12 +710,6 @@ int orc_integrate_tsdf(orc_volume* v) {
 int orc_predict_inlier(orc_volume* v) {
   const Geo& g = v->g;
   const Cam& c = v->cam;
-  auto x64 = [&](const double* p) {
-    const double* m = v->cTw64;
-    return std::array<double, 3>{m[0] * p[0] + m[1] * p[1] + m[2] * p[2] + m[3],
-                                 m[4] * p[0] + m[5] * p[1] + m[6] * p[2] + m[7],
-                                 m[8] * p[0] + m[9] * p[1] + m[10] * p[2] + m[11]};
-  };
   for (int ptr : v->cand) {
     Block& B = v->blocks[ptr];
     for (int i = 0; i < 512; ++i) {
@@ -726,27 +720,30 @@ int orc_predict_inlier(orc_volume* v) {
       const uint32_t pix = u + vv * c.w;
       const float depth = v->depth[pix];
       if (!depth_ok(depth, g.ub)) continue;
-      const double pc[3] = {((double)u - c.cx) / c.fx * depth, ((double)vv - c.cy) / c.fy * depth,
-                            (double)depth};
+      // FP32 in the reference's operation order: proj_disk is the square
+      // root of a difference that cancels when vec runs along the normal, so
+      // the ratio is decision-like there (a negative difference gives NaN,
+      // which fmaxf(., 0.01) turns into 0.01), as on the GPU
+      const F3 pc = c.reproj(u, vv, depth);
+      const float sdf = (float)B.sdf[i], is2 = (float)B.w[i];
       for (int ax = 0; ax < 3; ++ax) {
         auto it = v->verts.find(VKey{{vp.x, vp.y, vp.z, ax}});
         if (it == v->verts.end()) continue;
         const Vtx& vt = it->second;
-        const double xp[3] = {vt.pos[0], vt.pos[1], vt.pos[2]};
-        const auto nc0 = x64(vt.normal);
-        const double nl = std::sqrt(nc0[0] * nc0[0] + nc0[1] * nc0[1] + nc0[2] * nc0[2]);
-        const double nc[3] = {nc0[0] / nl, nc0[1] / nl, nc0[2] / nl};
-        const double pl = std::sqrt(pc[0] * pc[0] + pc[1] * pc[1] + pc[2] * pc[2]);
-        const double cos_a = (-pc[0] * nc[0] - pc[1] * nc[1] - pc[2] * nc[2]) / pl;
-        const auto xc = x64(xp);
-        const double vec[3] = {pc[0] - xc[0], pc[1] - xc[1], pc[2] - xc[2]};
-        const double pd = vec[0] * nc[0] + vec[1] * nc[1] + vec[2] * nc[2];
-        const double disk = std::sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2] - pd * pd);
-        const double w_dist = std::exp(-(pd - B.sdf[i]) * (pd - B.sdf[i]) * B.w[i]);
-        const double w_disk = 0.5 + 0.5 / (1.0 + std::exp(-std::fabs(disk / vt.radius)));
-        const double w_ang = std::max((cos_a - 0.1736) / (1.0 - 0.1736), 0.1);
-        const double r = std::max(w_disk * w_dist * w_ang, 0.01);
-        v->inlier[pix] = std::max(v->inlier[pix], r);
+        const F3 n{(float)vt.normal[0], (float)vt.normal[1], (float)vt.normal[2]};
+        const F3 x{vt.pos[0], vt.pos[1], vt.pos[2]};
+        const F3 nc = normalizef(xf(v->cTw, n));  // the normal as a point
+        const float cos_a = dotf(normalizef(F3{-pc.x, -pc.y, -pc.z}), nc);
+        const F3 xc = xf(v->cTw, x);
+        const F3 vec{pc.x - xc.x, pc.y - xc.y, pc.z - xc.z};
+        const float pd = dotf(vec, nc);
+        const float disk = std::sqrt(dotf(vec, vec) - pd * pd);
+        const float dd = pd - sdf;
+        const float w_dist = std::exp(-(dd * dd) * is2);
+        const float w_disk = 0.5f + 0.5f / (1.0f + std::exp(-std::fabs(disk / (float)vt.radius)));
+        const float w_ang = std::fmax((cos_a - 0.1736f) / (1.0f - 0.1736f), 0.1f);
+        const float r = std::fmax(w_disk * w_dist * w_ang, 0.01f);
+        v->inlier[pix] = std::max(v->inlier[pix], (double)r);
       }
     }
   }
@@ -1778,6 +1775,25 @@ int orc_set_vertex(orc_volume* v, const int32_t* key, const float* pos, const do
   }
   vt.radius = radius;
   v->verts[VKey{{key[0], key[1], key[2], key[3]}}] = vt;
+  return 0;
+}
+
+// Replaces the payload (position, normal, colour, radius) of n vertices,
+// keyed like orc_dump_mesh, keeping their reference counts: lockstep tests
+// hand the library's vertices to the predict step.
+int orc_load_vertices(orc_volume* v, uint32_t n, const int32_t* keys, const double* vals) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const int32_t* k = keys + 4 * (size_t)i;
+    if (k[3] < 0 || k[3] > 2) return -1;
+    Vtx& vt = v->verts[VKey{{k[0], k[1], k[2], k[3]}}];
+    const double* d = vals + 10 * (size_t)i;
+    for (int j = 0; j < 3; ++j) {
+      vt.pos[j] = (float)d[j];
+      vt.normal[j] = d[3 + j];
+      vt.color[j] = d[6 + j];
+    }
+    vt.radius = d[9];
+  }
   return 0;
 }
 

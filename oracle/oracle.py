@@ -88,6 +88,7 @@ def lib() -> C.CDLL:
             "orc_dump_cand_flags": (C.c_uint32, [vp, P(C.c_int32)]),
             "orc_val_to_rgb": (None, [C.c_double, C.c_double, C.c_double, P(C.c_double)]),
             "orc_set_threads": (None, [C.c_int]),
+            "orc_load_vertices": (C.c_int, [vp, C.c_uint32, P(C.c_int32), P(C.c_double)]),
             "orc_get_threads": (C.c_int, []),
             "orc_set_vertex": (C.c_int, [vp, P(C.c_int32), P(C.c_float), P(C.c_double),
                                          C.c_double]),
@@ -180,6 +181,13 @@ class Oracle:
         n = np.ascontiguousarray(normal, np.float64)
         assert self.L.orc_set_vertex(self.h, _p(k, C.c_int32), _p(p, C.c_float),
                                      _p(n, C.c_double), float(radius)) == 0
+
+    def load_vertices(self, vkey, vval):
+        """Replace the payloads of the given vertices (dump_mesh layout),
+        keeping their reference counts."""
+        k = np.ascontiguousarray(vkey, np.int32)
+        v = np.ascontiguousarray(vval, np.float64)
+        assert self.L.orc_load_vertices(self.h, len(k), _p(k, C.c_int32), _p(v, C.c_double)) == 0
 
     def pd_init(self):
         self.L.orc_pd_init(self.h)

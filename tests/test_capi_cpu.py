@@ -36,7 +36,8 @@ def test_struct_layouts():
     assert C.sizeof(A.SensorParams) == 36
     assert C.sizeof(A.RayCasterParams) == 48
     assert C.sizeof(A.McParams) == 32
-    assert C.sizeof(A.Status) == 80
+    assert C.sizeof(A.Status) == 88
+    assert C.sizeof(A.RaycastGrad) == 360
 
 
 def test_mc_defaults_are_the_reference_constants():

@@ -64,6 +64,8 @@ def test_raycast_backward_matches_oracle(hip_device, oracle_mod):
     gsdf = vol.dump_voxel_field(gs.data_ptr())
     print(f"[adjoint gpu] pixels {out['pixels']} loss {out['loss']:.6e} vs {ref['loss']:.6e}")
     assert out["pixels"] == ref["pixels"] and out["pixels"] > 20000
+    # every finite positive observation counts as observed, hit or not
+    assert out["observed"] == np.count_nonzero(np.isfinite(obs) & (obs > 0)) > out["pixels"]
     assert abs(out["loss"] - ref["loss"]) <= 1e-4 * ref["loss"]
     ge = np.abs(out["grad"] - ref["grad"]).max() / np.abs(ref["grad"]).max()
     he = np.abs(out["hessian"] - ref["hessian"]).max() / np.abs(ref["hessian"]).max()
@@ -145,3 +147,44 @@ def test_raycast_backward_matches_oracle_at_5mm_640x480(hip_device, oracle_mod):
         assert abs(out["loss"] - ref["loss"]) <= 1e-4 * ref["loss"]
         assert ge < 1e-4 and he < 1e-4 and se < 1e-4
     vol.close()
+
+
+def test_c5_coarse_to_fine_refinement(hip_device):
+    """C5 pose refinement, coarse to fine: 3 Gauss-Newton steps on a 4 cm
+    copy of the room, 3 on a 2 cm copy, 4 on the 5 mm volume, from starts
+    perturbed by N(0, 2 deg) rotation and N(0, 2 cm) translation (the bench's
+    first eight query poses).  Every pose ends within 0.1 deg and 6 mm of the
+    truth: the 5 mm objective's own optimum lies a few millimetres off it
+    (refinement started at the truth moves by that much), so the bar is set
+    above that floor.  The same ten steps on the 5 mm volume alone end far
+    worse, which is what the coarse levels are for."""
+    import torch
+    from dataclasses import replace
+    from meshhashing_amd.configs import C5, C5_PYRAMID, C5_PYRAMID_STEPS
+    from meshhashing_amd.engine import refine_pose_pyramid
+    fine = fused_volume(C5, range(C5.frames))
+    coarse = [fused_volume(c, range(C5.frames)) for c in C5_PYRAMID]
+    levels = [(v, v.cfg.raycaster_params(), n) for v, n in zip(coarse + [fine], C5_PYRAMID_STEPS)]
+    q = replace(C5, frames=32)
+    rng = np.random.default_rng(5)
+    rows = []
+    for i in range(8):
+        gt = synth_pose(q, i).astype(np.float64)
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        xi = np.concatenate([axis * np.radians(rng.normal(0.0, 2.0)), rng.normal(size=3) * 0.02 / np.sqrt(3)])
+        start = se3_exp(xi) @ gt
+        obs = torch.from_numpy((synth_depth(q, i) * np.float32(0.001)).astype(np.float32)).to(hip_device)
+        out = refine_pose_pyramid(levels, start.astype(np.float32), obs.data_ptr())
+        single, _ = fine.refine_pose(start.astype(np.float32), obs.data_ptr(), sum(C5_PYRAMID_STEPS))
+        r0, t0 = pose_error(gt, start)
+        r1, t1 = pose_error(gt, out)
+        r2, t2 = pose_error(gt, single)
+        print(f"[c5 pyramid] pose {i}: {r0:.2f} deg {1000 * t0:.1f} mm -> {r1:.3f} deg {1000 * t1:.2f} mm "
+              f"(5 mm alone: {r2:.3f} deg {1000 * t2:.2f} mm)", flush=True)
+        rows.append((r1, t1, r2, t2))
+    rows = np.array(rows)
+    assert rows[:, 0].max() < 0.1 and rows[:, 1].max() < 0.006
+    assert np.median(rows[:, 1]) < np.median(rows[:, 3])
+    for v in coarse + [fine]:
+        v.close()
