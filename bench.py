@@ -56,26 +56,24 @@ def algorithmic_bytes(stage, c, frames, w, h):
     return None
 
 
-STAGE_KERNEL = {"raycast": "raycast_kernel<One>", "mesh": "mc_surfel_kernel<false, One>"}
+KERNEL_OF_STAGE = {"integrate": "integrate_tsdf_kernel", "mesh": "mc_surfel_kernel",
+                   "raycast": "raycast_kernel"}
 
 
-def measured_traffic(stage, tiling=None):
-    """HBM bytes per launch of the stage's dominant kernel, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the counters' reading
-    on a copy of known size); None if the summary does not cover it."""
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (each scaled by the counter's reading on
+    a copy of known size); None if the summary does not cover it."""
     path = ROOT / "profiles" / "c2_640x480_5mm_traffic.json"
     if not path.exists():
         return None
     prof = json.loads(path.read_text())
-    name = STAGE_KERNEL.get(stage)
-    if stage == "integrate" and tiling:
-        name = f"integrate_tsdf_kernel<One, {tiling[0]}, {tiling[1]}>"
-    k = prof["kernels"].get(name) if name else None
+    k = prof["kernels"].get(kernel)
     if not k or k.get("hbm_bytes") is None:
         return None
-    return {"bytes_per_launch": k["hbm_bytes"], "kernel": name,
+    return {"bytes_per_launch": k["hbm_bytes"], "kernel": kernel,
             "fetch_factor": prof["fetch_factor"], "write_factor": prof["write_factor"],
-            "source": "profiles/c2_640x480_5mm_kernels.md"}
+            "l2_hit_rate": k.get("l2_hit_rate"), "source": "profiles/c2_640x480_5mm_traffic.json"}
 
 
 def host_cores():
@@ -131,6 +129,16 @@ def cpu_baseline(cfg, runs=5):
                            "sample": f"frame {runs + 1} of the same sequence, one run"}}
 
 
+def wall_clock(fn):
+    """Host milliseconds of fn, between two device-wide synchronisations."""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return 1e3 * (time.perf_counter() - t0)
+
+
 def c4_batched(dev, stream, n_scenes=64, n_frames=4):
     """Batched form: n_scenes independent rooms (voxel sizes 1-4 cm), one frame
     of all scenes per step, fused per stage and replayed as a HIP graph,
@@ -181,6 +189,9 @@ def c4_batched(dev, stream, n_scenes=64, n_frames=4):
     c0 = [v.status() for v in bt.volumes]
     ms_b = timed(lambda: batch_pass(True))
     st_b = [v.status() for v in bt.volumes]
+    for v in bt.volumes:
+        v.reset()
+    ms_b_wall = wall_clock(lambda: batch_pass(True))
     upd_b = sum(b["voxel_updates"] - a["voxel_updates"] for a, b in zip(c0, st_b))
     blocks = [s["block_count"] for s in st_b]
     bt.close()
@@ -210,25 +221,31 @@ def c4_batched(dev, stream, n_scenes=64, n_frames=4):
         v.close()
 
     # the same scene-frames one scene after another, each scene replaying its
-    # own one-scene frame graph with its own tuned integration tiling: the
-    # batched form's gain over the plain path with the same launch machinery
+    # own one-scene frame graph with the batch's tuned integration tiling:
+    # the batched form's gain over the plain path with the same launch
+    # machinery
+    # (each batch keeps its own stream: several batches capturing and
+    # replaying graphs on one shared stream were refused by the runtime, so
+    # both this leg and the batched one are timed on the host clock around a
+    # device-wide synchronisation)
     ones = [Batch([c]) for c in cfgs]
     for s, b1 in enumerate(ones):
-        b1.set_stream(stream.cuda_stream)
         b1.set_raycast(rd.data_ptr(), 0, rn.data_ptr())
+        if tune is not None:
+            b1.set_integrate_tiling(tune.tiling, tune.workgroups_per_cu)
 
-    def seq_graph_pass(tune_first=False):
+    def seq_graph_pass():
         for s, b1 in enumerate(ones):
             for f in range(n_frames):
                 b1.frame(depth_dev.data_ptr() + f * fbytes + s * sbytes, poses[f][s][None])
-                b1.run(mesh=True, raycast=True, graph=not (tune_first and f == 0),
-                       tune=tune_first and f == 0)
+                b1.run(mesh=True, raycast=True, graph=True)
+            b1.synchronize()  # one scene after another
 
-    seq_graph_pass(tune_first=True)
+    seq_graph_pass()
     torch.cuda.synchronize()
     for b1 in ones:
         b1.volumes[0].reset()
-    ms_g = timed(seq_graph_pass)
+    ms_g = wall_clock(seq_graph_pass)
     for b1 in ones:
         b1.close()
     sf = n_scenes * n_frames
@@ -241,13 +258,15 @@ def c4_batched(dev, stream, n_scenes=64, n_frames=4):
         "batched_ms_per_step": ms_b / n_frames,
         "scene_frames_per_s_batched": sf / (ms_b * 1e-3),
         "scene_frames_per_s_sequential": sf / (ms_g * 1e-3),
-        "speedup_per_scene": ms_g / ms_b,
+        "speedup_per_scene": ms_g / ms_b_wall,
+        "timing": "speedup_per_scene: host clock around device-wide synchronisations for both "
+                  "legs; batched_ms_per_step: HIP events",
         "scene_frames_per_s_sequential_eager": sf / (ms_s * 1e-3),
         "speedup_vs_eager_default_tiling": ms_s / ms_b,
         "voxel_updates_per_s_batched": upd_b / (ms_b * 1e-3),
         "mode": "per-frame chain replayed as one HIP graph, integration tiling autotuned; "
-                "sequential: each scene alone, replaying its own one-scene frame graph with its "
-                "own tuned tiling (and, as _eager, through the single-volume entry points with "
+                "sequential: each scene alone, replaying its own one-scene frame graph with the "
+                "same tuned tiling (and, as _eager, through the single-volume entry points with "
                 "the default tiling)",
         "tuned_tiling": None if tune is None else {
             "workgroup_size": tune.workgroup_size, "blocks_per_workgroup": tune.blocks_per_workgroup,
@@ -280,19 +299,25 @@ def c5_pose_refinement(dev, stream, n_poses=32, iterations=10, grad_check=True):
     import torch
     from dataclasses import replace
 
-    from meshhashing_amd.configs import C5
-    from meshhashing_amd.engine import Volume, synth_depth, synth_pose
+    from meshhashing_amd.configs import C5, C5_PYRAMID, C5_PYRAMID_STEPS
+    from meshhashing_amd.engine import Volume, refine_pose_pyramid, synth_depth, synth_pose
 
+    # the 5 mm volume and its coarse copies (4 cm, 2 cm) of the same frames
     vol = Volume(C5)
-    vol.set_stream(stream.cuda_stream)
+    coarse = [Volume(c) for c in C5_PYRAMID]
+    for v_ in [vol] + coarse:
+        v_.set_stream(stream.cuda_stream)
     for f in range(C5.frames):
-        vol.upload_depth(synth_depth(C5, f), synth_pose(C5, f))
-        vol.alloc()
-        vol.collect_visible()
-        vol.integrate_tsdf()
+        for v_ in [vol] + coarse:
+            v_.upload_depth(synth_depth(v_.cfg, f), synth_pose(v_.cfg, f))
+            v_.alloc()
+            v_.collect_visible()
+            v_.integrate_tsdf()
+    levels = [(v_, v_.cfg.raycaster_params(), n)
+              for v_, n in zip(coarse + [vol], C5_PYRAMID_STEPS)]
     q = replace(C5, frames=n_poses)
     rng = np.random.default_rng(5)
-    err0, err1, ms = [], [], []
+    err0, err1, err_single, err_floor, ms = [], [], [], [], []
     loc = {"reference": ([], []), "corrected": ([], [])}
     first = None
     for i in range(n_poses):
@@ -305,8 +330,11 @@ def c5_pose_refinement(dev, stream, n_poses=32, iterations=10, grad_check=True):
         obs = torch.from_numpy(obs_np).to(dev)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        out, trace = vol.refine_pose(start.astype(np.float32), obs.data_ptr(), iterations)
+        out = refine_pose_pyramid(levels, start.astype(np.float32), obs.data_ptr())
         ms.append(1e3 * (time.perf_counter() - t0))
+        single, _ = vol.refine_pose(start.astype(np.float32), obs.data_ptr(), iterations)
+        # the 5 mm objective's own optimum: refinement started at the truth
+        floor, _ = vol.refine_pose(gt.astype(np.float32), obs.data_ptr(), iterations)
         # the reference's own solver from the same start: point-to-PSDF on
         # the observed frame, 20 steps at lambda 1e5 with its Jacobian
         # (MainEngine::Localizing as app/slam.cc calls it), and the
@@ -319,7 +347,8 @@ def c5_pose_refinement(dev, stream, n_poses=32, iterations=10, grad_check=True):
             found[key] = vol.localize(start.astype(np.float32), iterations=20, lam=lam,
                                       corrected=corr)["wTc"]
             loc[key][1].append(1e3 * (time.perf_counter() - t0))
-        for lst, T in ((err0, start), (err1, out), (loc["reference"][0], found["reference"]),
+        for lst, T in ((err0, start), (err1, out), (err_single, single), (err_floor, floor),
+                       (loc["reference"][0], found["reference"]),
                        (loc["corrected"][0], found["corrected"])):
             D = np.linalg.inv(gt) @ T.astype(np.float64)
             ang = np.degrees(np.arccos(np.clip((np.trace(D[:3, :3]) - 1) / 2, -1, 1)))
@@ -328,11 +357,28 @@ def c5_pose_refinement(dev, stream, n_poses=32, iterations=10, grad_check=True):
             first = (np.linalg.inv(start).astype(np.float32), obs_np, obs)
     res = {
         "workload": f"{n_poses} poses, 640x480 depth, 5 mm volume of the 320x240 room "
-                    f"({vol.status()['block_count']} blocks), {iterations} Gauss-Newton steps each",
+                    f"({vol.status()['block_count']} blocks), {sum(C5_PYRAMID_STEPS)} Gauss-Newton steps "
+                    f"each (coarse to fine)",
         "initial_error_deg_mm_median": [float(np.median([e[0] for e in err0])), float(np.median([e[1] for e in err0]))],
         "final_error_deg_mm_median": [float(np.median([e[0] for e in err1])), float(np.median([e[1] for e in err1]))],
         "final_error_deg_mm_max": [float(max(e[0] for e in err1)), float(max(e[1] for e in err1))],
+        "poses_worse_than_start": int(sum(e1[0] > e0[0] or e1[1] > e0[1] for e0, e1 in zip(err0, err1))),
+        "poses_worse_than_start_by_0.05deg_or_1mm": int(sum(e1[0] > e0[0] + 0.05 or e1[1] > e0[1] + 1.0
+                                                            for e0, e1 in zip(err0, err1))),
         "ms_per_pose": float(np.median(ms)),
+        "schedule": "Gauss-Newton steps per level (4 cm, 2 cm, 5 mm copies of the room): "
+                    + ", ".join(map(str, C5_PYRAMID_STEPS)),
+        "objective_floor_5mm": {
+            "what": "10 steps on the 5 mm volume started at the true pose: where its own optimum lies",
+            "final_error_deg_mm_median": [float(np.median([e[0] for e in err_floor])),
+                                          float(np.median([e[1] for e in err_floor]))],
+            "final_error_deg_mm_max": [float(max(e[0] for e in err_floor)),
+                                       float(max(e[1] for e in err_floor))]},
+        "single_level_5mm": {
+            "final_error_deg_mm_median": [float(np.median([e[0] for e in err_single])),
+                                          float(np.median([e[1] for e in err_single]))],
+            "final_error_deg_mm_max": [float(max(e[0] for e in err_single)),
+                                       float(max(e[1] for e in err_single))]},
     }
     for key, (errs, times) in loc.items():
         res[f"point_to_psdf_{key}"] = {
@@ -354,6 +400,8 @@ def c5_pose_refinement(dev, stream, n_poses=32, iterations=10, grad_check=True):
                                                         np.abs(ref["grad"]).max())
         res["grad_check_pixels"] = [g["pixels"], ref["pixels"]]
     vol.close()
+    for v_ in coarse:
+        v_.close()
     return res
 
 
@@ -443,6 +491,10 @@ def main():
     for f in range(args.warmup):
         run_frame(f, False)
     torch.cuda.synchronize()
+    # per-kernel HIP events and the raycast's sampled-block count for the
+    # timed frames (the graph leg below runs without them)
+    vol.set_diagnostics(kernel_times=True, ray_blocks=True)
+    vol.kernel_times(reset=True)
     c0 = vol.status()
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
@@ -453,31 +505,51 @@ def main():
     torch.cuda.synchronize()
     ms_eager = t_start.elapsed_time(t_end)
     c1 = vol.status()
+    ktimes = vol.kernel_times(reset=True)
     cnt = {k: c1[k] - c0[k] for k in ("voxel_updates", "blocks_integrated", "mc_vertex_writes",
-                                      "mc_triangle_writes", "mc_blocks", "ray_samples")}
+                                      "mc_triangle_writes", "mc_blocks", "ray_samples",
+                                      "ray_blocks")}
     stage_ms = {s: sum(a.elapsed_time(b) for a, b in ev[s]) for s in stages} if args.stage_events else {}
     st = vol.status()
     vol.close()
 
+    # roofline per kernel: the stage's algorithmic bytes per launch over the
+    # kernel's own average duration (HIP events on the volume's stream)
+    kernels = {}
+    for stage, name in KERNEL_OF_STAGE.items():
+        if name not in ktimes:
+            continue
+        launches, tot = ktimes[name]
+        b = algorithmic_bytes(stage, cnt, K, W, H)
+        avg = tot / launches
+        row = {"launches": launches, "avg_us": 1e3 * avg}
+        if b is not None:
+            bpl = b / launches
+            row.update({"algorithmic_bytes_per_launch": bpl, "GB/s": bpl / (avg * 1e-3) / 1e9,
+                        "frac": bpl / (avg * 1e-3) / HBM_PEAK})
+        kernels[name] = row
+    other = {n: {"launches": l, "avg_us": 1e3 * t / l} for n, (l, t) in ktimes.items()
+             if n not in kernels}
     roof = None
+    priced = [n for n in kernels if "frac" in kernels[n]]
+    if priced:
+        dom = max(priced, key=lambda n: kernels[n]["avg_us"])
+        k = kernels[dom]
+        tr = measured_traffic(dom)
+        roof = {"bound": "hbm", "kernel": dom, "achieved": k["GB/s"], "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": k["frac"],
+                "traffic": None if tr is None else tr["bytes_per_launch"],
+                "traffic_source": tr, "algorithmic_bytes_per_launch": k["algorithmic_bytes_per_launch"],
+                "avg_kernel_us": k["avg_us"], "kernels": kernels, "other_kernels": other}
     if stage_ms:
-        dom = max(("integrate", "mesh", "raycast"), key=lambda s: stage_ms[s])
         rows = {}
         for s in ("integrate", "mesh", "raycast"):
             b = algorithmic_bytes(s, cnt, K, W, H)
             rows[s] = {"ms_per_frame": stage_ms[s] / K,
                        "GB/s": None if b is None else b / (stage_ms[s] * 1e-3) / 1e9,
                        "frac": None if b is None else b / (stage_ms[s] * 1e-3) / HBM_PEAK}
-        dom = max((s for s in rows if rows[s]["GB/s"] is not None), key=lambda s: stage_ms[s])
-        b = algorithmic_bytes(dom, cnt, K, W, H)
-        ach = b / (stage_ms[dom] * 1e-3)
-        tr = measured_traffic(dom, None if tune is None else (tune.workgroup_size,
-                                                              tune.blocks_per_workgroup))
-        roof = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": ach / HBM_PEAK,
-                "traffic": None if tr is None else tr["bytes_per_launch"],
-                "traffic_source": tr, "algorithmic_bytes_per_launch": b / K,
-                "stages": rows}
+        if roof is not None:
+            roof["stages"] = rows
 
     # ---- 2. the same frames as one HIP graph per frame (a one-scene batch)
     bt = Batch([cfg])
