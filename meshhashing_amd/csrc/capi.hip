@@ -1280,6 +1280,7 @@ struct mh_batch {
   int ring = 0;
   uint32_t* d_work = nullptr;
   uint32_t* d_work_count = nullptr;
+  uint32_t* d_dyn = nullptr;       // marching cubes' on-demand item counter
   uint16_t* d_raw = nullptr;       // staging for host depth uploads
   uint32_t* d_range = nullptr;     // raycast tile ranges, ray_range_words(W, H) per scene
   int32_t* d_nbt = nullptr;        // marching cubes: 27 neighbours per concatenated-worklist item
@@ -1294,7 +1295,7 @@ struct mh_batch {
   hipGraphExec_t exec = nullptr;
   uint32_t graph_flags = 0;
 
-  BatchDev dev() const { return BatchDev{d_frames, d_work, d_work_count, n, W, H, 0}; }
+  BatchDev dev() const { return BatchDev{d_frames, d_work, d_work_count, d_dyn, n, W, H, 0}; }
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1312,7 +1313,7 @@ void batch_free(mh_batch* b) {
     if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
     if (b->pinned[k]) (void)hipHostFree(b->pinned[k]);
   }
-  for (void* p : {(void*)b->d_frames, (void*)b->d_work, (void*)b->d_work_count, (void*)b->d_raw,
+  for (void* p : {(void*)b->d_frames, (void*)b->d_work, (void*)b->d_work_count, (void*)b->d_dyn, (void*)b->d_raw,
                   (void*)b->d_range, (void*)b->d_nbt})
     if (p) (void)hipFree(p);
   for (mh_volume* v : b->vols) mh_volume_destroy(v);
@@ -1461,7 +1462,8 @@ int mh_batch_create(uint32_t n_scenes, const mh_hash_params* hash, const mh_volu
   }
   const size_t npix = (size_t)b->W * b->H * n_scenes;
   if (dalloc(&b->d_frames, n_scenes) != hipSuccess || dalloc(&b->d_work, work_cap) != hipSuccess ||
-      dalloc(&b->d_work_count, 1) != hipSuccess || dalloc(&b->d_raw, npix) != hipSuccess ||
+      dalloc(&b->d_work_count, 1) != hipSuccess || dalloc(&b->d_dyn, 1) != hipSuccess ||
+      dalloc(&b->d_raw, npix) != hipSuccess ||
       dalloc(&b->d_nbt, work_cap * 27) != hipSuccess)
     return bail(fail(MH_ERR_HIP, "device allocation failed"));
   for (int k = 0; k < mh_batch::kRing; ++k) {

@@ -33,8 +33,12 @@ struct Shell {
   float w[kShellN];
   float a[PSDF ? kShellN : 1];
   float b[PSDF ? kShellN : 1];
-  uint8_t ok[kShellN];
+  uint8_t f[kShellN];  // kShOk | kShCube | kShNeg per voxel
 };
+// Shell voxel flags: the voxel exists (its block is allocated); it passes
+// the cube tests of a corner (|sdf| <= threshold, rho >= rho_min,
+// weight >= weight_min); its SDF is negative.
+constexpr uint8_t kShOk = 1, kShCube = 2, kShNeg = 4;
 
 __device__ __forceinline__ int sidx(int x, int y, int z) {
   return (x + 1) + kShell * (y + 1) + kShell * kShell * (z + 1);
@@ -55,6 +59,14 @@ __device__ __forceinline__ const McConst& mc_of(const Many& m, uint32_t s, const
   return m.f[s].mc;
 }
 constexpr uint32_t kLookupLane = kBlockVoxels - 64;  // the surfel pass's last wave
+// The surfel pass hands out its first items statically (workgroup g takes
+// g, g + grid, ...) and the rest, about a quarter, on demand from a counter,
+// so workgroups whose blocks held little surface take more of them.  Items
+// below this bound are static; the counter starts there.
+__device__ __forceinline__ uint32_t mc_static_items(uint32_t n, uint32_t grid) {
+  const uint32_t k = (3u * n / 4u) / grid;
+  return (k ? k : 1u) * grid;
+}
 constexpr int32_t kMeshedBit = 1 << 30;
 template <class Src>
 __global__ __launch_bounds__(256) void mc_neighbours_kernel(Src src_, const SceneFrame* __restrict__ tab,
@@ -68,7 +80,7 @@ __global__ __launch_bounds__(256) void mc_neighbours_kernel(Src src_, const Scen
     const View& V = src.view(scene);
     const i3 b = unpack_key(V.block_keys[self]);
     const int dx = (int)(t % 3) - 1, dy = (int)((t / 3) % 3) - 1, dz = (int)(t / 9) - 1;
-    const int32_t p = (dx | dy | dz) == 0 ? self : hash_find(V, i3{b.x + dx, b.y + dy, b.z + dz});
+    const int32_t p = (dx | dy | dz) == 0 ? self : hash_find_eager(V, i3{b.x + dx, b.y + dy, b.z + dz});
     int32_t e = kFreePtr;
     if (p >= 0) e = p | (V.block_mark[p] == mc_of(src, scene, mc1).pass ? kMeshedBit : 0);
     nbt[i] = e;
@@ -78,7 +90,8 @@ __global__ __launch_bounds__(256) void mc_neighbours_kernel(Src src_, const Scen
 // nbe: the lane's neighbour-table entry for this block (lanes
 // kLookupLane + t, t < 27).
 template <bool PSDF>
-__device__ void stage_shell(const View& V, int32_t nbe, Shell<PSDF>& S, int32_t* nb, uint8_t* meshed) {
+__device__ void stage_shell(const View& V, const McConst& mc, int32_t nbe, Shell<PSDF>& S, int32_t* nb,
+                            uint8_t* meshed) {
   const uint32_t t = threadIdx.x - kLookupLane;
   if (t < 27) {
     nb[t] = nbe < 0 ? nbe : (nbe & (kMeshedBit - 1));
@@ -116,7 +129,12 @@ __device__ void stage_shell(const View& V, int32_t nbe, Shell<PSDF>& S, int32_t*
   for (int r = 0; r < kPer; ++r) {
     const int s = (int)threadIdx.x + 512 * r;
     if (s >= kShellN) continue;
-    S.ok[s] = ok[r] ? 1 : 0;
+    // the corner tests of IsValid (marching_cubes.cu:96-130), once per
+    // voxel here instead of once per cube corner
+    const float rho = PSDF ? va[r] / (va[r] + vb[r]) : 0.0f;
+    const bool cube_ok = ok[r] && !(fabsf(sd[r]) > mc.sdf_threshold) && (!PSDF || !(rho < mc.rho_min)) &&
+                         !(wt[r] < mc.weight_min);
+    S.f[s] = (uint8_t)((ok[r] ? kShOk : 0) | (cube_ok ? kShCube : 0) | (sd[r] < 0.0f ? kShNeg : 0));
     S.sdf[s] = sd[r];
     S.w[s] = wt[r];
     if (PSDF) {
@@ -168,7 +186,7 @@ __device__ __forceinline__ bool shell_value(const Shell<PSDF>& S, const SAxis& X
   for (int i = 0; i < 8; ++i) {
     const int mx = (i >> 2) & 1, my = (i >> 1) & 1, mz = i & 1;
     const int s = sidx(cx[mx], cy[my], cz[mz]);
-    ok = ok && S.ok[s];
+    ok = ok && (S.f[s] & kShOk);
     const float w = (mx ? rx : 1.0f - rx) * (my ? ry : 1.0f - ry) * (mz ? rz : 1.0f - rz);
     sdf += w * S.sdf[s];
     if (FULL) {
@@ -271,24 +289,20 @@ __device__ void vertex_payload(const Shell<PSDF>& S, const Geo& g, i3 base, f3 p
 // 96-130): every corner must exist, |sdf| <= threshold, rho >= rho_min and
 // weight >= weight_min; returns validity and the cube index.
 template <bool PSDF>
-__device__ __forceinline__ bool cube_index(const Shell<PSDF>& S, int x, int y, int z,
-                                           const McConst& mc, int* cube) {
-  // All eight corners are read and tested without early exits, so the lanes
-  // of a wave do not diverge on which corner fails first.
+__device__ __forceinline__ bool cube_index(const Shell<PSDF>& S, int x, int y, int z, int* cube) {
+  // The corners' flags (tested once per voxel when the shell was staged),
+  // all eight read without early exits, so the lanes of a wave do not
+  // diverge on which corner fails first.
   int c = 0;
-  bool valid = true;
+  uint32_t all = kShCube;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const int s = sidx(x + corner_off(k, 0), y + corner_off(k, 1), z + corner_off(k, 2));
-    const float d = S.sdf[s];
-    const float a = PSDF ? S.a[s] : 0.0f, bb = PSDF ? S.b[s] : 0.0f;
-    const float rho = a / (a + bb);
-    valid = valid && S.ok[s] && !(fabsf(d) > mc.sdf_threshold) && !(rho < mc.rho_min) &&
-            !(S.w[s] < mc.weight_min);
-    if (d < 0.0f) c |= 1 << k;
+    const uint32_t fl = S.f[sidx(x + corner_off(k, 0), y + corner_off(k, 1), z + corner_off(k, 2))];
+    all &= fl;
+    c |= (int)((fl >> 2) & 1u) << k;
   }
   *cube = c;
-  return valid;
+  return all != 0;
 }
 
 // Cubes at local coordinates -1..8 (10^3), for the writer rule below.
@@ -528,7 +542,12 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
   const uint32_t nb_lane = threadIdx.x - kLookupLane;
   int32_t nbe = kFreePtr;
   if (nb_lane < 27 && blockIdx.x < n) nbe = nbt[(size_t)blockIdx.x * 27u + nb_lane];
-  for (uint32_t ci = blockIdx.x; ci < n; ci += gridDim.x) {
+  const uint32_t n_static = mc_static_items(n, gridDim.x);
+  __shared__ uint32_t s_next;
+  for (uint32_t ci = blockIdx.x; ci < n;) {
+    // the next item, read by every lane after the shell's barriers and
+    // rewritten only after the next barrier
+    if (threadIdx.x == 0) s_next = ci + gridDim.x < n_static ? ci + gridDim.x : atomicAdd(src.next_ctr(), 1u);
     uint32_t scene;
     const int32_t ptr = src.item(ci, &scene);
     if (scene != cur_scene) {
@@ -542,7 +561,8 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
     const uint32_t i = threadIdx.x;
     const size_t idx = (size_t)ptr * kBlockVoxels + i;
     const uint32_t prev = V.cube[idx] & 255u;  // in flight with the shell loads
-    stage_shell<PSDF>(V, nbe, S, nb, meshed);
+    stage_shell<PSDF>(V, mc, nbe, S, nb, meshed);
+    const uint32_t next = s_next;
     if (threadIdx.x == 0) qn = 0;
     __syncthreads();
     // every cube of the 10^3 halo, the block's own among them: validity for
@@ -552,7 +572,7 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
       const int dx = x < 0 ? -1 : (x >= 8 ? 1 : 0), dy = y < 0 ? -1 : (y >= 8 ? 1 : 0),
                 dz = z < 0 ? -1 : (z >= 8 ? 1 : 0);
       int c;
-      const bool valid = cube_index(S, x, y, z, mc, &c);
+      const bool valid = cube_index(S, x, y, z, &c);
       cv[s] = meshed[nb27(dx, dy, dz)] && valid;
       if ((dx | dy | dz) == 0) own_cube[x + 8 * y + 64 * z] = (uint8_t)(valid ? c : 0);
     }
@@ -652,7 +672,7 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
       }
     }
     __syncthreads();
-    if (nb_lane < 27 && ci + gridDim.x < n) nbe = nbt[(size_t)(ci + gridDim.x) * 27u + nb_lane];
+    if (nb_lane < 27 && next < n) nbe = nbt[(size_t)next * 27u + nb_lane];
     const uint32_t nq = qn < kQueue ? qn : kQueue;
     // payloads, eight lanes per vertex, on the waves before the last one
     // (which looks up the next block's neighbours meanwhile)
@@ -670,15 +690,17 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
       }
       __syncthreads();
     }
+    ci = next;
   }
   if (cur_scene != ~0u) cnt.flush(src.view(cur_scene));
 }
 
 // Meshed-in-this-pass marks for the writer rule.
 template <class Src>
-__global__ void mc_mark_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1) {
+__global__ void mc_mark_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1, uint32_t surfel_grid) {
   const Src src = with_table(src_, tab);
   const uint32_t n = src.count();
+  if (blockIdx.x == 0 && threadIdx.x == 0) *src.next_ctr() = mc_static_items(n, surfel_grid);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     uint32_t scene;
     const int32_t ptr = src.lane_item(i, &scene);
@@ -1057,17 +1079,17 @@ int resident_grid(int cus) {
 
 void launch_marching_cubes(const View& V, const McConst& p, int32_t* nbt, hipStream_t s, int cus,
                            const KernelClock* kc) {
+  const int grid = V.va ? resident_grid<mc_surfel_kernel<true, One>, kBlockVoxels>(cus)
+                        : resident_grid<mc_surfel_kernel<false, One>, kBlockVoxels>(cus);
   tick(kc, s, "mc_mark_kernel");
-  mc_mark_kernel<<<cus, 256, 0, s>>>(One{V}, nullptr, p);
+  mc_mark_kernel<<<cus, 256, 0, s>>>(One{V}, nullptr, p, (uint32_t)grid);
   tick(kc, s, "mc_neighbours_kernel");
   mc_neighbours_kernel<<<cus * 8, 256, 0, s>>>(One{V}, nullptr, p, nbt);
   tick(kc, s, "mc_surfel_kernel");
   if (V.va)
-    mc_surfel_kernel<true><<<resident_grid<mc_surfel_kernel<true, One>, kBlockVoxels>(cus), kBlockVoxels, 0, s>>>(
-        One{V}, nullptr, p, nbt);
+    mc_surfel_kernel<true><<<grid, kBlockVoxels, 0, s>>>(One{V}, nullptr, p, nbt);
   else
-    mc_surfel_kernel<false><<<resident_grid<mc_surfel_kernel<false, One>, kBlockVoxels>(cus), kBlockVoxels, 0,
-                              s>>>(One{V}, nullptr, p, nbt);
+    mc_surfel_kernel<false><<<grid, kBlockVoxels, 0, s>>>(One{V}, nullptr, p, nbt);
   tick(kc, s, "mc_recycle_vertices_kernel");
   mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<One>, 256>(cus), 256, 0, s>>>(One{V},
                                                                                                    nullptr);
@@ -1081,14 +1103,14 @@ void launch_gc_recycle_vertices(const View& V, hipStream_t s, int cus) {
 
 void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus) {
   const McConst unused{};
-  mc_mark_kernel<<<cus, 256, 0, s>>>(B.src(), B.frames, unused);
+  const int grid = B.psdf ? resident_grid<mc_surfel_kernel<true, Many>, kBlockVoxels>(cus)
+                          : resident_grid<mc_surfel_kernel<false, Many>, kBlockVoxels>(cus);
+  mc_mark_kernel<<<cus, 256, 0, s>>>(B.src(), B.frames, unused, (uint32_t)grid);
   mc_neighbours_kernel<<<cus * 8, 256, 0, s>>>(B.src(), B.frames, unused, nbt);
   if (B.psdf)
-    mc_surfel_kernel<true><<<resident_grid<mc_surfel_kernel<true, Many>, kBlockVoxels>(cus), kBlockVoxels, 0,
-                             s>>>(B.src(), B.frames, unused, nbt);
+    mc_surfel_kernel<true><<<grid, kBlockVoxels, 0, s>>>(B.src(), B.frames, unused, nbt);
   else
-    mc_surfel_kernel<false><<<resident_grid<mc_surfel_kernel<false, Many>, kBlockVoxels>(cus), kBlockVoxels, 0,
-                              s>>>(B.src(), B.frames, unused, nbt);
+    mc_surfel_kernel<false><<<grid, kBlockVoxels, 0, s>>>(B.src(), B.frames, unused, nbt);
   mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<Many>, 256>(cus), 256, 0, s>>>(
       B.src(), B.frames);
 }

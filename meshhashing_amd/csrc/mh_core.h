@@ -241,6 +241,7 @@ struct Counters {
   uint32_t dda_cap;
   uint32_t list_inserts;
   uint32_t gc_free_count;  // garbage blocks left with no vertex (to be freed)
+  uint32_t mc_next;        // marching cubes: next worklist item handed out on demand
   WorkStripe work[kWorkStripes];
 };
 
@@ -356,13 +357,17 @@ __device__ __forceinline__ bool occ_test(const uint32_t* occ, uint64_t key) {
 // placed in the first free slot of its bucket and the tables run at a few
 // keys per thousand buckets, so slot 0 is tried alone first (one 16-byte
 // load); the other nine slots are then loaded together (one round trip).
+// kSlot0Done: the caller has compared slot 0 already.
+template <bool kSlot0Done = false>
 __device__ __forceinline__ int32_t hash_find_key(const View& V, i3 p, uint64_t key) {
   const uint32_t first = hash_bucket(p, V.bucket_count) * V.bucket_size;
   const uint32_t last = first + V.bucket_size - 1;
   uint32_t last_offset;
   if (V.bucket_size == 10) {
-    const Entry e0 = V.entries[first];
-    if (e0.key == key) return e0.ptr;
+    if (!kSlot0Done) {
+      const Entry e0 = V.entries[first];
+      if (e0.key == key) return e0.ptr;
+    }
     Entry e[9];
 #pragma unroll
     for (int j = 0; j < 9; ++j) e[j] = V.entries[first + 1 + j];
@@ -376,7 +381,7 @@ __device__ __forceinline__ int32_t hash_find_key(const View& V, i3 p, uint64_t k
       return found;
     last_offset = e[8].offset;
   } else {
-    for (uint32_t j = 0; j < V.bucket_size; ++j) {
+    for (uint32_t j = kSlot0Done ? 1u : 0u; j < V.bucket_size; ++j) {
       const Entry e = V.entries[first + j];
       if (e.key == key) return e.ptr;
     }
@@ -398,6 +403,21 @@ __device__ __forceinline__ int32_t hash_find(const View& V, i3 p) {
   const uint64_t key = pack_key(p);
   if (!occ_test(V.occ, key)) return kFreePtr;
   return hash_find_key(V, p, key);
+}
+
+// hash_find for latency-bound callers: the occupancy word and the bucket's
+// first slot are loaded together (one round trip instead of two for a key
+// in slot 0, the usual place), at the cost of a wasted slot load for an
+// absent key.  Same result as hash_find.
+__device__ __forceinline__ int32_t hash_find_eager(const View& V, i3 p) {
+  const uint64_t key = pack_key(p);
+  const uint32_t bit = occ_bit(key);
+  const uint32_t first = hash_bucket(p, V.bucket_count) * V.bucket_size;
+  const uint32_t word = V.occ[bit >> 5];
+  const Entry e0 = V.entries[first];
+  if (!((word >> (bit & 31u)) & 1u)) return kFreePtr;
+  if (e0.key == key) return e0.ptr;
+  return hash_find_key<true>(V, p, key);
 }
 
 __device__ __forceinline__ void flag(const View& V, uint32_t bit) {

@@ -55,6 +55,7 @@ struct One {
   View V;
   __device__ __forceinline__ const View& view(uint32_t) const { return V; }
   __device__ __forceinline__ uint32_t count() const { return V.ctr->cand_count; }
+  __device__ __forceinline__ uint32_t* next_ctr() const { return &V.ctr->mc_next; }
   __device__ __forceinline__ int32_t item(uint32_t ci, uint32_t* scene) const {
     *scene = 0;
     return V.cand[ci];
@@ -68,8 +69,10 @@ struct Many {
   const SceneFrame* __restrict__ f;
   const uint32_t* __restrict__ work;
   const uint32_t* work_count;
+  uint32_t* dyn;  // items handed out on demand (marching cubes)
   __device__ __forceinline__ const View& view(uint32_t s) const { return f[s].V; }
   __device__ __forceinline__ uint32_t count() const { return *work_count; }
+  __device__ __forceinline__ uint32_t* next_ctr() const { return dyn; }
   __device__ __forceinline__ int32_t item(uint32_t ci, uint32_t* scene) const {
     const uint32_t w = __builtin_amdgcn_readfirstlane(work[ci]);  // ci is wave-uniform
     *scene = w >> kSceneShift;
@@ -104,7 +107,7 @@ struct Listed {
 __device__ __forceinline__ One with_table(const One& o, const SceneFrame*) { return o; }
 __device__ __forceinline__ Listed with_table(const Listed& o, const SceneFrame*) { return o; }
 __device__ __forceinline__ Many with_table(const Many& m, const SceneFrame* t) {
-  return Many{t, m.work, m.work_count};
+  return Many{t, m.work, m.work_count, m.dyn};
 }
 inline const SceneFrame* table_of(const One&) { return nullptr; }
 inline const SceneFrame* table_of(const Listed&) { return nullptr; }
@@ -115,10 +118,11 @@ struct BatchDev {
   SceneFrame* frames;   // device table, nscenes entries
   uint32_t* work;       // concatenated worklist
   uint32_t* work_count;
+  uint32_t* dyn;        // marching cubes' on-demand item counter
   uint32_t nscenes;
   uint32_t width, height;  // common image size
   int psdf;             // scenes carry PSDF fields (all or none)
-  Many src() const { return Many{frames, work, work_count}; }
+  Many src() const { return Many{frames, work, work_count, dyn}; }
 };
 
 // Integration tilings: workgroup size x blocks per workgroup (each block

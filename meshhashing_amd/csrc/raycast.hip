@@ -43,14 +43,14 @@ __device__ __forceinline__ void wave_table_init(WaveTable* t) {
 __device__ __forceinline__ int32_t table_find(const View& V, WaveTable* t, i3 b) {
   const int lim = 1 << 15;
   if (b.x < -lim || b.x >= lim || b.y < -lim || b.y >= lim || b.z < -lim || b.z >= lim)
-    return coord_ok(b) ? hash_find(V, b) : kFreePtr;
+    return coord_ok(b) ? hash_find_eager(V, b) : kFreePtr;
   const uint32_t slot = (uint32_t)(b.x & 7) | ((uint32_t)(b.y & 7) << 3) | ((uint32_t)(b.z & 3) << 6);
   const uint64_t tag = (uint64_t)((uint32_t)(b.x >> 3) & 0x1FFFu) |
                        ((uint64_t)((uint32_t)(b.y >> 3) & 0x1FFFu) << 13) |
                        ((uint64_t)((uint32_t)(b.z >> 2) & 0x3FFFu) << 26);
   const uint64_t e = t->e[slot];
   if (e != 0ull && (e >> 24) == tag) return (int32_t)(e & 0xFFFFFFu) - 3;
-  const int32_t p = coord_ok(b) ? hash_find(V, b) : kFreePtr;
+  const int32_t p = coord_ok(b) ? hash_find_eager(V, b) : kFreePtr;
   if (V.ray_mark && p >= 0) V.ray_mark[p] = V.ray_stamp;  // diagnostics: block sampled
   if (p >= (1 << 24) - 3) return p;  // does not fit the entry
   const uint32_t lane = threadIdx.x & 63u;
