@@ -741,7 +741,10 @@ static_assert(edge_with(0, 0, 0, 0) >= 0 && edge_with(0, 1, 1, 0) >= 0 && edge_w
                   edge_with(1, 1, 0, 2) >= 0,
               "every owner offset across an edge's axis names an edge");
 template <class Src>
-__global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, const SceneFrame* __restrict__ tab) {
+// nbt: the surfel pass's neighbour table of the same worklist (its rows
+// give the blocks at offsets {-1, 0}^3 without hashing), or null.
+__global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, const SceneFrame* __restrict__ tab,
+                                                                  const int32_t* __restrict__ nbt) {
   const Src src = with_table(src_, tab);
   __shared__ int32_t nb[8];  // blocks at offsets (dx, dy, dz) in {-1, 0}^3
   // edges used by the triangles of each staged cube's configuration
@@ -764,6 +767,12 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
     // a block without any is done at once (no slot to free, no surfels).
     int32_t vps[2][3];
     bool any = false;
+    // lanes 1..7: the neighbour-table entry, loaded with the vertex slots
+    int32_t nbe = kFreePtr;
+    if (nbt && threadIdx.x >= 1 && threadIdx.x < 8) {
+      const int t = (int)threadIdx.x;
+      nbe = nbt[(size_t)ci * 27u + nb27(-(t & 1), -((t >> 1) & 1), -((t >> 2) & 1))];
+    }
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -778,10 +787,16 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
       }
       continue;
     }
-    const i3 b = unpack_key(V.block_keys[ptr]);
     if (threadIdx.x < 8) {
       const int t = (int)threadIdx.x;
-      nb[t] = t == 0 ? ptr : hash_find(V, i3{b.x - (t & 1), b.y - ((t >> 1) & 1), b.z - ((t >> 2) & 1)});
+      if (t == 0) {
+        nb[t] = ptr;
+      } else if (nbt) {
+        nb[t] = nbe < 0 ? nbe : (nbe & (kMeshedBit - 1));
+      } else {
+        const i3 b = unpack_key(V.block_keys[ptr]);
+        nb[t] = hash_find(V, i3{b.x - (t & 1), b.y - ((t >> 1) & 1), b.z - ((t >> 2) & 1)});
+      }
     }
     if (threadIdx.x == 0) { counts[0] = 0; counts[1] = 0; }
     __syncthreads();
@@ -1092,13 +1107,13 @@ void launch_marching_cubes(const View& V, const McConst& p, int32_t* nbt, hipStr
     mc_surfel_kernel<false><<<grid, kBlockVoxels, 0, s>>>(One{V}, nullptr, p, nbt);
   tick(kc, s, "mc_recycle_vertices_kernel");
   mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<One>, 256>(cus), 256, 0, s>>>(One{V},
-                                                                                                   nullptr);
+                                                                                                   nullptr, nbt);
   tick(kc, s, nullptr);
 }
 
 void launch_gc_recycle_vertices(const View& V, hipStream_t s, int cus) {
   mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<Listed>, 256>(cus), 256, 0, s>>>(
-      Listed{V, V.gc_list, &V.ctr->gc_count}, nullptr);
+      Listed{V, V.gc_list, &V.ctr->gc_count}, nullptr, nullptr);
 }
 
 void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus) {
@@ -1112,7 +1127,7 @@ void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s,
   else
     mc_surfel_kernel<false><<<grid, kBlockVoxels, 0, s>>>(B.src(), B.frames, unused, nbt);
   mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<Many>, 256>(cus), 256, 0, s>>>(
-      B.src(), B.frames);
+      B.src(), B.frames, nbt);
 }
 
 }  // namespace mh
