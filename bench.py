@@ -405,6 +405,9 @@ def c5_pose_refinement(dev, stream, n_poses=32, iterations=10, grad_check=True):
     return res
 
 
+RAY_TILES = [(8, 8), (16, 4), (4, 16), (32, 2)]  # raycast tile shapes, by index
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -482,6 +485,8 @@ def main():
         vol.marching_cubes()
         if record:
             e[4].record(stream)
+        if f == tune_frame:
+            vol.tune_raycast(vol.pose()[1], reps=3)
         vol.raycast(vol.pose()[1], out_depth.data_ptr(), 0, out_normal.data_ptr())
         if record:
             e[5].record(stream)
@@ -607,6 +612,8 @@ def main():
             "workgroup_size": tune.workgroup_size, "blocks_per_workgroup": tune.blocks_per_workgroup,
             "workgroups_per_cu": tune.workgroups_per_cu, "tuned_ms": tune.best_ms,
             "default_ms": tune.default_ms},
+        "raycast_tile": None if gtune is None or gtune.ray_shape < 0 else
+        "{0}x{1}".format(*RAY_TILES[gtune.ray_shape]),
         "graph_matches_eager_updates": gupd == cnt["voxel_updates"],
         "counters_per_frame": {k: v / K for k, v in cnt.items()},
         "blocks_allocated": st["block_count"],

@@ -334,11 +334,34 @@ typedef struct mh_tune_result {
   float default_ms;             /* fastest trial of the default tiling      */
   int32_t early_loads;          /* 1: voxel state loaded with the depth
                                    gathers, block keys fetched per wave    */
+  int32_t ray_shape;            /* mh_batch_run with MH_RUN_TUNE and
+                                   MH_RUN_RAYCAST: the raycast tile shape
+                                   chosen (mh_tune_raycast); otherwise -1   */
 } mh_tune_result;
 #define MH_NUM_TILINGS 18
 #define MH_NUM_TUNE_GRIDS 5 /* persistent-grid sizes timed per tiling */
 int mh_tune_integrate(mh_volume* v, uint32_t reps, mh_tune_result* out);
 int mh_set_integrate_tiling(mh_volume* v, int tiling, int workgroups_per_cu);
+
+/* ---- raycast tile shape (no counterpart in the reference, whose CastKernel
+ * runs 16 x 16-thread blocks, ray_caster.cu:190-224) ----------------------
+ * Each 64-lane workgroup of mh_raycast marches a tile of W x 64/W pixels,
+ * W = 8, 16, 4 or 32 (shapes 0..3); every shape gives identical results.
+ * mh_tune_raycast times the march of each shape on one raycast at cTw
+ * (outputs to scratch, work counters restored, reps trials each, 0 = 3)
+ * and keeps the fastest for the volume's later raycasts.  Synchronises.   */
+typedef struct mh_tune_shape_result {
+  int32_t shape;        /* index of the chosen shape                        */
+  int32_t tile_width;   /* pixels                                           */
+  int32_t tile_height;
+  int32_t candidates;   /* shapes timed                                     */
+  float best_ms;        /* fastest trial of the chosen shape                */
+  float default_ms;     /* fastest trial of shape 0 (8 x 8)                 */
+} mh_tune_shape_result;
+#define MH_NUM_RAY_SHAPES 4
+int mh_tune_raycast(mh_volume* v, const mh_raycaster_params* p, const float cTw[16],
+                    uint32_t reps, mh_tune_shape_result* out);
+int mh_set_raycast_shape(mh_volume* v, int shape);
 
 /* ---- localisation: PointToSurface, localizing/point_to_psdf.h -----------
  * Normal equations of the current frame's depth against the volume at the

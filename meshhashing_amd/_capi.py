@@ -96,11 +96,18 @@ class TuneResult(C.Structure):
     _fields_ = [("tiling", C.c_int32), ("workgroup_size", C.c_int32),
                 ("blocks_per_workgroup", C.c_int32), ("voxels_per_lane", C.c_int32),
                 ("workgroups_per_cu", C.c_int32), ("candidates", C.c_int32),
-                ("best_ms", C.c_float), ("default_ms", C.c_float), ("early_loads", C.c_int32)]
+                ("best_ms", C.c_float), ("default_ms", C.c_float), ("early_loads", C.c_int32),
+                ("ray_shape", C.c_int32)]
+
+
+class TuneShapeResult(C.Structure):
+    _fields_ = [("shape", C.c_int32), ("tile_width", C.c_int32), ("tile_height", C.c_int32),
+                ("candidates", C.c_int32), ("best_ms", C.c_float), ("default_ms", C.c_float)]
 
 
 MH_NUM_TILINGS = 18
 MH_NUM_TUNE_GRIDS = 5
+MH_NUM_RAY_SHAPES = 4
 MH_FIELD_SDF, MH_FIELD_WEIGHT, MH_FIELD_COLOR, MH_FIELD_BETA_A, MH_FIELD_BETA_B = range(5)
 MH_DIAG_KERNEL_TIMES = 1
 MH_DIAG_RAY_BLOCKS = 2
@@ -194,6 +201,9 @@ SIGNATURES = {
     "mh_dump_voxel_field": (C.c_int, [VP, VP, C.c_uint32, P_F32, P_U32]),
     "mh_tune_integrate": (C.c_int, [VP, C.c_uint32, C.POINTER(TuneResult)]),
     "mh_set_integrate_tiling": (C.c_int, [VP, C.c_int, C.c_int]),
+    "mh_tune_raycast": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, C.c_uint32,
+                                  C.POINTER(TuneShapeResult)]),
+    "mh_set_raycast_shape": (C.c_int, [VP, C.c_int]),
     "mh_batch_create": (C.c_int, [C.c_uint32, C.POINTER(HashParams), C.POINTER(VolumeParams),
                                   C.POINTER(MeshParams), C.POINTER(SensorParams),
                                   C.POINTER(VP)]),

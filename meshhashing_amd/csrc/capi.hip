@@ -146,6 +146,7 @@ struct mh_volume {
   uint32_t ray_range_words = 0;
   int tiling = kDefaultIntegrateTiling;  // integration tiling (kIntegrateTilings)
   int grid_per_cu = 8;                   // integration workgroups per CU
+  int ray_shape = 0;                     // raycast tile shape (kRayTileWidths)
   std::vector<void*> owned;
   // diagnostics (mh_volume_set_diagnostics)
   uint32_t diag = 0;
@@ -771,10 +772,10 @@ int mh_raycast(mh_volume* v, const mh_raycaster_params* p, const float cTw[16], 
     View Vr = v->V;  // lookups record the blocks they find; then they are counted
     Vr.ray_mark = v->ray_mark;
     Vr.ray_stamp = ++v->ray_stamp;
-    launch_raycast(Vr, R, depth_dev, vertex_dev, normal_dev, v->stream, v->kc());
+    launch_raycast(Vr, R, depth_dev, vertex_dev, normal_dev, v->stream, v->kc(), v->ray_shape);
     launch_ray_mark_count(Vr, v->stream, v->cus * 2);
   } else {
-    launch_raycast(v->V, R, depth_dev, vertex_dev, normal_dev, v->stream, v->kc());
+    launch_raycast(v->V, R, depth_dev, vertex_dev, normal_dev, v->stream, v->kc(), v->ray_shape);
   }
   HIP_TRY(hipGetLastError());
   return MH_OK;
@@ -1192,6 +1193,7 @@ namespace {
 constexpr int kGridPerCu[] = {2, 4, 8, 16, 32};
 static_assert(sizeof(kGridPerCu) / sizeof(kGridPerCu[0]) == MH_NUM_TUNE_GRIDS, "grid list");
 static_assert(kNumIntegrateTilings == MH_NUM_TILINGS, "tiling list");
+static_assert(kNumRayShapes == MH_NUM_RAY_SHAPES, "raycast shape list");
 constexpr int kDefaultGridPerCu = 8;
 
 struct Events {
@@ -1248,6 +1250,50 @@ int tune_loop(hipStream_t s, int cus, uint32_t reps, Run run, Restore restore, m
     out->candidates = cands;
     out->best_ms = best;
     out->default_ms = def;
+    out->ray_shape = -1;
+  }
+  return MH_OK;
+}
+
+// Times run(shape) for every raycast tile shape (reps trials each, restore()
+// after each) and returns the fastest; the kernels write only outputs and
+// work counters, which restore() puts back.
+template <class Run, class Restore>
+int tune_ray_shapes(hipStream_t s, uint32_t reps, Run run, Restore restore, mh_tune_shape_result* out,
+                    int* best_shape) {
+  Events ev;
+  HIP_TRY(hipEventCreate(&ev.a));
+  HIP_TRY(hipEventCreate(&ev.b));
+  float best = INFINITY, def = INFINITY;
+  *best_shape = 0;
+  for (int k = 0; k < kNumRayShapes; ++k) {
+    float tmin = INFINITY;
+    for (uint32_t r = 0; r < reps; ++r) {
+      HIP_TRY(hipEventRecord(ev.a, s));
+      run(k);
+      HIP_TRY(hipEventRecord(ev.b, s));
+      HIP_TRY(hipGetLastError());
+      const int rc = restore();
+      if (rc) return rc;
+      HIP_TRY(hipEventSynchronize(ev.b));
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+      tmin = fminf(tmin, ms);
+    }
+    if (tmin < best) {
+      best = tmin;
+      *best_shape = k;
+    }
+    if (k == 0) def = tmin;
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  if (out) {
+    out->shape = *best_shape;
+    out->tile_width = kRayTileWidths[*best_shape];
+    out->tile_height = 64 / kRayTileWidths[*best_shape];
+    out->candidates = kNumRayShapes;
+    out->best_ms = best;
+    out->default_ms = def;
   }
   return MH_OK;
 }
@@ -1291,6 +1337,7 @@ struct mh_batch {
   bool have_ray = false;
   int tiling = kDefaultIntegrateTiling;
   int grid_per_cu = kDefaultGridPerCu;
+  int ray_shape = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint32_t graph_flags = 0;
@@ -1330,7 +1377,8 @@ void batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   if (!integrate) return;
   launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
   if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus);
-  if (flags & MH_RUN_RAYCAST) launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream);
+  if (flags & MH_RUN_RAYCAST)
+    launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream, b->ray_shape);
 }
 
 int batch_tune(mh_batch* b, uint32_t reps, mh_tune_result* out) {
@@ -1375,6 +1423,43 @@ int batch_tune(mh_batch* b, uint32_t reps, mh_tune_result* out) {
 }  // namespace
 
 extern "C" {
+
+int mh_tune_raycast(mh_volume* v, const mh_raycaster_params* p, const float cTw[16], uint32_t reps,
+                    mh_tune_shape_result* out) {
+  if (check_vol(v) || !p || !cTw) return fail(MH_ERR_ARG, "null argument");
+  if (reps == 0) reps = 3;
+  RayConst R;
+  int rc = build_ray(v, p, cTw, &R);
+  if (rc) return rc;
+  rc = attach_range(v, &R);
+  if (rc) return rc;
+  const size_t npix = (size_t)p->width * p->height;
+  Scratch sc;
+  float *d = nullptr, *n = nullptr;
+  Counters* sctr = nullptr;
+  HIP_TRY(sc.get(&d, npix));
+  HIP_TRY(sc.get(&n, npix * 3));
+  HIP_TRY(sc.get(&sctr, 1));
+  if (R.range) launch_ray_range_one(v->V, R, v->stream);
+  HIP_TRY(hipMemcpyAsync(sctr, v->V.ctr, sizeof(Counters), hipMemcpyDeviceToDevice, v->stream));
+  auto run = [&](int k) { launch_raycast_march(v->V, R, d, nullptr, n, v->stream, k); };
+  auto restore = [&]() -> int {
+    HIP_TRY(hipMemcpyAsync(v->V.ctr, sctr, sizeof(Counters), hipMemcpyDeviceToDevice, v->stream));
+    return MH_OK;
+  };
+  int best = 0;
+  rc = tune_ray_shapes(v->stream, reps, run, restore, out, &best);
+  if (rc) return rc;
+  v->ray_shape = best;
+  return MH_OK;
+}
+
+int mh_set_raycast_shape(mh_volume* v, int shape) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  if (shape < 0 || shape >= kNumRayShapes) return fail(MH_ERR_ARG, "shape out of range");
+  v->ray_shape = shape;
+  return MH_OK;
+}
 
 int mh_set_integrate_tiling(mh_volume* v, int tiling, int grid_per_cu) {
   if (check_vol(v)) return MH_ERR_ARG;
@@ -1589,8 +1674,29 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
     const BatchDev B = b->dev();
     launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
     if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus);
-    if (flags & MH_RUN_RAYCAST)
-      launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream);
+    if (flags & MH_RUN_RAYCAST) {
+      // tile shape: each trial writes the outputs and the work counters,
+      // which are put back; the chosen shape's run below is the frame's
+      Scratch sc;
+      Counters* sctr = nullptr;
+      HIP_TRY(sc.get(&sctr, b->n));
+      for (uint32_t s = 0; s < b->n; ++s)
+        HIP_TRY(hipMemcpyAsync(sctr + s, b->vols[s]->V.ctr, sizeof(Counters), hipMemcpyDeviceToDevice,
+                               b->stream));
+      auto run = [&](int k) { launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream, k); };
+      auto restore = [&]() -> int {
+        for (uint32_t s = 0; s < b->n; ++s)
+          HIP_TRY(hipMemcpyAsync(b->vols[s]->V.ctr, sctr + s, sizeof(Counters), hipMemcpyDeviceToDevice,
+                                 b->stream));
+        return MH_OK;
+      };
+      int best = 0;
+      const int rc2 = tune_ray_shapes(b->stream, 3, run, restore, nullptr, &best);
+      if (rc2) return rc2;
+      b->ray_shape = best;
+      if (tune_out) tune_out->ray_shape = best;
+      launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream, b->ray_shape);
+    }
   } else if (flags & MH_RUN_GRAPH) {
     const uint32_t shape = flags & (MH_RUN_MESH | MH_RUN_RAYCAST);
     if (!b->exec || b->graph_flags != shape) {

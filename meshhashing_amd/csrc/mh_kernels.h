@@ -204,13 +204,20 @@ void launch_compact_emit(const View& V, uint32_t n, const uint32_t* blk_off, con
                          hipStream_t s, int grid);
 
 // raycast.hip
+// Raycast tile shapes: tile width in pixels (tile = width x 64 / width).
+constexpr int kRayTileWidths[] = {8, 16, 4, 32};
+constexpr int kNumRayShapes = sizeof(kRayTileWidths) / sizeof(kRayTileWidths[0]);
 void launch_raycast(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,
-                    hipStream_t s, const KernelClock* kc = nullptr);
+                    hipStream_t s, const KernelClock* kc = nullptr, int shape = 0);
+// the march alone (the tile ranges already filled)
+void launch_raycast_march(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,
+                          hipStream_t s, int shape);
+void launch_ray_range_one(const View& V, const RayConst& p, hipStream_t s);
 // diagnostics: add the distinct blocks the last raycast sampled (V.ray_mark
 // == V.ray_stamp) to the work counters
 void launch_ray_mark_count(const View& V, hipStream_t s, int grid);
 void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,
-                          hipStream_t s);
+                          hipStream_t s, int shape = 0);
 // adjoint: 31 FP64 sums (loss, dL/dxi[6], J^T J upper triangle[21], pixels,
 // gated-out pixels, observed pixels)
 constexpr int kGradSlots = 31;

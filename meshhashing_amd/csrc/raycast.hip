@@ -798,7 +798,9 @@ __device__ __forceinline__ const RayConst& ray_of(const Many& m, uint32_t s, con
 // Scene = blockIdx.z; a batch's outputs are scene-major (W*H per scene).
 // At most 96 registers: five waves per SIMD hold a 640x480 frame's 4800
 // tiles at once on 256 CUs (four would leave a partial second round).
-template <class Src>
+// TW: tile width in pixels (the tile is TW x 64 / TW; every shape gives the
+// same per-pixel results, the autotuner picks the fastest).
+template <class Src, int TW = 8>
 __global__ __launch_bounds__(64, 5) void raycast_kernel(Src src_, const SceneFrame* __restrict__ tab, RayConst R1, float* __restrict__ depth_out,
                                                      float* __restrict__ vertex_out,
                                                      float* __restrict__ normal_out) {
@@ -810,8 +812,9 @@ __global__ __launch_bounds__(64, 5) void raycast_kernel(Src src_, const SceneFra
   // reloaded after the stores of the march (LDS table, outputs)
   const View V = src.view(scene);
   const RayConst R = ray_of(src, scene, R1);
-  const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7);
-  const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
+  constexpr int TH = 64 / TW;
+  const uint32_t x = blockIdx.x * TW + (threadIdx.x % TW);
+  const uint32_t y = blockIdx.y * TH + (threadIdx.x / TW);
   uint32_t ns = 0;
   if (x < R.cam.width && y < R.cam.height) {
     const size_t pix = (size_t)scene * R.cam.width * R.cam.height + y * R.cam.width + x;
@@ -905,15 +908,36 @@ void launch_ray_range(Src src, const SceneFrame* tab, const RayConst& R, uint32_
   ray_range_kernel<<<dim3(128, nscenes), 256, 0, s>>>(src, tab, R);
 }
 
+template <class Src>
+void raycast_shaped(Src src, const SceneFrame* tab, const RayConst& p, uint32_t w, uint32_t h, uint32_t nscenes,
+                    int shape, float* depth, float* vertex, float* normal, hipStream_t s) {
+  const int tw = kRayTileWidths[shape], th = 64 / tw;
+  const dim3 grid((w + tw - 1) / tw, (h + th - 1) / th, nscenes);
+  switch (tw) {
+    case 16: raycast_kernel<Src, 16><<<grid, 64, 0, s>>>(src, tab, p, depth, vertex, normal); break;
+    case 4: raycast_kernel<Src, 4><<<grid, 64, 0, s>>>(src, tab, p, depth, vertex, normal); break;
+    case 32: raycast_kernel<Src, 32><<<grid, 64, 0, s>>>(src, tab, p, depth, vertex, normal); break;
+    default: raycast_kernel<Src, 8><<<grid, 64, 0, s>>>(src, tab, p, depth, vertex, normal); break;
+  }
+}
+
+void launch_ray_range_one(const View& V, const RayConst& p, hipStream_t s) {
+  launch_ray_range(One{V}, nullptr, p, 1, s);
+}
+
+void launch_raycast_march(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,
+                          hipStream_t s, int shape) {
+  raycast_shaped(One{V}, nullptr, p, p.cam.width, p.cam.height, 1, shape, depth, vertex, normal, s);
+}
+
 void launch_raycast(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,
-                    hipStream_t s, const KernelClock* kc) {
-  dim3 grid((p.cam.width + 7) / 8, (p.cam.height + 7) / 8);
+                    hipStream_t s, const KernelClock* kc, int shape) {
   if (p.range) {
     tick(kc, s, "ray_range_kernel");
     launch_ray_range(One{V}, nullptr, p, 1, s);
   }
   tick(kc, s, "raycast_kernel");
-  raycast_kernel<<<grid, 64, 0, s>>>(One{V}, nullptr, p, depth, vertex, normal);
+  raycast_shaped(One{V}, nullptr, p, p.cam.width, p.cam.height, 1, shape, depth, vertex, normal, s);
   tick(kc, s, nullptr);
 }
 
@@ -946,10 +970,9 @@ void launch_raycast_backward(const View& V, const RayConst& p, const float* obs,
 }
 
 void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,
-                          hipStream_t s) {
-  dim3 grid((B.width + 7) / 8, (B.height + 7) / 8, B.nscenes);
+                          hipStream_t s, int shape) {
   launch_ray_range(B.src(), B.frames, RayConst{}, B.nscenes, s);
-  raycast_kernel<<<grid, 64, 0, s>>>(B.src(), B.frames, RayConst{}, depth, vertex, normal);
+  raycast_shaped(B.src(), B.frames, RayConst{}, B.width, B.height, B.nscenes, shape, depth, vertex, normal, s);
 }
 
 }  // namespace mh

@@ -169,6 +169,18 @@ class Volume:
         A.check(self.lib.mh_set_integrate_tiling(self.h, tiling, workgroups_per_cu),
                 "mh_set_integrate_tiling")
 
+    def tune_raycast(self, cTw: np.ndarray, reps: int = 3,
+                     params: A.RayCasterParams | None = None) -> A.TuneShapeResult:
+        """Times the raycast tile shapes at cTw and keeps the fastest (mh_tune_raycast)."""
+        p = params if params is not None else self.cfg.raycaster_params()
+        r = A.TuneShapeResult()
+        A.check(self.lib.mh_tune_raycast(self.h, C.byref(p), A.f32x16(cTw), reps, C.byref(r)),
+                "mh_tune_raycast")
+        return r
+
+    def set_raycast_shape(self, shape: int):
+        A.check(self.lib.mh_set_raycast_shape(self.h, shape), "mh_set_raycast_shape")
+
     def pose(self) -> tuple[np.ndarray, np.ndarray]:
         """(wTc, cTw) of the current frame, FP32, as the kernels use them."""
         w = np.zeros(16, np.float32)

@@ -151,6 +151,7 @@ def test_batch_tuning_changes_nothing(hip_device):
           f"{tune.best_ms:.4f} ms vs default {tune.default_ms:.4f} ms")
     assert tune.candidates == A.MH_NUM_TILINGS * A.MH_NUM_TUNE_GRIDS and 0 <= tune.tiling < A.MH_NUM_TILINGS
     assert tune.best_ms <= tune.default_ms
+    assert 0 <= tune.ray_shape < A.MH_NUM_RAY_SHAPES  # the raycast tile shape, tuned too
     for f in frames:
         for s in range(len(cfgs)):
             assert np.array_equal(img0[f][s][0], img1[f][s][0])
@@ -220,3 +221,46 @@ def test_c4_all_scenes_batched_equal_sequential(hip_device):
     print(f"[c4] 64 scenes equal; blocks per scene {min(counts)}-{max(counts)}, tiling {tune.tiling}")
     assert len(set(c.voxel_size for c in cfgs)) == 4
     b.close()
+
+
+def test_raycast_shapes_are_output_invariant(hip_device):
+    """Every raycast tile shape (8x8, 16x4, 4x16, 32x2 pixels per wave) gives
+    bit-identical depth, vertex and normal images; mh_tune_raycast picks one
+    and leaves the work counters as they were."""
+    import torch
+    cfg = replace(C1, frames=8)
+    vol = Volume(cfg)
+    for f in range(0, 8, 2):
+        vol.upload_depth(synth_depth(cfg, f), synth_pose(cfg, f))
+        vol.alloc()
+        vol.collect_visible()
+        vol.integrate_tsdf()
+    cTw = vol.pose()[1]
+    n = cfg.width * cfg.height
+
+    def cast():
+        d = torch.empty(n, dtype=torch.float32, device=hip_device)
+        v = torch.empty(3 * n, dtype=torch.float32, device=hip_device)
+        nn = torch.empty(3 * n, dtype=torch.float32, device=hip_device)
+        vol.raycast(cTw, d.data_ptr(), v.data_ptr(), nn.data_ptr())
+        vol.synchronize()
+        return [x.cpu().numpy().view(np.uint32) for x in (d, v, nn)]
+
+    want = cast()
+    assert np.isfinite(want[0].view(np.float32)).mean() > 0.5
+    for shape in range(A.MH_NUM_RAY_SHAPES):
+        vol.set_raycast_shape(shape)
+        got = cast()
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b), f"shape {shape}"
+    before = vol.status()
+    r = vol.tune_raycast(cTw, reps=2)
+    print(f"[ray shape] {r.tile_width}x{r.tile_height}: {r.best_ms:.4f} ms vs 8x8 {r.default_ms:.4f} ms")
+    assert 0 <= r.shape < A.MH_NUM_RAY_SHAPES and r.candidates == A.MH_NUM_RAY_SHAPES
+    assert r.tile_width * r.tile_height == 64 and r.best_ms <= r.default_ms
+    assert vol.status() == before
+    for a, b in zip(cast(), want):
+        assert np.array_equal(a, b)
+    with pytest.raises(A.MHError):
+        vol.set_raycast_shape(A.MH_NUM_RAY_SHAPES)
+    vol.close()
