@@ -322,7 +322,10 @@ int mh_frame_get_pose(mh_volume* v, float wTc[16], float cTw[16]);
  * not depend on the tiling: each voxel is updated by one lane with the same
  * arithmetic.  mh_tune_integrate runs after mh_collect_* and before
  * mh_integrate_tsdf; it saves and restores the worklist's fused state and
- * the counters around every trial, so it changes no output.               */
+ * the counters around every trial, so it changes no output.
+ * On a PSDF volume it tunes mh_integrate_psdf instead: tiling t picks 128,
+ * 256 or 512 lanes per workgroup (one block at a time), times 5 grids per CU
+ * (15 candidates); mh_set_integrate_tiling takes the same indices there.   */
 typedef struct mh_tune_result {
   int32_t tiling;               /* index of the chosen tiling               */
   int32_t workgroup_size;       /* lanes per workgroup                      */
@@ -362,6 +365,33 @@ typedef struct mh_tune_shape_result {
 int mh_tune_raycast(mh_volume* v, const mh_raycaster_params* p, const float cTw[16],
                     uint32_t reps, mh_tune_shape_result* out);
 int mh_set_raycast_shape(mh_volume* v, int shape);
+
+/* ---- marching-cubes launch shape (no counterpart in the reference, whose
+ * SurfelExtractionKernel runs one 512-thread block per worklist block,
+ * marching_cubes.cu:339-381) ---------------------------------------------
+ * The surfel pass runs 512-lane workgroups over the worklist, persistent:
+ * shape k sets the workgroups per CU (0: as many as are resident) and the
+ * quarters of the worklist handed out statically (the rest on demand);
+ * MH_NUM_MC_SHAPES shapes, shape 0 the default, all with the same mesh.
+ * Meshing changes the volume, so tuning runs online: mh_tune_marching_cubes
+ * makes the next reps x MH_NUM_MC_SHAPES calls of mh_marching_cubes each run
+ * one shape in turn (timed with HIP events; every call still meshes as
+ * asked), and the call after them keeps the shape with the least mean time.
+ * mh_marching_cubes_tuning reports the progress (done = 0) or the result;
+ * mh_set_marching_cubes_shape sets a shape (and ends any tuning).         */
+typedef struct mh_tune_mc_result {
+  int32_t done;               /* 1 once the shape is chosen                 */
+  int32_t shape;              /* the chosen (or current) shape              */
+  int32_t workgroups_per_cu;  /* 0: all resident                            */
+  int32_t static_quarters;    /* of the worklist handed out statically      */
+  int32_t candidates;
+  float best_ms;              /* mean of the chosen shape's trials          */
+  float default_ms;           /* mean of shape 0's trials                   */
+} mh_tune_mc_result;
+#define MH_NUM_MC_SHAPES 9
+int mh_tune_marching_cubes(mh_volume* v, uint32_t reps);
+int mh_marching_cubes_tuning(mh_volume* v, mh_tune_mc_result* out);
+int mh_set_marching_cubes_shape(mh_volume* v, int shape);
 
 /* ---- localisation: PointToSurface, localizing/point_to_psdf.h -----------
  * Normal equations of the current frame's depth against the volume at the

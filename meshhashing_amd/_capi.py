@@ -100,6 +100,12 @@ class TuneResult(C.Structure):
                 ("ray_shape", C.c_int32)]
 
 
+class TuneMcResult(C.Structure):
+    _fields_ = [("done", C.c_int32), ("shape", C.c_int32), ("workgroups_per_cu", C.c_int32),
+                ("static_quarters", C.c_int32), ("candidates", C.c_int32), ("best_ms", C.c_float),
+                ("default_ms", C.c_float)]
+
+
 class TuneShapeResult(C.Structure):
     _fields_ = [("shape", C.c_int32), ("tile_width", C.c_int32), ("tile_height", C.c_int32),
                 ("candidates", C.c_int32), ("best_ms", C.c_float), ("default_ms", C.c_float)]
@@ -108,6 +114,7 @@ class TuneShapeResult(C.Structure):
 MH_NUM_TILINGS = 18
 MH_NUM_TUNE_GRIDS = 5
 MH_NUM_RAY_SHAPES = 4
+MH_NUM_MC_SHAPES = 9
 MH_FIELD_SDF, MH_FIELD_WEIGHT, MH_FIELD_COLOR, MH_FIELD_BETA_A, MH_FIELD_BETA_B = range(5)
 MH_DIAG_KERNEL_TIMES = 1
 MH_DIAG_RAY_BLOCKS = 2
@@ -204,6 +211,9 @@ SIGNATURES = {
     "mh_tune_raycast": (C.c_int, [VP, C.POINTER(RayCasterParams), P_F32, C.c_uint32,
                                   C.POINTER(TuneShapeResult)]),
     "mh_set_raycast_shape": (C.c_int, [VP, C.c_int]),
+    "mh_tune_marching_cubes": (C.c_int, [VP, C.c_uint32]),
+    "mh_marching_cubes_tuning": (C.c_int, [VP, C.POINTER(TuneMcResult)]),
+    "mh_set_marching_cubes_shape": (C.c_int, [VP, C.c_int]),
     "mh_batch_create": (C.c_int, [C.c_uint32, C.POINTER(HashParams), C.POINTER(VolumeParams),
                                   C.POINTER(MeshParams), C.POINTER(SensorParams),
                                   C.POINTER(VP)]),

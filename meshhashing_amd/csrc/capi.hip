@@ -147,6 +147,20 @@ struct mh_volume {
   int tiling = kDefaultIntegrateTiling;  // integration tiling (kIntegrateTilings)
   int grid_per_cu = 8;                   // integration workgroups per CU
   int ray_shape = 0;                     // raycast tile shape (kRayTileWidths)
+  int mc_shape = 0;                      // marching-cubes launch shape (kMcShapes)
+  int psdf_tiling = kDefaultPsdfTiling;  // PSDF fusion workgroup size (kPsdfLanes)
+  int psdf_grid_per_cu = 8;
+  // online marching-cubes tuning (mh_tune_marching_cubes): trial k of
+  // n_trials runs shape k % kNumMcShapes; its time is read at the next call
+  struct McTune {
+    bool active = false;
+    uint32_t trial = 0, n_trials = 0;
+    int pending = -1;  // shape whose events are recorded but not yet read
+    hipEvent_t a = nullptr, b = nullptr;
+    double total_ms[kNumMcShapes] = {};
+    uint32_t runs[kNumMcShapes] = {};
+    mh_tune_mc_result result{};
+  } mct;
   std::vector<void*> owned;
   // diagnostics (mh_volume_set_diagnostics)
   uint32_t diag = 0;
@@ -486,6 +500,8 @@ int mh_volume_destroy(mh_volume* v) {
   if (v->ray_mark) (void)hipFree(v->ray_mark);
   for (hipEvent_t e : v->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : v->ev_used) (void)hipEventDestroy(e);
+  if (v->mct.a) (void)hipEventDestroy(v->mct.a);
+  if (v->mct.b) (void)hipEventDestroy(v->mct.b);
   delete v;
   return MH_OK;
 }
@@ -745,18 +761,104 @@ int mh_integrate_psdf(mh_volume* v) {
   if (!v->psdf) return fail(MH_ERR_STATE, "volume created without PSDF fields");
   if (!v->have_frame) return fail(MH_ERR_STATE, "no frame uploaded");
   flush_frame(v);
-  launch_integrate_psdf(v->V, v->stream, v->grid_wg256());
+  launch_integrate_psdf(v->V, v->stream, v->cus * v->psdf_grid_per_cu, kPsdfLanes[v->psdf_tiling]);
   HIP_TRY(hipGetLastError());
+  return MH_OK;
+}
+
+// Reads the last tuning trial's time; after the last trial fixes the
+// fastest shape (mean over its trials).
+static int mc_tune_collect(mh_volume* v) {
+  auto& t = v->mct;
+  if (t.pending >= 0) {
+    HIP_TRY(hipEventSynchronize(t.b));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+    t.total_ms[t.pending] += ms;
+    ++t.runs[t.pending];
+    t.pending = -1;
+  }
+  if (t.active && t.trial >= t.n_trials) {
+    int best = 0;
+    double best_ms = INFINITY;
+    for (int k = 0; k < kNumMcShapes; ++k) {
+      const double m = t.runs[k] ? t.total_ms[k] / t.runs[k] : INFINITY;
+      if (m < best_ms) {
+        best_ms = m;
+        best = k;
+      }
+    }
+    v->mc_shape = best;
+    t.active = false;
+    t.result.done = 1;
+    t.result.shape = best;
+    t.result.workgroups_per_cu = kMcShapes[best].grid_per_cu;
+    t.result.static_quarters = kMcShapes[best].static_quarters;
+    t.result.candidates = kNumMcShapes;
+    t.result.best_ms = (float)best_ms;
+    t.result.default_ms = t.runs[0] ? (float)(t.total_ms[0] / t.runs[0]) : INFINITY;
+  }
   return MH_OK;
 }
 
 int mh_marching_cubes(mh_volume* v, const mh_mc_params* p) {
   if (check_vol(v)) return MH_ERR_ARG;
   McConst c;
-  const int rc = build_mc(v, p, &c);
+  int rc = build_mc(v, p, &c);
   if (rc) return rc;
-  launch_marching_cubes(v->V, c, v->mc_nbt, v->stream, v->cus, v->kc());
+  auto& t = v->mct;
+  if (t.active || t.pending >= 0) {
+    rc = mc_tune_collect(v);
+    if (rc) return rc;
+  }
+  if (t.active) {
+    const int k = (int)(t.trial++ % kNumMcShapes);
+    HIP_TRY(hipEventRecord(t.a, v->stream));
+    launch_marching_cubes(v->V, c, v->mc_nbt, v->stream, v->cus, v->kc(), kMcShapes[k]);
+    HIP_TRY(hipEventRecord(t.b, v->stream));
+    t.pending = k;
+  } else {
+    launch_marching_cubes(v->V, c, v->mc_nbt, v->stream, v->cus, v->kc(), kMcShapes[v->mc_shape]);
+  }
   HIP_TRY(hipGetLastError());
+  return MH_OK;
+}
+
+int mh_tune_marching_cubes(mh_volume* v, uint32_t reps) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  auto& t = v->mct;
+  if (!t.a) HIP_TRY(hipEventCreate(&t.a));
+  if (!t.b) HIP_TRY(hipEventCreate(&t.b));
+  const int rc = mc_tune_collect(v);  // a trial of an earlier tuning still pending
+  if (rc) return rc;
+  t.active = true;
+  t.trial = 0;
+  t.n_trials = (reps ? reps : 1) * kNumMcShapes;
+  for (int k = 0; k < kNumMcShapes; ++k) {
+    t.total_ms[k] = 0.0;
+    t.runs[k] = 0;
+  }
+  t.result = mh_tune_mc_result{};
+  return MH_OK;
+}
+
+int mh_marching_cubes_tuning(mh_volume* v, mh_tune_mc_result* out) {
+  if (check_vol(v) || !out) return fail(MH_ERR_ARG, "null argument");
+  const int rc = mc_tune_collect(v);
+  if (rc) return rc;
+  *out = v->mct.result;
+  if (!out->done) {
+    out->candidates = kNumMcShapes;
+    out->shape = v->mc_shape;
+  }
+  return MH_OK;
+}
+
+int mh_set_marching_cubes_shape(mh_volume* v, int shape) {
+  if (check_vol(v)) return MH_ERR_ARG;
+  if (shape < 0 || shape >= kNumMcShapes) return fail(MH_ERR_ARG, "shape out of range");
+  v->mc_shape = shape;
+  v->mct.active = false;
   return MH_OK;
 }
 
@@ -1194,6 +1296,7 @@ constexpr int kGridPerCu[] = {2, 4, 8, 16, 32};
 static_assert(sizeof(kGridPerCu) / sizeof(kGridPerCu[0]) == MH_NUM_TUNE_GRIDS, "grid list");
 static_assert(kNumIntegrateTilings == MH_NUM_TILINGS, "tiling list");
 static_assert(kNumRayShapes == MH_NUM_RAY_SHAPES, "raycast shape list");
+static_assert(kNumMcShapes == MH_NUM_MC_SHAPES, "marching-cubes shape list");
 constexpr int kDefaultGridPerCu = 8;
 
 struct Events {
@@ -1463,10 +1566,91 @@ int mh_set_raycast_shape(mh_volume* v, int shape) {
 
 int mh_set_integrate_tiling(mh_volume* v, int tiling, int grid_per_cu) {
   if (check_vol(v)) return MH_ERR_ARG;
+  if (v->psdf) {
+    if (tiling < 0 || tiling >= kNumPsdfTilings) return fail(MH_ERR_ARG, "PSDF tiling out of range");
+    if (grid_per_cu < 1 || grid_per_cu > 64) return fail(MH_ERR_ARG, "grid_per_cu out of range");
+    v->psdf_tiling = tiling;
+    v->psdf_grid_per_cu = grid_per_cu;
+    return MH_OK;
+  }
   const int rc = check_tiling(tiling, grid_per_cu);
   if (rc) return rc;
   v->tiling = tiling;
   v->grid_per_cu = grid_per_cu;
+  return MH_OK;
+}
+
+// PSDF fusion tuning: workgroup sizes kPsdfLanes x workgroups per CU, the
+// worklist's sdf / weight (1/sigma^2) / a / b and the counters restored
+// after every trial.
+static int tune_psdf(mh_volume* v, uint32_t n, uint32_t reps, mh_tune_result* out) {
+  Scratch sc;
+  float *ss = nullptr, *sw = nullptr, *sa = nullptr, *sb = nullptr;
+  uint32_t *scol = nullptr, *scol2 = nullptr;
+  Counters* sctr = nullptr;
+  const size_t nv = (size_t)n * kBlockVoxels;
+  HIP_TRY(sc.get(&ss, nv));
+  HIP_TRY(sc.get(&sw, nv));
+  HIP_TRY(sc.get(&sa, nv));
+  HIP_TRY(sc.get(&sb, nv));
+  HIP_TRY(sc.get(&scol, nv));
+  HIP_TRY(sc.get(&scol2, nv));
+  HIP_TRY(sc.get(&sctr, 1));
+  View Vab = v->V;  // the Beta parameters through the same copy kernel
+  Vab.sdf = v->V.va;
+  Vab.weight = v->V.vb;
+  launch_state_save(v->V, ss, sw, scol, v->stream, v->cus * 4);
+  launch_state_save(Vab, sa, sb, scol2, v->stream, v->cus * 4);
+  HIP_TRY(hipMemcpyAsync(sctr, v->V.ctr, sizeof(Counters), hipMemcpyDeviceToDevice, v->stream));
+  auto restore = [&]() -> int {
+    launch_state_restore(v->V, ss, sw, scol, v->stream, v->cus * 4);
+    launch_state_restore(Vab, sa, sb, scol2, v->stream, v->cus * 4);
+    HIP_TRY(hipMemcpyAsync(v->V.ctr, sctr, sizeof(Counters), hipMemcpyDeviceToDevice, v->stream));
+    return MH_OK;
+  };
+  Events ev;
+  HIP_TRY(hipEventCreate(&ev.a));
+  HIP_TRY(hipEventCreate(&ev.b));
+  float best = INFINITY, def = INFINITY;
+  int bt = kDefaultPsdfTiling, bg = kDefaultGridPerCu, cands = 0;
+  for (int t = 0; t < kNumPsdfTilings; ++t)
+    for (int g : kGridPerCu) {
+      float tmin = INFINITY;
+      for (uint32_t r = 0; r < reps; ++r) {
+        HIP_TRY(hipEventRecord(ev.a, v->stream));
+        launch_integrate_psdf(v->V, v->stream, v->cus * g, kPsdfLanes[t]);
+        HIP_TRY(hipEventRecord(ev.b, v->stream));
+        HIP_TRY(hipGetLastError());
+        const int rc = restore();
+        if (rc) return rc;
+        HIP_TRY(hipEventSynchronize(ev.b));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+        tmin = fminf(tmin, ms);
+      }
+      ++cands;
+      if (tmin < best) {
+        best = tmin;
+        bt = t;
+        bg = g;
+      }
+      if (t == kDefaultPsdfTiling && g == kDefaultGridPerCu) def = tmin;
+    }
+  HIP_TRY(hipStreamSynchronize(v->stream));
+  v->psdf_tiling = bt;
+  v->psdf_grid_per_cu = bg;
+  if (out) {
+    out->tiling = bt;
+    out->workgroup_size = kPsdfLanes[bt];
+    out->blocks_per_workgroup = 1;
+    out->voxels_per_lane = kBlockVoxels / kPsdfLanes[bt];
+    out->workgroups_per_cu = bg;
+    out->candidates = cands;
+    out->best_ms = best;
+    out->default_ms = def;
+    out->early_loads = 0;
+    out->ray_shape = -1;
+  }
   return MH_OK;
 }
 
@@ -1477,6 +1661,7 @@ int mh_tune_integrate(mh_volume* v, uint32_t reps, mh_tune_result* out) {
   if (reps == 0) reps = 3;
   const uint32_t n = read_counters(v).cand_count;
   if (n == 0) return fail(MH_ERR_EMPTY, "empty worklist");
+  if (v->psdf) return tune_psdf(v, n, reps, out);
   Scratch sc;
   float *ss = nullptr, *sw = nullptr;
   uint32_t* scol = nullptr;

@@ -691,7 +691,7 @@ __global__ __launch_bounds__(256) void predict_inlier_kernel(View V) {
 }
 
 // UpdateBlocksBayesianKernel, update_bayesian.cu:103-185.
-__global__ __launch_bounds__(256) void integrate_psdf_kernel(View V) {
+__global__ __launch_bounds__(512) void integrate_psdf_kernel(View V) {
   const uint32_t n = V.ctr->cand_count;
   const Geo g = V.geo;
   const Cam c = V.cam;
@@ -1107,8 +1107,8 @@ void launch_predict_inlier(const View& V, hipStream_t s, int grid) {
   predict_inlier_kernel<<<grid, 256, 0, s>>>(V);
 }
 
-void launch_integrate_psdf(const View& V, hipStream_t s, int grid) {
-  integrate_psdf_kernel<<<grid, 256, 0, s>>>(V);
+void launch_integrate_psdf(const View& V, hipStream_t s, int grid, int lanes) {
+  integrate_psdf_kernel<<<grid, lanes, 0, s>>>(V);
 }
 
 // ------------------------------------------------------------ batched launchers

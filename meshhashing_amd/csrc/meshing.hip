@@ -60,11 +60,12 @@ __device__ __forceinline__ const McConst& mc_of(const Many& m, uint32_t s, const
 }
 constexpr uint32_t kLookupLane = kBlockVoxels - 64;  // the surfel pass's last wave
 // The surfel pass hands out its first items statically (workgroup g takes
-// g, g + grid, ...) and the rest, about a quarter, on demand from a counter,
-// so workgroups whose blocks held little surface take more of them.  Items
-// below this bound are static; the counter starts there.
-__device__ __forceinline__ uint32_t mc_static_items(uint32_t n, uint32_t grid) {
-  const uint32_t k = (3u * n / 4u) / grid;
+// g, g + grid, ...) and the rest on demand from a counter, so workgroups
+// whose blocks held little surface take more of them.  static_q quarters of
+// the worklist (rounded down to whole rounds of the grid, at least one) are
+// static; the counter starts at that bound.
+__device__ __forceinline__ uint32_t mc_static_items(uint32_t n, uint32_t grid, uint32_t static_q) {
+  const uint32_t k = (static_q * n / 4u) / grid;
   return (k ? k : 1u) * grid;
 }
 constexpr int32_t kMeshedBit = 1 << 30;
@@ -509,7 +510,7 @@ __device__ __forceinline__ int32_t* edge_slot(const View& V, const int32_t* nb, 
 
 template <bool PSDF, class Src>
 __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1,
-                                                           const int32_t* __restrict__ nbt) {
+                                                           const int32_t* __restrict__ nbt, uint32_t static_q) {
   const Src src = with_table(src_, tab);
   __shared__ Shell<PSDF> S;
   __shared__ int32_t nb[27];
@@ -542,7 +543,7 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
   const uint32_t nb_lane = threadIdx.x - kLookupLane;
   int32_t nbe = kFreePtr;
   if (nb_lane < 27 && blockIdx.x < n) nbe = nbt[(size_t)blockIdx.x * 27u + nb_lane];
-  const uint32_t n_static = mc_static_items(n, gridDim.x);
+  const uint32_t n_static = mc_static_items(n, gridDim.x, static_q);
   __shared__ uint32_t s_next;
   for (uint32_t ci = blockIdx.x; ci < n;) {
     // the next item, read by every lane after the shell's barriers and
@@ -697,10 +698,11 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
 
 // Meshed-in-this-pass marks for the writer rule.
 template <class Src>
-__global__ void mc_mark_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1, uint32_t surfel_grid) {
+__global__ void mc_mark_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1, uint32_t surfel_grid,
+                               uint32_t static_q) {
   const Src src = with_table(src_, tab);
   const uint32_t n = src.count();
-  if (blockIdx.x == 0 && threadIdx.x == 0) *src.next_ctr() = mc_static_items(n, surfel_grid);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *src.next_ctr() = mc_static_items(n, surfel_grid, static_q);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     uint32_t scene;
     const int32_t ptr = src.lane_item(i, &scene);
@@ -1092,19 +1094,29 @@ int resident_grid(int cus) {
   return per_cu * cus;
 }
 
+// Surfel-pass grid: shape.grid_per_cu workgroups per CU (0: as many as are
+// resident), at most the resident count.
+template <auto K>
+int surfel_grid(int cus, const McShape& shape) {
+  const int resident = resident_grid<K, kBlockVoxels>(cus);
+  const int g = shape.grid_per_cu > 0 ? shape.grid_per_cu * cus : resident;
+  return g < resident ? g : resident;
+}
+
 void launch_marching_cubes(const View& V, const McConst& p, int32_t* nbt, hipStream_t s, int cus,
-                           const KernelClock* kc) {
-  const int grid = V.va ? resident_grid<mc_surfel_kernel<true, One>, kBlockVoxels>(cus)
-                        : resident_grid<mc_surfel_kernel<false, One>, kBlockVoxels>(cus);
+                           const KernelClock* kc, McShape shape) {
+  const int grid = V.va ? surfel_grid<mc_surfel_kernel<true, One>>(cus, shape)
+                        : surfel_grid<mc_surfel_kernel<false, One>>(cus, shape);
+  const uint32_t q = (uint32_t)shape.static_quarters;
   tick(kc, s, "mc_mark_kernel");
-  mc_mark_kernel<<<cus, 256, 0, s>>>(One{V}, nullptr, p, (uint32_t)grid);
+  mc_mark_kernel<<<cus, 256, 0, s>>>(One{V}, nullptr, p, (uint32_t)grid, q);
   tick(kc, s, "mc_neighbours_kernel");
   mc_neighbours_kernel<<<cus * 8, 256, 0, s>>>(One{V}, nullptr, p, nbt);
   tick(kc, s, "mc_surfel_kernel");
   if (V.va)
-    mc_surfel_kernel<true><<<grid, kBlockVoxels, 0, s>>>(One{V}, nullptr, p, nbt);
+    mc_surfel_kernel<true><<<grid, kBlockVoxels, 0, s>>>(One{V}, nullptr, p, nbt, q);
   else
-    mc_surfel_kernel<false><<<grid, kBlockVoxels, 0, s>>>(One{V}, nullptr, p, nbt);
+    mc_surfel_kernel<false><<<grid, kBlockVoxels, 0, s>>>(One{V}, nullptr, p, nbt, q);
   tick(kc, s, "mc_recycle_vertices_kernel");
   mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<One>, 256>(cus), 256, 0, s>>>(One{V},
                                                                                                    nullptr, nbt);
@@ -1116,16 +1128,17 @@ void launch_gc_recycle_vertices(const View& V, hipStream_t s, int cus) {
       Listed{V, V.gc_list, &V.ctr->gc_count}, nullptr, nullptr);
 }
 
-void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus) {
+void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus, McShape shape) {
   const McConst unused{};
-  const int grid = B.psdf ? resident_grid<mc_surfel_kernel<true, Many>, kBlockVoxels>(cus)
-                          : resident_grid<mc_surfel_kernel<false, Many>, kBlockVoxels>(cus);
-  mc_mark_kernel<<<cus, 256, 0, s>>>(B.src(), B.frames, unused, (uint32_t)grid);
+  const int grid = B.psdf ? surfel_grid<mc_surfel_kernel<true, Many>>(cus, shape)
+                          : surfel_grid<mc_surfel_kernel<false, Many>>(cus, shape);
+  const uint32_t q = (uint32_t)shape.static_quarters;
+  mc_mark_kernel<<<cus, 256, 0, s>>>(B.src(), B.frames, unused, (uint32_t)grid, q);
   mc_neighbours_kernel<<<cus * 8, 256, 0, s>>>(B.src(), B.frames, unused, nbt);
   if (B.psdf)
-    mc_surfel_kernel<true><<<grid, kBlockVoxels, 0, s>>>(B.src(), B.frames, unused, nbt);
+    mc_surfel_kernel<true><<<grid, kBlockVoxels, 0, s>>>(B.src(), B.frames, unused, nbt, q);
   else
-    mc_surfel_kernel<false><<<grid, kBlockVoxels, 0, s>>>(B.src(), B.frames, unused, nbt);
+    mc_surfel_kernel<false><<<grid, kBlockVoxels, 0, s>>>(B.src(), B.frames, unused, nbt, q);
   mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<Many>, 256>(cus), 256, 0, s>>>(
       B.src(), B.frames, nbt);
 }

@@ -162,7 +162,12 @@ void launch_collect(const View& V, int frustum, hipStream_t s, int grid,
 void launch_integrate_tsdf(const View& V, hipStream_t s, int grid, int tiling,
                            const KernelClock* kc = nullptr);
 void launch_predict_inlier(const View& V, hipStream_t s, int grid);
-void launch_integrate_psdf(const View& V, hipStream_t s, int grid);
+// PSDF fusion: workgroup sizes the tuner tries (each lane fuses 512 / size
+// voxels of a block; one block per workgroup at a time).
+constexpr int kPsdfLanes[] = {128, 256, 512};
+constexpr int kNumPsdfTilings = sizeof(kPsdfLanes) / sizeof(kPsdfLanes[0]);
+constexpr int kDefaultPsdfTiling = 1;
+void launch_integrate_psdf(const View& V, hipStream_t s, int grid, int lanes = 256);
 void launch_load_insert(const View& V, const uint64_t* keys, uint32_t n, hipStream_t s);
 void launch_load_scatter(const View& V, const uint64_t* keys, uint32_t n, const float* sdf,
                          const float* weight, const uint32_t* color, const float* a,
@@ -190,9 +195,18 @@ void launch_gc_release_triangles(const View& V, hipStream_t s, int grid);
 void launch_gc_recycle_vertices(const View& V, hipStream_t s, int cus);
 // nbt: the 27-neighbour table, 27 int32 per worklist item (for a batch, per
 // item of the concatenated worklist)
+// Marching-cubes launch shape: surfel workgroups per CU (0: all resident)
+// and the quarters of the worklist handed out statically; output-invariant.
+struct McShape {
+  int grid_per_cu;
+  int static_quarters;
+};
+constexpr McShape kMcShapes[] = {{0, 3}, {0, 2}, {0, 4}, {2, 3}, {2, 2}, {2, 4}, {1, 3}, {1, 2}, {1, 4}};
+constexpr int kNumMcShapes = sizeof(kMcShapes) / sizeof(kMcShapes[0]);
 void launch_marching_cubes(const View& V, const McConst& p, int32_t* nbt, hipStream_t s, int cus,
-                           const KernelClock* kc = nullptr);
-void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus);
+                           const KernelClock* kc = nullptr, McShape shape = kMcShapes[0]);
+void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus,
+                                 McShape shape = kMcShapes[0]);
 // mesh compaction (CompressMesh) and an exclusive scan of n uint32 (chunk_sums:
 // ceil(n / 1024) entries)
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* chunk_sums,

@@ -181,6 +181,19 @@ class Volume:
     def set_raycast_shape(self, shape: int):
         A.check(self.lib.mh_set_raycast_shape(self.h, shape), "mh_set_raycast_shape")
 
+    def tune_marching_cubes(self, reps: int = 1):
+        """The next reps x MH_NUM_MC_SHAPES marching_cubes() calls try the launch
+        shapes in turn; the call after them keeps the fastest."""
+        A.check(self.lib.mh_tune_marching_cubes(self.h, reps), "mh_tune_marching_cubes")
+
+    def marching_cubes_tuning(self) -> A.TuneMcResult:
+        r = A.TuneMcResult()
+        A.check(self.lib.mh_marching_cubes_tuning(self.h, C.byref(r)), "mh_marching_cubes_tuning")
+        return r
+
+    def set_marching_cubes_shape(self, shape: int):
+        A.check(self.lib.mh_set_marching_cubes_shape(self.h, shape), "mh_set_marching_cubes_shape")
+
     def pose(self) -> tuple[np.ndarray, np.ndarray]:
         """(wTc, cTw) of the current frame, FP32, as the kernels use them."""
         w = np.zeros(16, np.float32)

@@ -78,8 +78,9 @@ def test_synth_spline_speed_limits():
 
 def test_header_constants_match_binding():
     text = (ROOT / "include" / "meshhashing.h").read_text()
-    for name in ("MH_NUM_TILINGS", "MH_NUM_TUNE_GRIDS", "MH_NUM_RAY_SHAPES"):
+    for name in ("MH_NUM_TILINGS", "MH_NUM_TUNE_GRIDS", "MH_NUM_RAY_SHAPES", "MH_NUM_MC_SHAPES"):
         m = re.search(rf"#define {name} (\d+)", text)
         assert m and int(m.group(1)) == getattr(A, name), name
     assert C.sizeof(A.TuneResult) == 40
     assert C.sizeof(A.TuneShapeResult) == 24
+    assert C.sizeof(A.TuneMcResult) == 28

@@ -264,3 +264,91 @@ def test_raycast_shapes_are_output_invariant(hip_device):
     with pytest.raises(A.MHError):
         vol.set_raycast_shape(A.MH_NUM_RAY_SHAPES)
     vol.close()
+
+
+def meshed_run(cfg, frames, shape=None, tune_reps=None):
+    vol = Volume(cfg)
+    if shape is not None:
+        vol.set_marching_cubes_shape(shape)
+    if tune_reps is not None:
+        vol.tune_marching_cubes(tune_reps)
+    for f in frames:
+        vol.upload_depth(synth_depth(cfg, f), synth_pose(cfg, f))
+        vol.alloc()
+        vol.collect_visible()
+        vol.integrate_tsdf()
+        vol.marching_cubes()
+    return vol
+
+
+def test_marching_cubes_shapes_are_output_invariant(hip_device):
+    """Every marching-cubes launch shape (surfel workgroups per CU, static
+    share of the worklist) leaves the same canonical mesh, cube indices,
+    vertex payloads and counters."""
+    cfg = replace(C1, frames=8)
+    frames = range(0, 8, 2)
+    want = scene_state(meshed_run(cfg, frames))
+    for k in range(A.MH_NUM_MC_SHAPES):
+        print(f"[mc shape] {k}", flush=True)
+        vol = meshed_run(cfg, frames, shape=k)
+        assert_same(scene_state(vol), want, f"mc shape {k}")
+        vol.close()
+    with pytest.raises(A.MHError):
+        meshed_run(cfg, frames, shape=A.MH_NUM_MC_SHAPES)
+
+
+def test_tune_marching_cubes_online(hip_device):
+    """Online tuning: the first MH_NUM_MC_SHAPES meshing calls each run one
+    shape, the next one keeps the fastest; the mesh is the untuned one."""
+    cfg = replace(C1, frames=20)
+    frames = range(A.MH_NUM_MC_SHAPES + 2)
+    ref = scene_state(meshed_run(cfg, frames))
+    vol = meshed_run(cfg, frames, tune_reps=1)
+    r = vol.marching_cubes_tuning()
+    print(f"[mc tune] shape {r.shape} ({r.workgroups_per_cu} wg/CU, {r.static_quarters}/4 static): "
+          f"{r.best_ms:.4f} ms vs default {r.default_ms:.4f} ms")
+    assert r.done == 1 and 0 <= r.shape < A.MH_NUM_MC_SHAPES and r.candidates == A.MH_NUM_MC_SHAPES
+    assert r.best_ms <= r.default_ms
+    assert_same(scene_state(vol), ref, "tuned")
+    vol.close()
+
+
+def test_tune_psdf_integration_changes_nothing(hip_device):
+    """mh_tune_integrate on a PSDF volume tunes the PSDF fusion's workgroup
+    size and grid with mu / (1/sigma^2) / a / b and the counters restored
+    after every trial: the fused state equals the untuned run's bit for bit,
+    and so does every fixed PSDF tiling."""
+    from meshhashing_amd.configs import C3
+    cfg = replace(C3, frames=8)
+    frames = range(0, 8, 2)
+
+    def run(tune_at=None, tiling=None):
+        vol = Volume(cfg)
+        if tiling is not None:
+            vol.set_integrate_tiling(*tiling)
+        res = None
+        for f in frames:
+            vol.upload_depth(synth_depth(cfg, f), synth_pose(cfg, f))
+            vol.alloc()
+            vol.collect_visible()
+            vol.predict_inlier()
+            if f == tune_at:
+                res = vol.tune_integrate(reps=2)
+            vol.integrate_psdf()
+        d = vol.dump_blocks()
+        st = vol.status()
+        vol.close()
+        return {k: d[k].view(np.uint32) for k in ("sdf", "weight", "a", "b")}, st["voxel_updates"], res
+
+    want, upd, _ = run()
+    got, upd2, res = run(tune_at=2)
+    print(f"[tune psdf] {res.workgroup_size} lanes x {res.workgroups_per_cu} wg/CU: {res.best_ms:.4f} ms "
+          f"vs default {res.default_ms:.4f} ms")
+    assert res.candidates == 15 and res.workgroup_size in (128, 256, 512) and res.best_ms <= res.default_ms
+    assert upd2 == upd
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+    for t in range(3):
+        got, _, _ = run(tiling=(t, 4))
+        for k in want:
+            assert np.array_equal(got[k], want[k]), (t, k)
