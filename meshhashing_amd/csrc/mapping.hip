@@ -253,69 +253,141 @@ __global__ __launch_bounds__(256) void alloc_kernel(Src src_, const SceneFrame* 
   flag(V, MH_STATUS_DDA_CAP);
 }
 
-// Serial list insertion for keys whose bucket was full (hash_table.h:145-177):
-// probe offsets 1..linked_list_size from the bucket's last slot, skipping
-// multiples of bucket_size, and link the first free slot in as list head.
-// After a frame's allocation (kClearWork) it also clears the worklist
-// counter for the collect pass that follows; loading blocks leaves the
-// worklist as it is.  One workgroup per scene (blockIdx.x), one working lane.
+// Overflow keys (their bucket was full when the allocation pass met them):
+// the reference's list insertion (hash_table.h:145-177): probe offsets
+// 1..linked_list_size from the bucket's last slot, skipping multiples of
+// bucket_size, and link the first free slot in as the list's head.  One
+// 1024-lane workgroup per scene (blockIdx.x) works the queue in chunks of
+// 1024 keys: a chunk's duplicates (one key is queued by every ray that met
+// it) are dropped through an LDS set, keys already in the table are
+// skipped, and the rest are inserted in parallel.  A slot is claimed by a
+// CAS on its key and gets a pool index before it is linked; the link is a
+// CAS on the head offset with the slot's own offset written first, so a
+// concurrent walk of the list never reads a half-linked slot.  Slots of one
+// list can come out in any order (the reference's order is a race too).
+// After a frame's allocation (kClearWork) the kernel also clears the
+// worklist counter for the collect pass that follows; loading blocks leaves
+// the worklist as it is.
+constexpr uint32_t kListChunk = 1024;
+constexpr uint32_t kListSet = 2 * kListChunk;  // LDS set of a chunk's keys
+
+// key_present with device-coherent loads (slots are changed by other lanes
+// of the same pass).
+__device__ __forceinline__ bool key_present_coherent(const View& V, uint64_t key, uint32_t first) {
+  for (uint32_t j = 0; j < V.bucket_size; ++j)
+    if (__hip_atomic_load(&V.entries[first + j].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == key) return true;
+  const uint32_t last = first + V.bucket_size - 1;
+  uint32_t i = last;
+  for (uint32_t it = 0; it < V.linked_list_size; ++it) {
+    if (__hip_atomic_load(&V.entries[i].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == key) return true;
+    const uint32_t o = __hip_atomic_load(&V.entries[i].offset, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (o == 0) break;
+    i = (last + o) % V.entry_count;
+  }
+  return false;
+}
+
+// Link slot i (offset o from the bucket's last slot) in as the list's head.
+__device__ __forceinline__ void list_link(const View& V, uint32_t last, uint32_t i, uint32_t o) {
+  uint32_t* head = &V.entries[last].offset;
+  uint32_t old = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (true) {
+    __hip_atomic_store(&V.entries[i].offset, old, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();  // the slot's offset before the head that leads to it
+    const uint32_t seen = atomicCAS(head, old, o);
+    if (seen == old) return;
+    old = seen;
+  }
+}
+
 template <class Src, bool kClearWork = true>
-__global__ void list_insert_kernel(Src src_, const SceneFrame* __restrict__ tab) {
+__global__ __launch_bounds__(kListChunk) void list_insert_kernel(Src src_, const SceneFrame* __restrict__ tab) {
   const Src src = with_table(src_, tab);
   const View& V = src.view(blockIdx.x);
-  if (threadIdx.x != 0) return;
+  __shared__ unsigned long long set[kListSet];
   const uint32_t n = min(V.ctr->ovf_count, kOverflowQueue);
-  for (uint32_t q = 0; q < n; ++q) {
-    const uint64_t key = V.ovf[q];
+  for (uint32_t base = 0; base < n; base += kListChunk) {
+    for (uint32_t k = threadIdx.x; k < kListSet; k += blockDim.x) set[k] = kEmptyKey;
+    __syncthreads();
+    const uint32_t q = base + threadIdx.x;
+    bool mine = false;
+    uint64_t key = kEmptyKey;
+    if (q < n) {
+      key = V.ovf[q];
+      // first of its chunk to claim the key in the set
+      uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 53) & (kListSet - 1);
+      while (true) {
+        const unsigned long long old = atomicCAS(&set[h], (unsigned long long)kEmptyKey, (unsigned long long)key);
+        if (old == kEmptyKey) {
+          mine = true;
+          break;
+        }
+        if (old == key) break;
+        h = (h + 1) & (kListSet - 1);
+      }
+    }
+    // presence tests, then (after a barrier) the inserts: a test never runs
+    // beside an insert of the same chunk
     const i3 p = unpack_key(key);
-    const uint32_t first = hash_bucket(p, V.bucket_count) * V.bucket_size;
-    if (key_present(V, key, first)) continue;
-    const uint32_t ob = occ_bit(key);
-    V.occ[ob >> 5] |= 1u << (ob & 31u);
-    bool done = false;
-    for (uint32_t j = 0; j < V.bucket_size && !done; ++j) {
-      Entry* e = &V.entries[first + j];
-      if (e->key == kEmptyKey) {
-        // a slot is written only once a pool index is in hand, so an
-        // exhausted heap (flagged by alloc_block) leaves no key behind
-        const int32_t ptr = alloc_block(V);
-        if (ptr >= 0) {
-          e->key = key;
-          e->ptr = ptr;
-          V.block_keys[ptr] = key;
+    const uint32_t first = mine ? hash_bucket(p, V.bucket_count) * V.bucket_size : 0;
+    const bool need = mine && !key_present_coherent(V, key, first);
+    __syncthreads();
+    if (need) {
+      {
+        const uint32_t ob = occ_bit(key);
+        atomicOr(&V.occ[ob >> 5], 1u << (ob & 31u));
+        bool done = false;
+        // a bucket slot freed since the allocation pass
+        for (uint32_t j = 0; j < V.bucket_size && !done; ++j) {
+          Entry* e = &V.entries[first + j];
+          if (__hip_atomic_load(&e->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kEmptyKey) continue;
+          if (atomicCAS((unsigned long long*)&e->key, (unsigned long long)kEmptyKey, (unsigned long long)key) !=
+              kEmptyKey)
+            continue;
+          const int32_t ptr = alloc_block(V);  // an exhausted heap is flagged there
+          if (ptr >= 0) {
+            e->ptr = ptr;
+            V.block_keys[ptr] = key;
+          } else {
+            __hip_atomic_store(&e->key, kEmptyKey, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          done = true;
         }
-        done = true;
+        const uint32_t last = first + V.bucket_size - 1;
+        uint32_t offset = 0;
+        for (uint32_t it = 0; it < V.linked_list_size && !done; ++it) {
+          ++offset;
+          if (offset % V.bucket_size == 0) continue;
+          const uint32_t i = (last + offset) % V.entry_count;
+          Entry* e = &V.entries[i];
+          if (__hip_atomic_load(&e->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kEmptyKey) continue;
+          if (atomicCAS((unsigned long long*)&e->key, (unsigned long long)kEmptyKey, (unsigned long long)key) !=
+              kEmptyKey)
+            continue;
+          const int32_t ptr = alloc_block(V);
+          if (ptr >= 0) {
+            e->ptr = ptr;
+            V.block_keys[ptr] = key;
+            list_link(V, last, i, offset);
+            atomicAdd(&V.ctr->list_inserts, 1u);
+          } else {
+            __hip_atomic_store(&e->key, kEmptyKey, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          done = true;
+        }
+        if (!done) {
+          atomicAdd(&V.ctr->dropped, 1u);
+          atomicOr(&V.ctr->flags, MH_STATUS_INSERT_DROPPED);
+        }
       }
     }
-    if (done) continue;
-    const uint32_t last = first + V.bucket_size - 1;
-    uint32_t offset = 0;
-    for (uint32_t it = 0; it < V.linked_list_size; ++it) {
-      ++offset;
-      if (offset % V.bucket_size == 0) continue;
-      const uint32_t i = (last + offset) % V.entry_count;
-      Entry* e = &V.entries[i];
-      if (e->key == kEmptyKey) {
-        const int32_t ptr = alloc_block(V);
-        if (ptr >= 0) {
-          e->key = key;
-          e->offset = V.entries[last].offset;
-          e->ptr = ptr;
-          V.block_keys[ptr] = key;
-          V.entries[last].offset = offset;
-          V.ctr->list_inserts += 1u;
-        }
-        done = true;
-        break;
-      }
-    }
-    if (!done) {
-      V.ctr->dropped += 1u;
-      V.ctr->flags |= MH_STATUS_INSERT_DROPPED;
-    }
+    __threadfence();
+    __syncthreads();  // the chunk's inserts before the next chunk's presence tests
   }
-  V.ctr->ovf_count = 0;
-  if (kClearWork) V.ctr->cand_count = 0;
+  if (threadIdx.x == 0) {
+    V.ctr->ovf_count = 0;
+    if (kClearWork) V.ctr->cand_count = 0;
+  }
 }
 
 // --------------------------------------------------------------- compaction
@@ -787,7 +859,7 @@ __global__ __launch_bounds__(256) void load_scatter_kernel(View V, const uint64_
 
 void launch_load_insert(const View& V, const uint64_t* keys, uint32_t n, hipStream_t s) {
   load_insert_kernel<<<(n + 255) / 256, 256, 0, s>>>(V, keys, n);
-  list_insert_kernel<One, false><<<1, 64, 0, s>>>(One{V}, nullptr);
+  list_insert_kernel<One, false><<<1, kListChunk, 0, s>>>(One{V}, nullptr);
 }
 
 void launch_load_scatter(const View& V, const uint64_t* keys, uint32_t n, const float* sdf,
@@ -1069,7 +1141,7 @@ void launch_alloc(const View& V, hipStream_t s, const KernelClock* kc) {
   tick(kc, s, "alloc_kernel");
   alloc_kernel<<<grid, 256, 0, s>>>(One{V}, nullptr);
   tick(kc, s, "list_insert_kernel");
-  list_insert_kernel<<<1, 64, 0, s>>>(One{V}, nullptr);
+  list_insert_kernel<<<1, kListChunk, 0, s>>>(One{V}, nullptr);
   tick(kc, s, nullptr);
 }
 
@@ -1123,7 +1195,7 @@ __global__ void batch_reset_counts_kernel(Many src_, const SceneFrame* __restric
 void launch_batch_alloc(const BatchDev& B, hipStream_t s) {
   dim3 grid((B.width + 15) / 16, (B.height + 15) / 16, B.nscenes);
   alloc_kernel<<<grid, 256, 0, s>>>(B.src(), B.frames);
-  list_insert_kernel<<<B.nscenes, 64, 0, s>>>(B.src(), B.frames);
+  list_insert_kernel<<<B.nscenes, kListChunk, 0, s>>>(B.src(), B.frames);
 }
 
 void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid) {

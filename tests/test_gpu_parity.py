@@ -117,6 +117,56 @@ def test_alloc_with_linked_lists(hip_device, oracle_mod):
     assert st["list_inserts"] > 0
 
 
+def test_parallel_list_insert_over_frames(hip_device, oracle_mod):
+    """The overflow lists under load: a 500-bucket table (3 578 blocks in 5 000
+    slots) puts ~200 keys in lists over five frames, each key queued by many
+    rays.  Every
+    frame the block set equals the oracle's (sequential inserts), no key is
+    stored twice, every stored key is found again by a lookup (each list is
+    intact) and fusion through the lists matches the oracle."""
+    cfg = replace(C1, bucket_count=500, value_capacity=1 << 14, linked_list_size=400)
+    vol = Volume(cfg)
+    orc = oracle_mod.Oracle(cfg)
+    total = 0
+    for f in range(0, 10, 2):
+        depth, pose = synth_depth(cfg, f), synth_pose(cfg, f)
+        vol.upload_depth(depth, pose)
+        vol.alloc()
+        vol.collect_visible()
+        vol.integrate_tsdf()
+        orc.frame(depth, pose, mesh=False)
+        st = vol.status()
+        assert st["dropped_inserts"] == 0 and st["flags"] == 0, st
+        rec = vol.dump_hash()
+        assert len({tuple(r[3:]) for r in rec}) == len(rec), f"frame {f}: duplicate keys"
+        g, o = vol.dump_blocks(), orc.dump_blocks()
+        assert np.array_equal(g["pos"], o["pos"]), f"frame {f}: block sets differ"
+        total = st["list_inserts"]
+    g = vol.dump_blocks()
+    o = orc.dump_blocks()
+    obs = o["weight"] > 0
+    assert np.array_equal(g["weight"] > 0, obs)
+    es = np.abs(g["sdf"][obs] - o["sdf"][obs]) / np.maximum(np.abs(o["sdf"][obs]), cfg.truncation)
+    assert es.max() < 1e-5
+    # every stored key is reachable through its bucket and list: the raycast
+    # (hash lookups of every block it meets) equals the oracle's over the
+    # same blocks
+    import torch
+    orc2 = oracle_mod.Oracle(cfg)
+    orc2.load_blocks(g["pos"], g["sdf"], g["weight"], g["color"])
+    cTw = np.linalg.inv(synth_pose(cfg, 5).astype(np.float64)).astype(np.float32)
+    d = torch.empty(cfg.width * cfg.height, dtype=torch.float32, device=hip_device)
+    vol.raycast(cTw, d.data_ptr(), 0, 0)
+    vol.synchronize()
+    dg = d.cpu().numpy().reshape(cfg.height, cfg.width)
+    do = orc2.raycast(cTw)[0]
+    assert np.array_equal(np.isfinite(dg), np.isfinite(do)) and np.isfinite(dg).mean() > 0.5
+    assert np.abs(dg[np.isfinite(dg)] - do[np.isfinite(do)]).max() <= 1e-4
+    print(f"[lists] {total} list inserts, {len(g['pos'])} blocks")
+    assert total > 150 and int((vol.dump_hash()[:, 2] == 0).sum()) == total
+    vol.close()
+
+
 @pytest.mark.parametrize("color", [False, True])
 def test_tsdf_fusion_end_to_end(hip_device, oracle_mod, color):
     cfg = C1
