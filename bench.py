@@ -429,6 +429,7 @@ def main():
 
     import torch
 
+    from meshhashing_amd import _capi as A
     from meshhashing_amd.configs import C2
     from meshhashing_amd.engine import Batch, Volume, synth_depth, synth_pose
 
@@ -493,9 +494,14 @@ def main():
             for i, s in enumerate(stages):
                 ev[s].append((e[i], e[i + 1]))
 
+    # the marching-cubes launch shape is tuned online over the first warmup
+    # frames (each meshes as usual), if there are enough of them
+    if args.warmup > A.MH_NUM_MC_SHAPES:
+        vol.tune_marching_cubes(reps=1)
     for f in range(args.warmup):
         run_frame(f, False)
     torch.cuda.synchronize()
+    mc_tune = vol.marching_cubes_tuning()
     # per-kernel HIP events and the raycast's sampled-block count for the
     # timed frames (the graph leg below runs without them)
     vol.set_diagnostics(kernel_times=True, ray_blocks=True)
@@ -560,6 +566,8 @@ def main():
     bt = Batch([cfg])
     bt.set_stream(stream.cuda_stream)
     bt.set_raycast(out_depth.data_ptr(), 0, out_normal.data_ptr())
+    if mc_tune.done:
+        bt.set_marching_cubes_shape(mc_tune.shape)
     gtune = None
     for f in range(args.warmup):
         bt.frame(base_ptr + f * frame_bytes, poses[f][None])
@@ -614,6 +622,10 @@ def main():
             "default_ms": tune.default_ms},
         "raycast_tile": None if gtune is None or gtune.ray_shape < 0 else
         "{0}x{1}".format(*RAY_TILES[gtune.ray_shape]),
+        "marching_cubes_shape": None if not mc_tune.done else {
+            "workgroups_per_cu": mc_tune.workgroups_per_cu or "all resident",
+            "static_quarters": mc_tune.static_quarters,
+            "tuned_us_per_block": 1e3 * mc_tune.best_ms, "default_us_per_block": 1e3 * mc_tune.default_ms},
         "graph_matches_eager_updates": gupd == cnt["voxel_updates"],
         "counters_per_frame": {k: v / K for k, v in cnt.items()},
         "blocks_allocated": st["block_count"],

@@ -375,8 +375,9 @@ int mh_set_raycast_shape(mh_volume* v, int shape);
  * MH_NUM_MC_SHAPES shapes, shape 0 the default, all with the same mesh.
  * Meshing changes the volume, so tuning runs online: mh_tune_marching_cubes
  * makes the next reps x MH_NUM_MC_SHAPES calls of mh_marching_cubes each run
- * one shape in turn (timed with HIP events; every call still meshes as
- * asked), and the call after them keeps the shape with the least mean time.
+ * one shape in turn (timed with HIP events per worklist block, since
+ * frames differ in size; every call still meshes as asked), and the call
+ * after them keeps the shape with the least mean time per block.
  * mh_marching_cubes_tuning reports the progress (done = 0) or the result;
  * mh_set_marching_cubes_shape sets a shape (and ends any tuning).         */
 typedef struct mh_tune_mc_result {
@@ -385,8 +386,10 @@ typedef struct mh_tune_mc_result {
   int32_t workgroups_per_cu;  /* 0: all resident                            */
   int32_t static_quarters;    /* of the worklist handed out statically      */
   int32_t candidates;
-  float best_ms;              /* mean of the chosen shape's trials          */
-  float default_ms;           /* mean of shape 0's trials                   */
+  float best_ms;              /* mean time per worklist block (ms) of the
+                                 chosen shape's trials                       */
+  float default_ms;           /* the same for shape 0                        */
+  float blocks_per_ms;        /* 1 / best_ms                                 */
 } mh_tune_mc_result;
 #define MH_NUM_MC_SHAPES 9
 int mh_tune_marching_cubes(mh_volume* v, uint32_t reps);
@@ -531,6 +534,9 @@ int mh_batch_frame_u16(mh_batch* b, const uint16_t* depth, int on_device,
 int mh_batch_set_raycast(mh_batch* b, const mh_raycaster_params* p,
                          float* depth_dev, float* vertex_dev, float* normal_dev);
 int mh_batch_set_integrate_tiling(mh_batch* b, int tiling, int workgroups_per_cu);
+/* The marching-cubes shape of every scene of a batch (as chosen by
+ * mh_tune_marching_cubes on a representative volume).                     */
+int mh_batch_set_marching_cubes_shape(mh_batch* b, int shape);
 /* One frame of every scene: convert -> alloc -> collect visible -> TSDF
  * [-> marching cubes (mc: n structs, or NULL for defaults)] [-> raycast].
  * tune_out receives the tuning result with MH_RUN_TUNE (may be NULL).     */

@@ -103,7 +103,7 @@ class TuneResult(C.Structure):
 class TuneMcResult(C.Structure):
     _fields_ = [("done", C.c_int32), ("shape", C.c_int32), ("workgroups_per_cu", C.c_int32),
                 ("static_quarters", C.c_int32), ("candidates", C.c_int32), ("best_ms", C.c_float),
-                ("default_ms", C.c_float)]
+                ("default_ms", C.c_float), ("blocks_per_ms", C.c_float)]
 
 
 class TuneShapeResult(C.Structure):
@@ -214,6 +214,7 @@ SIGNATURES = {
     "mh_tune_marching_cubes": (C.c_int, [VP, C.c_uint32]),
     "mh_marching_cubes_tuning": (C.c_int, [VP, C.POINTER(TuneMcResult)]),
     "mh_set_marching_cubes_shape": (C.c_int, [VP, C.c_int]),
+    "mh_batch_set_marching_cubes_shape": (C.c_int, [VP, C.c_int]),
     "mh_batch_create": (C.c_int, [C.c_uint32, C.POINTER(HashParams), C.POINTER(VolumeParams),
                                   C.POINTER(MeshParams), C.POINTER(SensorParams),
                                   C.POINTER(VP)]),

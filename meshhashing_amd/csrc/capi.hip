@@ -151,12 +151,14 @@ struct mh_volume {
   int psdf_tiling = kDefaultPsdfTiling;  // PSDF fusion workgroup size (kPsdfLanes)
   int psdf_grid_per_cu = 8;
   // online marching-cubes tuning (mh_tune_marching_cubes): trial k of
-  // n_trials runs shape k % kNumMcShapes; its time is read at the next call
+  // n_trials runs shape k % kNumMcShapes; its time per worklist block is
+  // read at the next call (frames differ in size, hence per block)
   struct McTune {
     bool active = false;
     uint32_t trial = 0, n_trials = 0;
     int pending = -1;  // shape whose events are recorded but not yet read
     hipEvent_t a = nullptr, b = nullptr;
+    uint32_t* n_host = nullptr;  // pinned: the pending trial's worklist length
     double total_ms[kNumMcShapes] = {};
     uint32_t runs[kNumMcShapes] = {};
     mh_tune_mc_result result{};
@@ -502,6 +504,7 @@ int mh_volume_destroy(mh_volume* v) {
   for (hipEvent_t e : v->ev_used) (void)hipEventDestroy(e);
   if (v->mct.a) (void)hipEventDestroy(v->mct.a);
   if (v->mct.b) (void)hipEventDestroy(v->mct.b);
+  if (v->mct.n_host) (void)hipHostFree(v->mct.n_host);
   delete v;
   return MH_OK;
 }
@@ -772,10 +775,13 @@ static int mc_tune_collect(mh_volume* v) {
   auto& t = v->mct;
   if (t.pending >= 0) {
     HIP_TRY(hipEventSynchronize(t.b));
+    HIP_TRY(hipStreamSynchronize(v->stream));  // the worklist length copy
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
-    t.total_ms[t.pending] += ms;
-    ++t.runs[t.pending];
+    if (*t.n_host > 0) {
+      t.total_ms[t.pending] += ms / *t.n_host;
+      ++t.runs[t.pending];
+    }
     t.pending = -1;
   }
   if (t.active && t.trial >= t.n_trials) {
@@ -797,6 +803,7 @@ static int mc_tune_collect(mh_volume* v) {
     t.result.candidates = kNumMcShapes;
     t.result.best_ms = (float)best_ms;
     t.result.default_ms = t.runs[0] ? (float)(t.total_ms[0] / t.runs[0]) : INFINITY;
+    t.result.blocks_per_ms = best_ms > 0.0 ? (float)(1.0 / best_ms) : 0.0f;
   }
   return MH_OK;
 }
@@ -816,6 +823,7 @@ int mh_marching_cubes(mh_volume* v, const mh_mc_params* p) {
     HIP_TRY(hipEventRecord(t.a, v->stream));
     launch_marching_cubes(v->V, c, v->mc_nbt, v->stream, v->cus, v->kc(), kMcShapes[k]);
     HIP_TRY(hipEventRecord(t.b, v->stream));
+    HIP_TRY(hipMemcpyAsync(t.n_host, &v->V.ctr->cand_count, sizeof(uint32_t), hipMemcpyDeviceToHost, v->stream));
     t.pending = k;
   } else {
     launch_marching_cubes(v->V, c, v->mc_nbt, v->stream, v->cus, v->kc(), kMcShapes[v->mc_shape]);
@@ -829,6 +837,7 @@ int mh_tune_marching_cubes(mh_volume* v, uint32_t reps) {
   auto& t = v->mct;
   if (!t.a) HIP_TRY(hipEventCreate(&t.a));
   if (!t.b) HIP_TRY(hipEventCreate(&t.b));
+  if (!t.n_host) HIP_TRY(hipHostMalloc((void**)&t.n_host, sizeof(uint32_t), hipHostMallocDefault));
   const int rc = mc_tune_collect(v);  // a trial of an earlier tuning still pending
   if (rc) return rc;
   t.active = true;
@@ -1441,6 +1450,7 @@ struct mh_batch {
   int tiling = kDefaultIntegrateTiling;
   int grid_per_cu = kDefaultGridPerCu;
   int ray_shape = 0;
+  int mc_shape = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint32_t graph_flags = 0;
@@ -1479,7 +1489,7 @@ void batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   launch_batch_collect(B, 1, b->stream, b->cus * 4);
   if (!integrate) return;
   launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
-  if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus);
+  if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape]);
   if (flags & MH_RUN_RAYCAST)
     launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream, b->ray_shape);
 }
@@ -1808,6 +1818,14 @@ int mh_batch_set_raycast(mh_batch* b, const mh_raycaster_params* p, float* depth
   return MH_OK;
 }
 
+int mh_batch_set_marching_cubes_shape(mh_batch* b, int shape) {
+  if (!b) return fail(MH_ERR_ARG, "null batch");
+  if (shape < 0 || shape >= kNumMcShapes) return fail(MH_ERR_ARG, "shape out of range");
+  b->mc_shape = shape;
+  b->drop_graph();
+  return MH_OK;
+}
+
 int mh_batch_set_integrate_tiling(mh_batch* b, int tiling, int grid_per_cu) {
   if (!b) return fail(MH_ERR_ARG, "null batch");
   const int rc = check_tiling(tiling, grid_per_cu);
@@ -1858,7 +1876,8 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
     b->drop_graph();
     const BatchDev B = b->dev();
     launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
-    if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus);
+    if (flags & MH_RUN_MESH)
+      launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape]);
     if (flags & MH_RUN_RAYCAST) {
       // tile shape: each trial writes the outputs and the work counters,
       // which are put back; the chosen shape's run below is the frame's

@@ -547,6 +547,10 @@ class Batch:
         A.check(self.lib.mh_batch_set_integrate_tiling(self.h, tiling, workgroups_per_cu),
                 "mh_batch_set_integrate_tiling")
 
+    def set_marching_cubes_shape(self, shape: int):
+        A.check(self.lib.mh_batch_set_marching_cubes_shape(self.h, shape),
+                "mh_batch_set_marching_cubes_shape")
+
     def run(self, mesh: bool = True, raycast: bool = False, graph: bool = False,
             tune: bool = False) -> A.TuneResult | None:
         flags = ((A.MH_RUN_MESH if mesh else 0) | (A.MH_RUN_RAYCAST if raycast else 0) |

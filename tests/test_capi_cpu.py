@@ -83,4 +83,4 @@ def test_header_constants_match_binding():
         assert m and int(m.group(1)) == getattr(A, name), name
     assert C.sizeof(A.TuneResult) == 40
     assert C.sizeof(A.TuneShapeResult) == 24
-    assert C.sizeof(A.TuneMcResult) == 28
+    assert C.sizeof(A.TuneMcResult) == 32

@@ -306,7 +306,7 @@ def test_tune_marching_cubes_online(hip_device):
     vol = meshed_run(cfg, frames, tune_reps=1)
     r = vol.marching_cubes_tuning()
     print(f"[mc tune] shape {r.shape} ({r.workgroups_per_cu} wg/CU, {r.static_quarters}/4 static): "
-          f"{r.best_ms:.4f} ms vs default {r.default_ms:.4f} ms")
+          f"{1e3 * r.best_ms:.3f} us/block vs default {1e3 * r.default_ms:.3f} us/block")
     assert r.done == 1 and 0 <= r.shape < A.MH_NUM_MC_SHAPES and r.candidates == A.MH_NUM_MC_SHAPES
     assert r.best_ms <= r.default_ms
     assert_same(scene_state(vol), ref, "tuned")
