@@ -496,8 +496,8 @@ def main():
 
     # the marching-cubes launch shape is tuned online over the first warmup
     # frames (each meshes as usual), if there are enough of them
-    if args.warmup > A.MH_NUM_MC_SHAPES:
-        vol.tune_marching_cubes(reps=1)
+    if args.warmup > 2 * A.MH_NUM_MC_SHAPES:
+        vol.tune_marching_cubes(reps=2)
     for f in range(args.warmup):
         run_frame(f, False)
     torch.cuda.synchronize()

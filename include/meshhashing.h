@@ -352,7 +352,8 @@ int mh_set_integrate_tiling(mh_volume* v, int tiling, int workgroups_per_cu);
  * W = 8, 16, 4 or 32 (shapes 0..3); every shape gives identical results.
  * mh_tune_raycast times the march of each shape on one raycast at cTw
  * (outputs to scratch, work counters restored, reps trials each, 0 = 3)
- * and keeps the fastest for the volume's later raycasts.  Synchronises.   */
+ * and keeps the fastest for the volume's later raycasts (shape 0 unless
+ * another is at least 3 % faster).  Synchronises.                         */
 typedef struct mh_tune_shape_result {
   int32_t shape;        /* index of the chosen shape                        */
   int32_t tile_width;   /* pixels                                           */
@@ -377,7 +378,8 @@ int mh_set_raycast_shape(mh_volume* v, int shape);
  * makes the next reps x MH_NUM_MC_SHAPES calls of mh_marching_cubes each run
  * one shape in turn (timed with HIP events per worklist block, since
  * frames differ in size; every call still meshes as asked), and the call
- * after them keeps the shape with the least mean time per block.
+ * after them keeps the shape with the least time per block over its trials
+ * (shape 0 unless another is at least 3 % faster).
  * mh_marching_cubes_tuning reports the progress (done = 0) or the result;
  * mh_set_marching_cubes_shape sets a shape (and ends any tuning).         */
 typedef struct mh_tune_mc_result {
@@ -386,8 +388,8 @@ typedef struct mh_tune_mc_result {
   int32_t workgroups_per_cu;  /* 0: all resident                            */
   int32_t static_quarters;    /* of the worklist handed out statically      */
   int32_t candidates;
-  float best_ms;              /* mean time per worklist block (ms) of the
-                                 chosen shape's trials                       */
+  float best_ms;              /* least time per worklist block (ms) over
+                                 the chosen shape's trials                   */
   float default_ms;           /* the same for shape 0                        */
   float blocks_per_ms;        /* 1 / best_ms                                 */
 } mh_tune_mc_result;

@@ -159,7 +159,7 @@ struct mh_volume {
     int pending = -1;  // shape whose events are recorded but not yet read
     hipEvent_t a = nullptr, b = nullptr;
     uint32_t* n_host = nullptr;  // pinned: the pending trial's worklist length
-    double total_ms[kNumMcShapes] = {};
+    double best_ms[kNumMcShapes] = {};  // least time per worklist block
     uint32_t runs[kNumMcShapes] = {};
     mh_tune_mc_result result{};
   } mct;
@@ -779,17 +779,21 @@ static int mc_tune_collect(mh_volume* v) {
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
     if (*t.n_host > 0) {
-      t.total_ms[t.pending] += ms / *t.n_host;
+      const double per_block = ms / *t.n_host;
+      t.best_ms[t.pending] = t.runs[t.pending] ? fmin(t.best_ms[t.pending], per_block) : per_block;
       ++t.runs[t.pending];
     }
     t.pending = -1;
   }
   if (t.active && t.trial >= t.n_trials) {
+    // least time per block over a shape's trials (the first frame is an
+    // outlier: every block is new); shape 0 stays unless another is at
+    // least 3 % faster
     int best = 0;
-    double best_ms = INFINITY;
-    for (int k = 0; k < kNumMcShapes; ++k) {
-      const double m = t.runs[k] ? t.total_ms[k] / t.runs[k] : INFINITY;
-      if (m < best_ms) {
+    double best_ms = t.runs[0] ? t.best_ms[0] : INFINITY;
+    for (int k = 1; k < kNumMcShapes; ++k) {
+      const double m = t.runs[k] ? t.best_ms[k] : INFINITY;
+      if (m < 0.97 * best_ms) {
         best_ms = m;
         best = k;
       }
@@ -802,7 +806,7 @@ static int mc_tune_collect(mh_volume* v) {
     t.result.static_quarters = kMcShapes[best].static_quarters;
     t.result.candidates = kNumMcShapes;
     t.result.best_ms = (float)best_ms;
-    t.result.default_ms = t.runs[0] ? (float)(t.total_ms[0] / t.runs[0]) : INFINITY;
+    t.result.default_ms = t.runs[0] ? (float)t.best_ms[0] : INFINITY;
     t.result.blocks_per_ms = best_ms > 0.0 ? (float)(1.0 / best_ms) : 0.0f;
   }
   return MH_OK;
@@ -844,7 +848,7 @@ int mh_tune_marching_cubes(mh_volume* v, uint32_t reps) {
   t.trial = 0;
   t.n_trials = (reps ? reps : 1) * kNumMcShapes;
   for (int k = 0; k < kNumMcShapes; ++k) {
-    t.total_ms[k] = 0.0;
+    t.best_ms[k] = 0.0;
     t.runs[k] = 0;
   }
   t.result = mh_tune_mc_result{};
@@ -1392,7 +1396,9 @@ int tune_ray_shapes(hipStream_t s, uint32_t reps, Run run, Restore restore, mh_t
       HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
       tmin = fminf(tmin, ms);
     }
-    if (tmin < best) {
+    // shape 0 is measured first; another replaces the best only if it is
+    // at least 3 % faster (a single frame's timing favours no shape by less)
+    if (k == 0 || tmin < 0.97f * best) {
       best = tmin;
       *best_shape = k;
     }
