@@ -122,6 +122,7 @@ struct mh_volume {
   // Bumped by every call that rebuilds or clears the worklist; the
   // primal-dual neighbour table records the value it was built at.
   uint64_t work_gen = 0;
+  bool cand_cleared = false;  // the worklist counter is known to be 0 (an allocation pass ran)
   uint64_t pd_work_gen = ~0ull;
   double* pd_partials = nullptr;
   double* pd_energy = nullptr;
@@ -685,12 +686,14 @@ int mh_alloc_blocks(mh_volume* v) {
   v->pending_color = nullptr;
   launch_alloc(Va, v->stream, v->kc());
   ++v->work_gen;  // allocation clears the worklist
+  v->cand_cleared = true;
   HIP_TRY(hipGetLastError());
   return MH_OK;
 }
 
 static int collect(mh_volume* v, int frustum, uint32_t* count_host) {
-  launch_collect(v->V, frustum, v->stream, v->cus * 4, v->kc());
+  launch_collect(v->V, frustum, v->stream, v->cus * 4, v->kc(), v->cand_cleared);
+  v->cand_cleared = false;
   ++v->work_gen;
   HIP_TRY(hipGetLastError());
   if (count_host) *count_host = read_counters(v).cand_count;
@@ -1492,7 +1495,7 @@ void batch_free(mh_batch* b) {
 void batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   const BatchDev B = b->dev();
   launch_batch_alloc(B, b->stream);  // converts each scene's raw depth (View.raw) as it goes
-  launch_batch_collect(B, 1, b->stream, b->cus * 4);
+  launch_batch_collect(B, 1, b->stream, b->cus * 4, true);  // the list pass cleared the counts
   if (!integrate) return;
   launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
   if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape]);

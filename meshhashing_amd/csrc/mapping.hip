@@ -300,8 +300,11 @@ __device__ __forceinline__ void list_link(const View& V, uint32_t last, uint32_t
   }
 }
 
+// work_count (a batch's concatenated worklist length, or null) is cleared
+// with the worklist counters.
 template <class Src, bool kClearWork = true>
-__global__ __launch_bounds__(kListChunk) void list_insert_kernel(Src src_, const SceneFrame* __restrict__ tab) {
+__global__ __launch_bounds__(kListChunk) void list_insert_kernel(Src src_, const SceneFrame* __restrict__ tab,
+                                                                 uint32_t* work_count = nullptr) {
   const Src src = with_table(src_, tab);
   const View& V = src.view(blockIdx.x);
   __shared__ unsigned long long set[kListSet];
@@ -387,6 +390,7 @@ __global__ __launch_bounds__(kListChunk) void list_insert_kernel(Src src_, const
   if (threadIdx.x == 0) {
     V.ctr->ovf_count = 0;
     if (kClearWork) V.ctr->cand_count = 0;
+    if (kClearWork && work_count && blockIdx.x == 0) *work_count = 0;
   }
 }
 
@@ -1145,9 +1149,12 @@ void launch_alloc(const View& V, hipStream_t s, const KernelClock* kc) {
   tick(kc, s, nullptr);
 }
 
-void launch_collect(const View& V, int frustum, hipStream_t s, int grid, const KernelClock* kc) {
-  tick(kc, s, "set_u32_kernel");
-  set_u32_kernel<<<1, 64, 0, s>>>(&V.ctr->cand_count, 0u);
+void launch_collect(const View& V, int frustum, hipStream_t s, int grid, const KernelClock* kc,
+                    bool count_cleared) {
+  if (!count_cleared) {  // the allocation pass clears it; otherwise here
+    tick(kc, s, "set_u32_kernel");
+    set_u32_kernel<<<1, 64, 0, s>>>(&V.ctr->cand_count, 0u);
+  }
   tick(kc, s, "collect_kernel");
   collect_kernel<<<grid, 256, 0, s>>>(One{V}, nullptr, frustum, nullptr, nullptr);
   tick(kc, s, nullptr);
@@ -1195,11 +1202,11 @@ __global__ void batch_reset_counts_kernel(Many src_, const SceneFrame* __restric
 void launch_batch_alloc(const BatchDev& B, hipStream_t s) {
   dim3 grid((B.width + 15) / 16, (B.height + 15) / 16, B.nscenes);
   alloc_kernel<<<grid, 256, 0, s>>>(B.src(), B.frames);
-  list_insert_kernel<<<B.nscenes, kListChunk, 0, s>>>(B.src(), B.frames);
+  list_insert_kernel<<<B.nscenes, kListChunk, 0, s>>>(B.src(), B.frames, B.work_count);
 }
 
-void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid) {
-  batch_reset_counts_kernel<<<1, 256, 0, s>>>(B.src(), B.frames, B.nscenes, B.work_count);
+void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid, bool counts_cleared) {
+  if (!counts_cleared) batch_reset_counts_kernel<<<1, 256, 0, s>>>(B.src(), B.frames, B.nscenes, B.work_count);
   const uint32_t gx = (uint32_t)grid / B.nscenes > 0 ? (uint32_t)grid / B.nscenes : 1u;
   collect_kernel<<<dim3(gx, B.nscenes), 256, 0, s>>>(B.src(), B.frames, frustum, B.work, B.work_count);
 }

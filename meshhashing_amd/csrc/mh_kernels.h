@@ -157,8 +157,8 @@ void launch_reset(const View& V, hipStream_t s, int grid);
 void launch_convert_depth(const View& V, const uint16_t* raw, hipStream_t s);
 void launch_convert_color(const View& V, const uint8_t* bgra, hipStream_t s);
 void launch_alloc(const View& V, hipStream_t s, const KernelClock* kc = nullptr);
-void launch_collect(const View& V, int frustum, hipStream_t s, int grid,
-                    const KernelClock* kc = nullptr);
+void launch_collect(const View& V, int frustum, hipStream_t s, int grid, const KernelClock* kc = nullptr,
+                    bool count_cleared = false);
 void launch_integrate_tsdf(const View& V, hipStream_t s, int grid, int tiling,
                            const KernelClock* kc = nullptr);
 void launch_predict_inlier(const View& V, hipStream_t s, int grid);
@@ -180,7 +180,7 @@ void launch_state_restore(const View& V, const float* sdf, const float* weight,
                           const uint32_t* color, hipStream_t s, int grid);
 
 void launch_batch_alloc(const BatchDev& B, hipStream_t s);
-void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid);
+void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid, bool counts_cleared = false);
 void launch_batch_integrate_tsdf(const BatchDev& B, hipStream_t s, int grid, int tiling);
 
 // block recycling (mapping/recycle.cu, core/hash_table.h:183-255)

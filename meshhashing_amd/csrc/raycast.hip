@@ -905,7 +905,9 @@ __global__ __launch_bounds__(256) void ray_range_kernel(Src src_, const SceneFra
 template <class Src>
 void launch_ray_range(Src src, const SceneFrame* tab, const RayConst& R, uint32_t nscenes, hipStream_t s) {
   ray_range_clear_kernel<<<dim3(8, nscenes), 256, 0, s>>>(src, tab, R);
-  ray_range_kernel<<<dim3(128, nscenes), 256, 0, s>>>(src, tab, R);
+  // one lane per pool slot of a 118 k-block volume over the whole chip (the
+  // projections are independent; 128 workgroups had left most CUs idle)
+  ray_range_kernel<<<dim3(nscenes > 1 ? 256 : 1024, nscenes), 256, 0, s>>>(src, tab, R);
 }
 
 template <class Src>
