@@ -438,6 +438,7 @@ int mh_volume_create(const mh_hash_params* hash, const mh_volume_params* vol,
   A(&V.block_keys, cap);
   A(&V.block_heap, cap);
   A(&V.occ, (size_t)1 << (kOccBits - 5));
+  A(&V.occ2, (size_t)1 << (kOcc2Bits - 5));
   A(&V.ctr, 1);
   A(&V.ovf, kOverflowQueue);
   A(&V.sdf, nvox);

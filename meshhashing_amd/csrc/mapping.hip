@@ -32,6 +32,9 @@ __global__ void reset_pool_kernel(View V) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (1u << (kOccBits - 5));
        i += gridDim.x * blockDim.x)
     V.occ[i] = 0u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (1u << (kOcc2Bits - 5));
+       i += gridDim.x * blockDim.x)
+    V.occ2[i] = 0u;
   const uint32_t nv = V.max_vertices;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
     V.vertex_heap[i] = (int32_t)(nv - 1u - i);
@@ -153,8 +156,7 @@ __device__ void insert_block(const View& V, i3 p) {
     const uint64_t old = atomicCAS((unsigned long long*)&e->key, (unsigned long long)kEmptyKey,
                                    (unsigned long long)key);
     if (old == kEmptyKey) {
-      const uint32_t ob = occ_bit(key);
-      atomicOr(&V.occ[ob >> 5], 1u << (ob & 31u));
+      occ_mark(V, key, p);
       const int32_t ptr = alloc_block(V);
       if (ptr < 0) {
         // Heap exhausted (flagged by alloc_block): give the slot back, so
@@ -337,8 +339,7 @@ __global__ __launch_bounds__(kListChunk) void list_insert_kernel(Src src_, const
     __syncthreads();
     if (need) {
       {
-        const uint32_t ob = occ_bit(key);
-        atomicOr(&V.occ[ob >> 5], 1u << (ob & 31u));
+        occ_mark(V, key, p);
         bool done = false;
         // a bucket slot freed since the allocation pass
         for (uint32_t j = 0; j < V.bucket_size && !done; ++j) {

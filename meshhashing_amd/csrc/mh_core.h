@@ -290,6 +290,7 @@ struct View {
   int32_t* block_heap;
   uint32_t capacity;
   uint32_t* occ;         // occupancy filter, 2^kOccBits bits
+  uint32_t* occ2;        // super-block (4^3 blocks) occupancy filter, 2^kOcc2Bits bits
   Counters* ctr;
   uint64_t* ovf;         // overflow queue
   // voxels, pool-index-major, 512 per block
@@ -347,6 +348,25 @@ constexpr int kOccBits = 24;
 MH_HD uint32_t occ_bit(uint64_t key) {
   return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - kOccBits));
 }
+// Super-block filter: one bit per hashed key of a 4 x 4 x 4-block cell, set
+// when any block of the cell is inserted and never cleared within a
+// volume's life, so a clear bit proves the cell holds no block (the
+// raycast steps over such a cell at once).
+constexpr int kOcc2Bits = 20;
+constexpr int kSuperShift = 2;  // blocks per super-block axis: 1 << kSuperShift
+MH_HD uint64_t super_key(i3 b) {
+  return pack_key(i3{b.x >> kSuperShift, b.y >> kSuperShift, b.z >> kSuperShift});
+}
+MH_HD uint32_t occ2_bit(uint64_t skey) {
+  return (uint32_t)((skey * 0xC2B2AE3D27D4EB4Full) >> (64 - kOcc2Bits));
+}
+__device__ __forceinline__ void occ_mark(const View& V, uint64_t key, i3 b) {
+  const uint32_t ob = occ_bit(key);
+  atomicOr(&V.occ[ob >> 5], 1u << (ob & 31u));
+  const uint32_t sb = occ2_bit(super_key(b));
+  atomicOr(&V.occ2[sb >> 5], 1u << (sb & 31u));
+}
+
 __device__ __forceinline__ bool occ_test(const uint32_t* occ, uint64_t key) {
   const uint32_t b = occ_bit(key);
   return (occ[b >> 5] >> (b & 31u)) & 1u;

@@ -282,15 +282,16 @@ __device__ __forceinline__ bool axes_sample(const View& V, BlockCache& cache, Oc
 }
 
 // A ray parameter below which every sample position o + t dir lies more
-// than 0.01 voxel inside the world box [8 b vs, 8 (b + 1) vs)^3 of block b.
-// A sample's lowest corner voxel is vox1(pos - vs/2) per axis, so its
-// lowest-corner block is b exactly on that box; the margin is 20 times the
+// than 0.01 voxel inside the world box [b bs, (b + 1) bs)^3 of cell b, bs
+// its side (8 voxels for a block, 32 for a super-block).  A sample's lowest
+// corner voxel is vox1(pos - vs/2) per axis, so its lowest-corner block lies
+// in cell b exactly on that box; the margin is 20 times the
 // FP32 error of the reference's quotient (pos - vs/2) / vs at 16 m and
 // covers the error of the position.  The box is shrunk by 0.02 voxel, so
 // the approximate divisions (a few ulp of t, well under 1e-5 m) only
 // lower the bound.
-__device__ __forceinline__ float block_exit_t(f3 o, f3 dir, int bx, int by, int bz, float vs) {
-  const float bs = (float)kBlockSide * vs, eps = 0.02f * vs;
+__device__ __forceinline__ float cell_exit_t(f3 o, f3 dir, int bx, int by, int bz, float bs, float vs) {
+  const float eps = 0.02f * vs;
   const float ex = dir.x > 0.0f ? (float)(bx + 1) * bs - eps : (float)bx * bs + eps;
   const float ey = dir.y > 0.0f ? (float)(by + 1) * bs - eps : (float)by * bs + eps;
   const float ez = dir.z > 0.0f ? (float)(bz + 1) * bs - eps : (float)bz * bs + eps;
@@ -299,6 +300,18 @@ __device__ __forceinline__ float block_exit_t(f3 o, f3 dir, int bx, int by, int 
   if (dir.y != 0.0f) te = fminf(te, __fdividef(ey - o.y, dir.y));
   if (dir.z != 0.0f) te = fminf(te, __fdividef(ez - o.z, dir.z));
   return te;
+}
+__device__ __forceinline__ float block_exit_t(f3 o, f3 dir, int bx, int by, int bz, float vs) {
+  return cell_exit_t(o, dir, bx, by, bz, (float)kBlockSide * vs, vs);
+}
+// The same for the super-block (4^3 blocks) holding block b, when no block of
+// it was ever allocated (its filter bit is clear): every sample whose
+// lowest-corner block lies in it is invalid.  Otherwise block_exit_t.
+__device__ __forceinline__ float absent_exit_t(const View& V, f3 o, f3 dir, int bx, int by, int bz, float vs) {
+  const uint32_t sb = occ2_bit(super_key(i3{bx, by, bz}));
+  if ((V.occ2[sb >> 5] >> (sb & 31u)) & 1u) return block_exit_t(o, dir, bx, by, bz, vs);
+  return cell_exit_t(o, dir, bx >> kSuperShift, by >> kSuperShift, bz >> kSuperShift,
+                     (float)(kBlockSide << kSuperShift) * vs, vs);
 }
 
 // The first element >= target of the FP32 sequence t, t + step, ...
@@ -368,10 +381,11 @@ __device__ __forceinline__ bool find_surface(const View& V, const RayConst& R, f
       if (!march_sample(V, cache, oc, mk3(o.x + t * dir.x, o.y + t * dir.y, o.z + t * dir.z), R.inv_vs,
                         &sv)) {
         // Lowest-corner block absent: the following samples inside its box
-        // are invalid too (an invalid sample changes nothing); step over
-        // them with the same FP32 additions, so the t sequence is kept.
+        // (its super-block's, if that never held a block) are invalid too
+        // (an invalid sample changes nothing); step over them with the same
+        // FP32 additions, so the t sequence is kept.
         if (oc.p[0] < 0) {
-          const float t_out = block_exit_t(o, dir, oc.bx, oc.by, oc.bz, V.geo.voxel_size);
+          const float t_out = absent_exit_t(V, o, dir, oc.bx, oc.by, oc.bz, V.geo.voxel_size);
           for (float tn = t + R.step; tn < t_out && tn > t; tn = t + R.step) t = tn;
         }
         continue;
