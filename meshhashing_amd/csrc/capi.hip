@@ -454,6 +454,7 @@ int mh_volume_create(const mh_hash_params* hash, const mh_volume_params* vol,
   A(&V.block_mark, cap);
   A(&V.vertices, mesh->max_vertex_count);
   A(&V.vertex_heap, mesh->max_vertex_count);
+  A(&V.vfree, mesh->max_vertex_count);
   A(&V.cand, cap);
   A(&v->mc_nbt, cap * 27);
   A(&V.cand_flag, cap);

@@ -47,6 +47,22 @@ __device__ __forceinline__ int nb27(int dx, int dy, int dz) {
   return (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
 }
 
+constexpr int kCfg = 9;  // local cube coordinates -1..7
+
+// The edge with owner offset (dx, dy, dz) and axis ax (a constant for
+// constant arguments).
+__host__ __device__ constexpr int edge_with(int dx, int dy, int dz, int ax) {
+  int found = -1;
+  for (int e = 0; e < 12; ++e)
+    if (edge_owner_off(e, 0) == dx && edge_owner_off(e, 1) == dy && edge_owner_off(e, 2) == dz &&
+        edge_axis(e) == ax)
+      found = e;
+  return found;
+}
+static_assert(edge_with(0, 0, 0, 0) >= 0 && edge_with(0, 1, 1, 0) >= 0 && edge_with(1, 0, 1, 1) >= 0 &&
+                  edge_with(1, 1, 0, 2) >= 0,
+              "every owner offset across an edge's axis names an edge");
+
 // The 27 neighbours of every worklist block (x fastest, the block itself at
 // 13) with their meshed-in-this-pass marks, one lane per (item, neighbour),
 // before the surfel pass: the dependent worklist -> key -> occupancy ->
@@ -517,6 +533,11 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
   __shared__ uint8_t meshed[27];
   __shared__ uint8_t cv[kCubeHalo * kCubeHalo * kCubeHalo];
   __shared__ uint8_t own_cube[kBlockVoxels];
+  // configurations of the cubes at local -1..7 (this block and its lower
+  // neighbours), for the recycling of this block's own vertex slots
+  __shared__ uint8_t hcfg[kCfg * kCfg * kCfg];
+  __shared__ int surfels[2];
+  __shared__ uint32_t free_base;
   __shared__ int32_t qptr[kQueue];
   __shared__ uint16_t qkey[kQueue];
   __shared__ uint32_t qn;
@@ -564,7 +585,13 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
     const uint32_t prev = V.cube[idx] & 255u;  // in flight with the shell loads
     stage_shell<PSDF>(V, mc, nbe, S, nb, meshed);
     const uint32_t next = s_next;
-    if (threadIdx.x == 0) qn = 0;
+    int32_t vps[3];  // this voxel's vertex slots (x, y, z edges)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vps[a] = V.vptr[3 * idx + a];
+    if (threadIdx.x == 0) {
+      qn = 0;
+      surfels[0] = surfels[1] = 0;
+    }
     __syncthreads();
     // every cube of the 10^3 halo, the block's own among them: validity for
     // the writer rule, and the configuration of the block's own cubes
@@ -574,10 +601,73 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
                 dz = z < 0 ? -1 : (z >= 8 ? 1 : 0);
       int c;
       const bool valid = cube_index(S, x, y, z, &c);
-      cv[s] = meshed[nb27(dx, dy, dz)] && valid;
+      const int q = nb27(dx, dy, dz);
+      cv[s] = meshed[q] && valid;
       if ((dx | dy | dz) == 0) own_cube[x + 8 * y + 64 * z] = (uint8_t)(valid ? c : 0);
+      if (dx <= 0 && dy <= 0 && dz <= 0) {
+        // what the block's cube word holds after this pass: computed here
+        // for a block meshed in it (its own pass computes the same bits from
+        // the same voxels), stored for one that is not
+        uint32_t hc = 0;
+        if (meshed[q])
+          hc = valid ? (uint32_t)c : 0u;
+        else if (nb[q] >= 0)
+          hc = V.cube[(size_t)nb[q] * kBlockVoxels + (x - 8 * dx) + 8 * (y - 8 * dy) + 64 * (z - 8 * dz)] & 255u;
+        hcfg[(x + 1) + kCfg * (y + 1) + kCfg * kCfg * (z + 1)] = (uint8_t)hc;
+      }
     }
     __syncthreads();
+    // RecycleVerticesKernel on this block's own slots (marching_cubes.cu:
+    // 303-321): a slot is referenced iff one of the four cubes around its edge
+    // (this block and its lower neighbours, whose configurations are all
+    // final above) crosses the edge.  Unreferenced allocated slots are freed
+    // (to the free list; they go on the heap after the pass, so no pop of
+    // this pass can take them).  The surfel counts are the slots allocated
+    // once the pass's claims are done: allocated now, or referenced (a
+    // referenced free slot is claimed in this pass, here or by a lower
+    // neighbour).  No other workgroup touches an unreferenced slot.
+    {
+      const i3 o = devectorize(i);
+      const bool is_inner = o.x >= 1 && o.y >= 1 && o.z >= 1 && o.x < 7 && o.y < 7 && o.z < 7;
+      int inner = 0, boundary = 0;
+      unsigned release = 0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int b1 = a == 0 ? 1 : 0, b2 = a == 2 ? 1 : 2;
+        bool used = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          int d[3] = {0, 0, 0};
+          d[b1] = k & 1;
+          d[b2] = k >> 1;
+          const unsigned m = etab[hcfg[(o.x - d[0] + 1) + kCfg * (o.y - d[1] + 1) + kCfg * kCfg * (o.z - d[2] + 1)]];
+          used = used || ((m >> edge_with(d[0], d[1], d[2], a)) & 1u) != 0;
+        }
+        if (vps[a] >= 0 || used) {
+          if (is_inner) ++inner;
+          else ++boundary;
+        }
+        if (vps[a] >= 0 && !used) release |= 1u << a;
+      }
+      uint32_t nfree;
+      uint32_t rank = block_excl_scan<2>((uint32_t)__popc(release), scan_wave, &nfree);  // <= 3
+      if (nfree) {  // uniform
+        if (threadIdx.x == 0) free_base = atomicAdd(&V.ctr->vfree_count, nfree);
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if (!((release >> a) & 1u)) continue;
+          const int32_t vp = vps[a];
+          Vertex zv{};
+          V.vertices[vp] = zv;
+          V.vptr[3 * idx + a] = kFreePtr;
+          V.vfree[free_base + rank] = vp;
+          ++rank;
+        }
+      }
+      if (inner) atomicAdd(&surfels[0], inner);
+      if (boundary) atomicAdd(&surfels[1], boundary);
+    }
     const i3 base = i3{b.x * kBlockSide, b.y * kBlockSide, b.z * kBlockSide};
     const int cube = own_cube[i];
     V.cube[idx] = (uint16_t)((uint32_t)cube | (prev << 8));
@@ -598,6 +688,10 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
     }
     __syncthreads();
     const uint32_t n_items = nitems;
+    if (threadIdx.x == 0) {
+      V.surfel[2 * ptr + 0] = surfels[0];
+      V.surfel[2 * ptr + 1] = surfels[1];
+    }
     for (uint32_t c0 = 0; c0 < n_items; c0 += kBlockVoxels) {
       // Vertex slot of each item: read it; claim it if free (CAS to the
       // lock value); pop fresh vertices for all claims of the workgroup
@@ -722,31 +816,15 @@ __global__ void mc_mark_kernel(Src src_, const SceneFrame* __restrict__ tab, McC
 // vertex's reference count is likewise the sum over the four cubes around
 // its edge of how many of their table triangles use that edge (edge_refs).
 //
-// RecycleVerticesKernel (marching_cubes.cu:303-321) then frees the block's
-// vertices with no references; it also produces the block's inner/boundary
-// surfel counts (TriangleExtractionKernel counts the allocated vertex slots
-// before recycling, :187-205).  Cube configurations of the block and of its
-// -x/-y/-z neighbours (voxels -1..7) are staged in LDS.
-constexpr int kCfg = 9;  // local cube coordinates -1..7
-
-// The edge with owner offset (dx, dy, dz) and axis ax (a constant for
-// constant arguments).
-__host__ __device__ constexpr int edge_with(int dx, int dy, int dz, int ax) {
-  int found = -1;
-  for (int e = 0; e < 12; ++e)
-    if (edge_owner_off(e, 0) == dx && edge_owner_off(e, 1) == dy && edge_owner_off(e, 2) == dz &&
-        edge_axis(e) == ax)
-      found = e;
-  return found;
-}
-static_assert(edge_with(0, 0, 0, 0) >= 0 && edge_with(0, 1, 1, 0) >= 0 && edge_with(1, 0, 1, 1) >= 0 &&
-                  edge_with(1, 1, 0, 2) >= 0,
-              "every owner offset across an edge's axis names an edge");
+// RecycleVerticesKernel (marching_cubes.cu:303-321) frees a block's
+// vertices with no references and produces its inner/boundary surfel counts
+// (TriangleExtractionKernel counts the allocated vertex slots before
+// recycling, :187-205).  The surfel pass does this for the blocks it meshes;
+// this kernel does it for a list of other blocks (the garbage blocks, after
+// their cubes lost their triangles).  Cube configurations of the block and of
+// its -x/-y/-z neighbours (voxels -1..7) are staged in LDS.
 template <class Src>
-// nbt: the surfel pass's neighbour table of the same worklist (its rows
-// give the blocks at offsets {-1, 0}^3 without hashing), or null.
-__global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, const SceneFrame* __restrict__ tab,
-                                                                  const int32_t* __restrict__ nbt) {
+__global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, const SceneFrame* __restrict__ tab) {
   const Src src = with_table(src_, tab);
   __shared__ int32_t nb[8];  // blocks at offsets (dx, dy, dz) in {-1, 0}^3
   // edges used by the triangles of each staged cube's configuration
@@ -769,12 +847,6 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
     // a block without any is done at once (no slot to free, no surfels).
     int32_t vps[2][3];
     bool any = false;
-    // lanes 1..7: the neighbour-table entry, loaded with the vertex slots
-    int32_t nbe = kFreePtr;
-    if (nbt && threadIdx.x >= 1 && threadIdx.x < 8) {
-      const int t = (int)threadIdx.x;
-      nbe = nbt[(size_t)ci * 27u + nb27(-(t & 1), -((t >> 1) & 1), -((t >> 2) & 1))];
-    }
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -793,8 +865,6 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
       const int t = (int)threadIdx.x;
       if (t == 0) {
         nb[t] = ptr;
-      } else if (nbt) {
-        nb[t] = nbe < 0 ? nbe : (nbe & (kMeshedBit - 1));
       } else {
         const i3 b = unpack_key(V.block_keys[ptr]);
         nb[t] = hash_find(V, i3{b.x - (t & 1), b.y - ((t >> 1) & 1), b.z - ((t >> 2) & 1)});
@@ -879,6 +949,32 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
     __syncthreads();
   }
 }
+
+// The vertex slots the surfel pass freed go on the heap once the pass is over,
+// so that no claim of the pass could pop one (pops and pushes of one stack
+// cannot overlap).  Workgroups (x, scene y) copy strided parts of the scene's
+// free list above the top; the last to finish moves the top.
+template <class Src>
+__global__ __launch_bounds__(256) void mc_push_freed_kernel(Src src_, const SceneFrame* __restrict__ tab) {
+  const Src src = with_table(src_, tab);
+  const View& V = src.view(blockIdx.y);
+  // both unchanged until every workgroup of the scene has counted itself done
+  const uint32_t n = V.ctr->vfree_count;
+  if (n == 0) return;  // the same for all of the scene's workgroups
+  const int32_t top = V.ctr->vertex_top;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    V.vertex_heap[top + 1 + (int32_t)i] = V.vfree[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&V.ctr->vfree_done, 1u) == gridDim.x - 1) {
+      V.ctr->vertex_top = top + (int32_t)n;
+      V.ctr->vfree_count = 0;
+      V.ctr->vfree_done = 0;
+    }
+  }
+}
+constexpr int kPushGrid = 64;  // workgroups per scene
 
 // ---------------------------------------------------------------- recycling
 // RecycleGarbageTrianglesKernel (mapping/recycle.cu:117-141): the garbage
@@ -1117,15 +1213,14 @@ void launch_marching_cubes(const View& V, const McConst& p, int32_t* nbt, hipStr
     mc_surfel_kernel<true><<<grid, kBlockVoxels, 0, s>>>(One{V}, nullptr, p, nbt, q);
   else
     mc_surfel_kernel<false><<<grid, kBlockVoxels, 0, s>>>(One{V}, nullptr, p, nbt, q);
-  tick(kc, s, "mc_recycle_vertices_kernel");
-  mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<One>, 256>(cus), 256, 0, s>>>(One{V},
-                                                                                                   nullptr, nbt);
+  tick(kc, s, "mc_push_freed_kernel");
+  mc_push_freed_kernel<<<kPushGrid, 256, 0, s>>>(One{V}, nullptr);
   tick(kc, s, nullptr);
 }
 
 void launch_gc_recycle_vertices(const View& V, hipStream_t s, int cus) {
   mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<Listed>, 256>(cus), 256, 0, s>>>(
-      Listed{V, V.gc_list, &V.ctr->gc_count}, nullptr, nullptr);
+      Listed{V, V.gc_list, &V.ctr->gc_count}, nullptr);
 }
 
 void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus, McShape shape) {
@@ -1139,8 +1234,7 @@ void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s,
     mc_surfel_kernel<true><<<grid, kBlockVoxels, 0, s>>>(B.src(), B.frames, unused, nbt, q);
   else
     mc_surfel_kernel<false><<<grid, kBlockVoxels, 0, s>>>(B.src(), B.frames, unused, nbt, q);
-  mc_recycle_vertices_kernel<<<resident_grid<mc_recycle_vertices_kernel<Many>, 256>(cus), 256, 0, s>>>(
-      B.src(), B.frames, nbt);
+  mc_push_freed_kernel<<<dim3(kPushGrid, B.nscenes), 256, 0, s>>>(B.src(), B.frames);
 }
 
 }  // namespace mh

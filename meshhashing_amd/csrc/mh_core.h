@@ -242,6 +242,8 @@ struct Counters {
   uint32_t list_inserts;
   uint32_t gc_free_count;  // garbage blocks left with no vertex (to be freed)
   uint32_t mc_next;        // marching cubes: next worklist item handed out on demand
+  uint32_t vfree_count;    // entries of vfree
+  uint32_t vfree_done;     // workgroups done pushing vfree on the vertex heap
   WorkStripe work[kWorkStripes];
 };
 
@@ -301,6 +303,7 @@ struct View {
   float* vb;             // Beta b (PSDF only)
   // mesh units (core/voxel.h:27-60), 512 per block
   int32_t* vptr;         // 3 per voxel: x, y, z edge vertex
+  int32_t* vfree;        // vertex slots freed by the surfel pass, pushed on the heap after it
   uint16_t* cube;        // curr | prev << 8
   int32_t* surfel;       // per block: inner, boundary surfel counts (block.h:17-18)
   uint32_t* block_mark;  // per block: last meshing pass that processed it
