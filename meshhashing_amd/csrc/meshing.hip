@@ -476,14 +476,16 @@ struct McCounts {
   }
 };
 
-// Exclusive scan over the wave of values v < 2^BITS, one bit plane at a time
-// (a ballot and a masked bit count per bit, no lane shuffles); *all gets the
-// wave's sum.
-template <int BITS>
+// Exclusive scan over the wave of values v whose set bits all lie in PLANES,
+// one bit plane at a time (a ballot and a masked bit count per plane, no lane
+// shuffles); *all gets the wave's sum.  Packed fields scan independently as
+// long as no field's sum carries into the next one.
+template <uint32_t PLANES>
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* all) {
   uint32_t before = 0, sum = 0;
 #pragma unroll
-  for (int k = 0; k < BITS; ++k) {
+  for (int k = 0; k < 32; ++k) {
+    if (!((PLANES >> k) & 1u)) continue;
     const uint64_t m = __ballot((v >> k) & 1u);
     before += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << k;
     sum += (uint32_t)__popcll(m) << k;
@@ -492,14 +494,14 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* all) {
   return before;
 }
 
-// Exclusive scan of one value v < 2^BITS per lane over the workgroup (up to
-// eight waves); *total gets the sum.  Every lane must call it; it
-// synchronises the workgroup twice.
-template <int BITS>
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wave_sums, uint32_t* total) {
+// Exclusive scan of one value per lane over the workgroup (up to eight
+// waves), its set bits in PLANES; *total gets the sum.  Every lane must call
+// it; it synchronises the workgroup twice.
+template <uint32_t PLANES>
+__device__ __forceinline__ uint32_t block_excl_scan_planes(uint32_t v, uint32_t* wave_sums, uint32_t* total) {
   const uint32_t wave = threadIdx.x >> 6;
   uint32_t wsum;
-  const uint32_t x = wave_excl_scan<BITS>(v, &wsum);
+  const uint32_t x = wave_excl_scan<PLANES>(v, &wsum);
   if ((threadIdx.x & 63u) == 0) wave_sums[wave] = wsum;
   __syncthreads();
   uint32_t before = 0, all = 0;
@@ -512,6 +514,11 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wave_s
   __syncthreads();
   *total = all;
   return before + x;
+}
+// The same for values v < 2^BITS.
+template <int BITS>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wave_sums, uint32_t* total) {
+  return block_excl_scan_planes<(1u << BITS) - 1u>(v, wave_sums, total);
 }
 
 // Vertex slot of edge e of the cube at local voxel o: the slot of the edge's
@@ -626,11 +633,11 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
     // once the pass's claims are done: allocated now, or referenced (a
     // referenced free slot is claimed in this pass, here or by a lower
     // neighbour).  No other workgroup touches an unreferenced slot.
+    unsigned release = 0;
     {
       const i3 o = devectorize(i);
       const bool is_inner = o.x >= 1 && o.y >= 1 && o.z >= 1 && o.x < 7 && o.y < 7 && o.z < 7;
       int inner = 0, boundary = 0;
-      unsigned release = 0;
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         const int b1 = a == 0 ? 1 : 0, b2 = a == 2 ? 1 : 2;
@@ -649,22 +656,6 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
         }
         if (vps[a] >= 0 && !used) release |= 1u << a;
       }
-      uint32_t nfree;
-      uint32_t rank = block_excl_scan<2>((uint32_t)__popc(release), scan_wave, &nfree);  // <= 3
-      if (nfree) {  // uniform
-        if (threadIdx.x == 0) free_base = atomicAdd(&V.ctr->vfree_count, nfree);
-        __syncthreads();
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          if (!((release >> a) & 1u)) continue;
-          const int32_t vp = vps[a];
-          Vertex zv{};
-          V.vertices[vp] = zv;
-          V.vptr[3 * idx + a] = kFreePtr;
-          V.vfree[free_base + rank] = vp;
-          ++rank;
-        }
-      }
       if (inner) atomicAdd(&surfels[0], inner);
       if (boundary) atomicAdd(&surfels[1], boundary);
     }
@@ -678,16 +669,33 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
     const unsigned em = etab[cube];  // 0 for configurations 0 and 255
     // The crossed edges of the block's cubes become a dense item list
     // (voxel | edge << 9), so the slot work below runs one item per lane
-    // instead of twelve edge rounds on every cube.
+    // instead of twelve edge rounds on every cube.  The same scan ranks the
+    // freed slots (bits 13 and up: the item counts sum to at most 6144).
+    uint32_t free_rank;
     {
-      const uint32_t mine = (uint32_t)__popc(em);
+      const uint32_t mine = (uint32_t)__popc(em) | ((uint32_t)__popc(release) << 13);  // <= 12, <= 3
       uint32_t total;
-      uint32_t at = block_excl_scan<4>(mine, scan_wave, &total);  // mine <= 12
+      const uint32_t at2 = block_excl_scan_planes<0xFu | (0x3u << 13)>(mine, scan_wave, &total);
+      uint32_t at = at2 & 8191u;
+      free_rank = at2 >> 13;
       for (unsigned m = em; m; m &= m - 1u) items[at++] = (uint16_t)(i | ((uint32_t)(__ffs(m) - 1) << 9));
-      if (threadIdx.x == 0) nitems = total;
+      if (threadIdx.x == 0) {
+        nitems = total & 8191u;
+        if (total >> 13) free_base = atomicAdd(&V.ctr->vfree_count, total >> 13);
+      }
     }
     __syncthreads();
     const uint32_t n_items = nitems;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (!((release >> a) & 1u)) continue;
+      const int32_t vp = vps[a];
+      Vertex zv{};
+      V.vertices[vp] = zv;
+      V.vptr[3 * idx + a] = kFreePtr;
+      V.vfree[free_base + free_rank] = vp;
+      ++free_rank;
+    }
     if (threadIdx.x == 0) {
       V.surfel[2 * ptr + 0] = surfels[0];
       V.surfel[2 * ptr + 1] = surfels[1];
