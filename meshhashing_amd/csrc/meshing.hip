@@ -332,7 +332,9 @@ __device__ __forceinline__ int cidx(int x, int y, int z) {
 // key >> 8 & 15), axis key >> 12) into vertex record vp: VertexIntersection
 // (marching_cubes.cu:16-28) in the canonical endpoint order, then
 // AllocateVertexWithMutex's payload (:45-72).
-constexpr uint32_t kQueue = 2048;
+// The surfel pass queues a block's vertex writes in LDS (a block writes ~40
+// on average); past kQueue the lane writes its vertex itself.
+constexpr uint32_t kQueue = 1024;
 template <bool PSDF>
 // Returns true when a stencil sample fell outside the staged shell.
 __device__ __forceinline__ bool write_vertex(const Shell<PSDF>& S, const View& V, const McConst& mc, i3 base,
@@ -540,9 +542,6 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
   __shared__ uint8_t meshed[27];
   __shared__ uint8_t cv[kCubeHalo * kCubeHalo * kCubeHalo];
   __shared__ uint8_t own_cube[kBlockVoxels];
-  // configurations of the cubes at local -1..7 (this block and its lower
-  // neighbours), for the recycling of this block's own vertex slots
-  __shared__ uint8_t hcfg[kCfg * kCfg * kCfg];
   __shared__ int surfels[2];
   __shared__ uint32_t free_base;
   __shared__ int32_t qptr[kQueue];
@@ -550,12 +549,20 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
   __shared__ uint32_t qn;
   __shared__ uint32_t scan_wave[8];
   __shared__ int32_t pop_top;
-  __shared__ uint16_t items[12 * kBlockVoxels];
-  __shared__ uint32_t nitems;
-  // payload exchange: eight lanes per vertex, kLookupLane / 8 vertices a round
+  // One LDS region, three uses that never overlap in time: the
+  // configurations of the cubes at local -1..7 (this block and its lower
+  // neighbours) from the halo loop to the recycling test; the item list from
+  // the item scan to the end of the slot claims; the payload exchange (eight
+  // lanes per vertex, kPayRound vertices a round) in the payload rounds.
+  // Barriers separate each use from the next.
   constexpr uint32_t kPayRound = kLookupLane / 8;
-  __shared__ float pay_s[kPayRound * 8];
-  __shared__ uint8_t pay_ok[kPayRound * 8];
+  __shared__ uint32_t region[6 * kBlockVoxels];
+  uint16_t* const items = reinterpret_cast<uint16_t*>(region);  // up to 12 per cube
+  uint8_t* const hcfg = reinterpret_cast<uint8_t*>(region);
+  float* const pay_s = reinterpret_cast<float*>(region);
+  uint8_t* const pay_ok = reinterpret_cast<uint8_t*>(region + kPayRound * 8);
+  static_assert(kCfg * kCfg * kCfg <= 4 * 6 * kBlockVoxels && kPayRound * 8 * 5 <= 4 * 6 * kBlockVoxels, "region");
+  __shared__ uint32_t nitems;
   // per-configuration crossed edges and triangle counts, read per lane with
   // divergent indices: from LDS rather than the constant tables
   __shared__ uint16_t etab[256];
