@@ -431,7 +431,7 @@ def main():
 
     from meshhashing_amd import _capi as A
     from meshhashing_amd.configs import C2
-    from meshhashing_amd.engine import Batch, Volume, synth_depth, synth_pose
+    from meshhashing_amd.engine import Batch, Volume, mc_tune_calls, synth_depth, synth_pose
 
     cfg = C2
     total = args.warmup + args.steps
@@ -494,11 +494,13 @@ def main():
             for i, s in enumerate(stages):
                 ev[s].append((e[i], e[i + 1]))
 
-    # the marching-cubes launch shape is tuned online over the first warmup
-    # frames (each meshes as usual), if there are enough of them
-    if args.warmup > 2 * A.MH_NUM_MC_SHAPES:
-        vol.tune_marching_cubes(reps=2)
+    # the marching-cubes launch shape is tuned online over warm-up frames
+    # 2.. (each meshes as usual; the first two allocate most blocks), if
+    # there are enough of them
+    mc_tune_from = 2 if args.warmup > 2 + mc_tune_calls(1) else -1
     for f in range(args.warmup):
+        if f == mc_tune_from:
+            vol.tune_marching_cubes(reps=1)
         run_frame(f, False)
     torch.cuda.synchronize()
     mc_tune = vol.marching_cubes_tuning()
@@ -566,14 +568,22 @@ def main():
     bt = Batch([cfg])
     bt.set_stream(stream.cuda_stream)
     bt.set_raycast(out_depth.data_ptr(), 0, out_normal.data_ptr())
-    if mc_tune.done:
+    # The replayed chain tunes its own marching-cubes shape over warm-up
+    # replays 1.. (a replay can prefer another shape than eager launches);
+    # with too few warm-up frames it takes the eager leg's choice.
+    graph_warm = [f for f in range(1, args.warmup) if f != tune_frame]
+    gmc_from = 1 if len(graph_warm) > mc_tune_calls(1) else -1
+    if gmc_from < 0 and mc_tune.done:
         bt.set_marching_cubes_shape(mc_tune.shape)
     gtune = None
     for f in range(args.warmup):
+        if f == gmc_from:
+            bt.tune_marching_cubes(1)
         bt.frame(base_ptr + f * frame_bytes, poses[f][None])
         r = bt.run(mesh=True, raycast=True, graph=f != tune_frame, tune=f == tune_frame)
         gtune = r or gtune
     torch.cuda.synchronize()
+    gmc = bt.marching_cubes_tuning()
     gv = bt.volumes[0]
     g0 = gv.status()
     fev = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
@@ -626,6 +636,10 @@ def main():
             "workgroups_per_cu": mc_tune.workgroups_per_cu or "all resident",
             "static_quarters": mc_tune.static_quarters,
             "tuned_us_per_block": 1e3 * mc_tune.best_ms, "default_us_per_block": 1e3 * mc_tune.default_ms},
+        "marching_cubes_shape_graph": None if not gmc.done else {
+            "workgroups_per_cu": gmc.workgroups_per_cu or "all resident",
+            "static_quarters": gmc.static_quarters,
+            "tuned_ms_per_replay": gmc.best_ms, "default_ms_per_replay": gmc.default_ms},
         "graph_matches_eager_updates": gupd == cnt["voxel_updates"],
         "counters_per_frame": {k: v / K for k, v in cnt.items()},
         "blocks_allocated": st["block_count"],

@@ -375,11 +375,13 @@ int mh_set_raycast_shape(mh_volume* v, int shape);
  * quarters of the worklist handed out statically (the rest on demand);
  * MH_NUM_MC_SHAPES shapes, shape 0 the default, all with the same mesh.
  * Meshing changes the volume, so tuning runs online: mh_tune_marching_cubes
- * makes the next reps x MH_NUM_MC_SHAPES calls of mh_marching_cubes each run
- * one shape in turn (timed with HIP events per worklist block, since
- * frames differ in size; every call still meshes as asked), and the call
- * after them keeps the shape with the least time per block over its trials
- * (shape 0 unless another is at least 3 % faster).
+ * makes the next reps x 2 (MH_NUM_MC_SHAPES - 1) + 1 calls of
+ * mh_marching_cubes alternate shape 0 with each other shape in turn (timed
+ * with HIP events per worklist block, since frames differ in size; every
+ * call still meshes as asked).  Each candidate trial is compared with the
+ * faster of the shape-0 trials just before and after it, since frames also
+ * differ in content; the call after the last trial keeps the candidate
+ * whose worst such ratio is least, if it is at most 0.97, else shape 0.
  * mh_marching_cubes_tuning reports the progress (done = 0) or the result;
  * mh_set_marching_cubes_shape sets a shape (and ends any tuning).         */
 typedef struct mh_tune_mc_result {
@@ -388,9 +390,10 @@ typedef struct mh_tune_mc_result {
   int32_t workgroups_per_cu;  /* 0: all resident                            */
   int32_t static_quarters;    /* of the worklist handed out statically      */
   int32_t candidates;
-  float best_ms;              /* least time per worklist block (ms) over
-                                 the chosen shape's trials                   */
-  float default_ms;           /* the same for shape 0                        */
+  float best_ms;              /* default_ms x the chosen shape's worst ratio
+                                 (default_ms for shape 0)                    */
+  float default_ms;           /* median time per worklist block (ms) of the
+                                 shape-0 trials                              */
   float blocks_per_ms;        /* 1 / best_ms                                 */
 } mh_tune_mc_result;
 #define MH_NUM_MC_SHAPES 9
@@ -537,8 +540,18 @@ int mh_batch_set_raycast(mh_batch* b, const mh_raycaster_params* p,
                          float* depth_dev, float* vertex_dev, float* normal_dev);
 int mh_batch_set_integrate_tiling(mh_batch* b, int tiling, int workgroups_per_cu);
 /* The marching-cubes shape of every scene of a batch (as chosen by
- * mh_tune_marching_cubes on a representative volume).                     */
+ * mh_tune_marching_cubes on a representative volume); ends a tuning.       */
 int mh_batch_set_marching_cubes_shape(mh_batch* b, int shape);
+/* Online tuning of the batch's marching-cubes shape over its graph replays
+ * (no counterpart in the reference; a replayed chain can run faster with
+ * another shape than eager launches do): the next
+ * reps x 2 (MH_NUM_MC_SHAPES - 1) + 1 replays with MH_RUN_MESH alternate
+ * shape 0 with each other shape, each timed whole (HIP events around the
+ * replay), and the replay after them keeps the shape chosen by the rule of
+ * mh_tune_marching_cubes.  best_ms and default_ms are then ms per replay
+ * and blocks_per_ms is 0.  Every replay still runs the full chain.         */
+int mh_batch_tune_marching_cubes(mh_batch* b, uint32_t reps);
+int mh_batch_marching_cubes_tuning(mh_batch* b, mh_tune_mc_result* out);
 /* One frame of every scene: convert -> alloc -> collect visible -> TSDF
  * [-> marching cubes (mc: n structs, or NULL for defaults)] [-> raycast].
  * tune_out receives the tuning result with MH_RUN_TUNE (may be NULL).     */

@@ -215,6 +215,8 @@ SIGNATURES = {
     "mh_marching_cubes_tuning": (C.c_int, [VP, C.POINTER(TuneMcResult)]),
     "mh_set_marching_cubes_shape": (C.c_int, [VP, C.c_int]),
     "mh_batch_set_marching_cubes_shape": (C.c_int, [VP, C.c_int]),
+    "mh_batch_tune_marching_cubes": (C.c_int, [VP, C.c_uint32]),
+    "mh_batch_marching_cubes_tuning": (C.c_int, [VP, C.POINTER(TuneMcResult)]),
     "mh_batch_create": (C.c_int, [C.c_uint32, C.POINTER(HashParams), C.POINTER(VolumeParams),
                                   C.POINTER(MeshParams), C.POINTER(SensorParams),
                                   C.POINTER(VP)]),

@@ -99,6 +99,27 @@ struct Scratch {
 
 }  // namespace
 
+// Online marching-cubes tuning (of a volume's meshing calls or a batch's
+// graph replays): trial p of n_trials runs mc_trial_shape(p); its time is
+// read at the next call, per worklist block for a volume (n_host set: frames
+// differ in size), per frame for a batch.
+struct McTune {
+  bool active = false;
+  uint32_t trial = 0, n_trials = 0;
+  int pending = -1;  // shape whose events are recorded but not yet read
+  hipEvent_t a = nullptr, b = nullptr;
+  uint32_t* n_host = nullptr;  // pinned: the pending trial's worklist length (volumes)
+  std::vector<double> ms;      // each trial's time, in order
+  mh_tune_mc_result result{};
+};
+
+static void mc_tune_free(McTune& t) {
+  if (t.a) (void)hipEventDestroy(t.a);
+  if (t.b) (void)hipEventDestroy(t.b);
+  if (t.n_host) (void)hipHostFree(t.n_host);
+  t = McTune{};
+}
+
 struct mh_volume {
   mh_hash_params hp;
   mh_volume_params vp;
@@ -151,19 +172,7 @@ struct mh_volume {
   int mc_shape = 0;                      // marching-cubes launch shape (kMcShapes)
   int psdf_tiling = kDefaultPsdfTiling;  // PSDF fusion workgroup size (kPsdfLanes)
   int psdf_grid_per_cu = 8;
-  // online marching-cubes tuning (mh_tune_marching_cubes): trial k of
-  // n_trials runs shape k % kNumMcShapes; its time per worklist block is
-  // read at the next call (frames differ in size, hence per block)
-  struct McTune {
-    bool active = false;
-    uint32_t trial = 0, n_trials = 0;
-    int pending = -1;  // shape whose events are recorded but not yet read
-    hipEvent_t a = nullptr, b = nullptr;
-    uint32_t* n_host = nullptr;  // pinned: the pending trial's worklist length
-    double best_ms[kNumMcShapes] = {};  // least time per worklist block
-    uint32_t runs[kNumMcShapes] = {};
-    mh_tune_mc_result result{};
-  } mct;
+  McTune mct;  // online marching-cubes tuning (mh_tune_marching_cubes)
   std::vector<void*> owned;
   // diagnostics (mh_volume_set_diagnostics)
   uint32_t diag = 0;
@@ -505,9 +514,7 @@ int mh_volume_destroy(mh_volume* v) {
   if (v->ray_mark) (void)hipFree(v->ray_mark);
   for (hipEvent_t e : v->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : v->ev_used) (void)hipEventDestroy(e);
-  if (v->mct.a) (void)hipEventDestroy(v->mct.a);
-  if (v->mct.b) (void)hipEventDestroy(v->mct.b);
-  if (v->mct.n_host) (void)hipHostFree(v->mct.n_host);
+  mc_tune_free(v->mct);
   delete v;
   return MH_OK;
 }
@@ -774,36 +781,55 @@ int mh_integrate_psdf(mh_volume* v) {
   return MH_OK;
 }
 
-// Reads the last tuning trial's time; after the last trial fixes the
-// fastest shape (mean over its trials).
-static int mc_tune_collect(mh_volume* v) {
-  auto& t = v->mct;
+// Trial p of an online tuning runs shape 0 for even p and candidate
+// 1 + ((p - 1) / 2) % (shapes - 1) for odd p, so that every candidate trial
+// sits between two trials of shape 0 on the frames just before and after.
+static int mc_trial_shape(uint32_t p) {
+  return (p & 1u) ? 1 + (int)(((p - 1u) / 2u) % (uint32_t)(kNumMcShapes - 1)) : 0;
+}
+
+// Reads the last tuning trial's time; after the last trial sets *shape.
+static int mc_tune_collect(McTune& t, hipStream_t stream, int* shape) {
   if (t.pending >= 0) {
     HIP_TRY(hipEventSynchronize(t.b));
-    HIP_TRY(hipStreamSynchronize(v->stream));  // the worklist length copy
+    if (t.n_host) HIP_TRY(hipStreamSynchronize(stream));  // the worklist length copy
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
-    if (*t.n_host > 0) {
-      const double per_block = ms / *t.n_host;
-      t.best_ms[t.pending] = t.runs[t.pending] ? fmin(t.best_ms[t.pending], per_block) : per_block;
-      ++t.runs[t.pending];
-    }
+    t.ms.push_back(!t.n_host ? (double)ms : (*t.n_host > 0 ? ms / *t.n_host : (double)NAN));
     t.pending = -1;
   }
   if (t.active && t.trial >= t.n_trials) {
-    // least time per block over a shape's trials (the first frame is an
-    // outlier: every block is new); shape 0 stays unless another is at
-    // least 3 % faster
+    // Frames differ in content as well as size, and a trial's time drifts
+    // with them (a first frame, where every block is new, costs several
+    // times a later one): each candidate trial is compared with the faster
+    // of the shape-0 trials on either side of it, and a candidate counts
+    // with its worst ratio over its trials.  Shape 0 stays unless a
+    // candidate's worst ratio is at most 0.97.
+    const std::vector<double>& tp = t.ms;
+    double worst[kNumMcShapes];
+    for (double& w : worst) w = 0.0;
+    std::vector<double> dflt;
+    for (size_t p = 0; p < tp.size(); ++p) {
+      if (!(p & 1)) {
+        if (std::isfinite(tp[p])) dflt.push_back(tp[p]);
+        continue;
+      }
+      const double r = p + 1 < tp.size() ? tp[p] / fmin(tp[p - 1], tp[p + 1]) : (double)INFINITY;
+      const int c = mc_trial_shape((uint32_t)p);
+      worst[c] = std::isfinite(r) ? fmax(worst[c], r) : (double)INFINITY;
+    }
     int best = 0;
-    double best_ms = t.runs[0] ? t.best_ms[0] : INFINITY;
-    for (int k = 1; k < kNumMcShapes; ++k) {
-      const double m = t.runs[k] ? t.best_ms[k] : INFINITY;
-      if (m < 0.97 * best_ms) {
-        best_ms = m;
+    double best_r = 0.97;
+    for (int k = 1; k < kNumMcShapes; ++k)
+      if (worst[k] > 0.0 && worst[k] <= best_r) {
+        best_r = worst[k];
         best = k;
       }
-    }
-    v->mc_shape = best;
+    // shape 0's median trial
+    std::sort(dflt.begin(), dflt.end());
+    const double default_ms = dflt.empty() ? (double)INFINITY : dflt[dflt.size() / 2];
+    const double best_ms = best ? default_ms * best_r : default_ms;
+    *shape = best;
     t.active = false;
     t.result.done = 1;
     t.result.shape = best;
@@ -811,9 +837,25 @@ static int mc_tune_collect(mh_volume* v) {
     t.result.static_quarters = kMcShapes[best].static_quarters;
     t.result.candidates = kNumMcShapes;
     t.result.best_ms = (float)best_ms;
-    t.result.default_ms = t.runs[0] ? (float)t.best_ms[0] : INFINITY;
-    t.result.blocks_per_ms = best_ms > 0.0 ? (float)(1.0 / best_ms) : 0.0f;
+    t.result.default_ms = (float)default_ms;
+    t.result.blocks_per_ms = t.n_host && best_ms > 0.0 ? (float)(1.0 / best_ms) : 0.0f;
   }
+  return MH_OK;
+}
+
+// Starts an online tuning of reps rounds (2 (kNumMcShapes - 1) trials each,
+// and one more trial of shape 0).
+static int mc_tune_start(McTune& t, hipStream_t stream, uint32_t reps, bool per_block, int* shape) {
+  if (!t.a) HIP_TRY(hipEventCreate(&t.a));
+  if (!t.b) HIP_TRY(hipEventCreate(&t.b));
+  if (per_block && !t.n_host) HIP_TRY(hipHostMalloc((void**)&t.n_host, sizeof(uint32_t), hipHostMallocDefault));
+  const int rc = mc_tune_collect(t, stream, shape);  // a trial of an earlier tuning still pending
+  if (rc) return rc;
+  t.active = true;
+  t.trial = 0;
+  t.n_trials = (reps ? reps : 1) * 2u * (kNumMcShapes - 1) + 1u;
+  t.ms.clear();
+  t.result = mh_tune_mc_result{};
   return MH_OK;
 }
 
@@ -824,11 +866,11 @@ int mh_marching_cubes(mh_volume* v, const mh_mc_params* p) {
   if (rc) return rc;
   auto& t = v->mct;
   if (t.active || t.pending >= 0) {
-    rc = mc_tune_collect(v);
+    rc = mc_tune_collect(t, v->stream, &v->mc_shape);
     if (rc) return rc;
   }
   if (t.active) {
-    const int k = (int)(t.trial++ % kNumMcShapes);
+    const int k = mc_trial_shape(t.trial++);
     HIP_TRY(hipEventRecord(t.a, v->stream));
     launch_marching_cubes(v->V, c, v->mc_nbt, v->stream, v->cus, v->kc(), kMcShapes[k]);
     HIP_TRY(hipEventRecord(t.b, v->stream));
@@ -843,26 +885,12 @@ int mh_marching_cubes(mh_volume* v, const mh_mc_params* p) {
 
 int mh_tune_marching_cubes(mh_volume* v, uint32_t reps) {
   if (check_vol(v)) return MH_ERR_ARG;
-  auto& t = v->mct;
-  if (!t.a) HIP_TRY(hipEventCreate(&t.a));
-  if (!t.b) HIP_TRY(hipEventCreate(&t.b));
-  if (!t.n_host) HIP_TRY(hipHostMalloc((void**)&t.n_host, sizeof(uint32_t), hipHostMallocDefault));
-  const int rc = mc_tune_collect(v);  // a trial of an earlier tuning still pending
-  if (rc) return rc;
-  t.active = true;
-  t.trial = 0;
-  t.n_trials = (reps ? reps : 1) * kNumMcShapes;
-  for (int k = 0; k < kNumMcShapes; ++k) {
-    t.best_ms[k] = 0.0;
-    t.runs[k] = 0;
-  }
-  t.result = mh_tune_mc_result{};
-  return MH_OK;
+  return mc_tune_start(v->mct, v->stream, reps, true, &v->mc_shape);
 }
 
 int mh_marching_cubes_tuning(mh_volume* v, mh_tune_mc_result* out) {
   if (check_vol(v) || !out) return fail(MH_ERR_ARG, "null argument");
-  const int rc = mc_tune_collect(v);
+  const int rc = mc_tune_collect(v->mct, v->stream, &v->mc_shape);
   if (rc) return rc;
   *out = v->mct.result;
   if (!out->done) {
@@ -1462,6 +1490,7 @@ struct mh_batch {
   int grid_per_cu = kDefaultGridPerCu;
   int ray_shape = 0;
   int mc_shape = 0;
+  McTune mct;  // online marching-cubes tuning over graph replays
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint32_t graph_flags = 0;
@@ -1480,6 +1509,7 @@ namespace {
 void batch_free(mh_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   b->drop_graph();
+  mc_tune_free(b->mct);
   for (int k = 0; k < mh_batch::kRing; ++k) {
     if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
     if (b->pinned[k]) (void)hipHostFree(b->pinned[k]);
@@ -1833,7 +1863,36 @@ int mh_batch_set_marching_cubes_shape(mh_batch* b, int shape) {
   if (!b) return fail(MH_ERR_ARG, "null batch");
   if (shape < 0 || shape >= kNumMcShapes) return fail(MH_ERR_ARG, "shape out of range");
   b->mc_shape = shape;
+  b->mct.active = false;
   b->drop_graph();
+  return MH_OK;
+}
+
+int mh_batch_tune_marching_cubes(mh_batch* b, uint32_t reps) {
+  if (!b) return fail(MH_ERR_ARG, "null batch");
+  int shape = b->mc_shape;
+  const int rc = mc_tune_start(b->mct, b->stream, reps, false, &shape);
+  if (shape != b->mc_shape) {
+    b->mc_shape = shape;
+    b->drop_graph();
+  }
+  return rc;
+}
+
+int mh_batch_marching_cubes_tuning(mh_batch* b, mh_tune_mc_result* out) {
+  if (!b || !out) return fail(MH_ERR_ARG, "null argument");
+  int shape = b->mc_shape;
+  const int rc = mc_tune_collect(b->mct, b->stream, &shape);
+  if (shape != b->mc_shape) {
+    b->mc_shape = shape;
+    b->drop_graph();
+  }
+  if (rc) return rc;
+  *out = b->mct.result;
+  if (!out->done) {
+    out->candidates = kNumMcShapes;
+    out->shape = b->mc_shape;
+  }
   return MH_OK;
 }
 
@@ -1913,6 +1972,25 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
       launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream, b->ray_shape);
     }
   } else if (flags & MH_RUN_GRAPH) {
+    // an online marching-cubes tuning times whole replays, one shape each
+    McTune& t = b->mct;
+    int trial = -1;
+    if (t.active || t.pending >= 0) {
+      int chosen = b->mc_shape;
+      const int rc = mc_tune_collect(t, b->stream, &chosen);
+      if (rc) return rc;
+      if (chosen != b->mc_shape) {
+        b->mc_shape = chosen;
+        b->drop_graph();
+      }
+    }
+    if (t.active && (flags & MH_RUN_MESH)) {
+      trial = mc_trial_shape(t.trial++);
+      if (trial != b->mc_shape) {
+        b->mc_shape = trial;
+        b->drop_graph();
+      }
+    }
     const uint32_t shape = flags & (MH_RUN_MESH | MH_RUN_RAYCAST);
     if (!b->exec || b->graph_flags != shape) {
       b->drop_graph();
@@ -1925,7 +2003,12 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
       HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
       b->graph_flags = shape;
     }
+    if (trial >= 0) HIP_TRY(hipEventRecord(t.a, b->stream));
     HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    if (trial >= 0) {
+      HIP_TRY(hipEventRecord(t.b, b->stream));
+      t.pending = trial;
+    }
   } else {
     batch_chain(b, flags, true);
   }

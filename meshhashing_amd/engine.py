@@ -47,6 +47,11 @@ def synth_depth_exact(cfg: Config, wTc: np.ndarray) -> np.ndarray:
     return d
 
 
+def mc_tune_calls(reps: int = 1) -> int:
+    """Meshing calls an online marching-cubes tuning of reps rounds takes."""
+    return max(reps, 1) * 2 * (A.MH_NUM_MC_SHAPES - 1) + 1
+
+
 class Volume:
     """Hash table + block pool + mesh + frame buffers on the current GPU."""
 
@@ -182,8 +187,10 @@ class Volume:
         A.check(self.lib.mh_set_raycast_shape(self.h, shape), "mh_set_raycast_shape")
 
     def tune_marching_cubes(self, reps: int = 1):
-        """The next reps x MH_NUM_MC_SHAPES marching_cubes() calls try the launch
-        shapes in turn; the call after them keeps the fastest."""
+        """The next mc_tune_calls(reps) marching_cubes() calls alternate launch
+        shape 0 with each other shape in turn; the call after them keeps the
+        shape that beat the shape-0 trials around it by 3 % in every trial,
+        the best such, or shape 0."""
         A.check(self.lib.mh_tune_marching_cubes(self.h, reps), "mh_tune_marching_cubes")
 
     def marching_cubes_tuning(self) -> A.TuneMcResult:
@@ -550,6 +557,19 @@ class Batch:
     def set_marching_cubes_shape(self, shape: int):
         A.check(self.lib.mh_batch_set_marching_cubes_shape(self.h, shape),
                 "mh_batch_set_marching_cubes_shape")
+
+    def tune_marching_cubes(self, reps: int = 1):
+        """The next mc_tune_calls(reps) graph replays with meshing alternate
+        launch shape 0 with each other shape, each replay timed whole; the
+        replay after them keeps the chosen shape (the rule of
+        Volume.tune_marching_cubes)."""
+        A.check(self.lib.mh_batch_tune_marching_cubes(self.h, reps), "mh_batch_tune_marching_cubes")
+
+    def marching_cubes_tuning(self) -> A.TuneMcResult:
+        r = A.TuneMcResult()
+        A.check(self.lib.mh_batch_marching_cubes_tuning(self.h, C.byref(r)),
+                "mh_batch_marching_cubes_tuning")
+        return r
 
     def run(self, mesh: bool = True, raycast: bool = False, graph: bool = False,
             tune: bool = False) -> A.TuneResult | None:

@@ -15,7 +15,7 @@ import pytest
 
 from meshhashing_amd import _capi as A
 from meshhashing_amd.configs import C1, c4_scenes
-from meshhashing_amd.engine import Batch, Volume, synth_depth, synth_pose
+from meshhashing_amd.engine import Batch, Volume, mc_tune_calls, synth_depth, synth_pose
 
 pytestmark = pytest.mark.gpu
 
@@ -80,9 +80,11 @@ def sequential(cfg, frames, hip_device, raycast=True):
     return vol, images
 
 
-def batched(cfgs, frames, hip_device, graph=False, tune_first=False):
+def batched(cfgs, frames, hip_device, graph=False, tune_first=False, mc_tune=False):
     import torch
     b = Batch(cfgs)
+    if mc_tune:
+        b.tune_marching_cubes(1)
     W, H, S = cfgs[0].width, cfgs[0].height, len(cfgs)
     d = torch.empty(S * W * H, dtype=torch.float32, device=hip_device)
     nn = torch.empty(S * W * H * 3, dtype=torch.float32, device=hip_device)
@@ -298,10 +300,11 @@ def test_marching_cubes_shapes_are_output_invariant(hip_device):
 
 
 def test_tune_marching_cubes_online(hip_device):
-    """Online tuning: the first MH_NUM_MC_SHAPES meshing calls each run one
-    shape, the next one keeps the fastest; the mesh is the untuned one."""
+    """Online tuning: the first mc_tune_calls(1) meshing calls alternate shape
+    0 with each other shape, the next one keeps the chosen shape; the mesh is
+    the untuned one."""
     cfg = replace(C1, frames=20)
-    frames = range(A.MH_NUM_MC_SHAPES + 2)
+    frames = range(mc_tune_calls(1) + 1)
     ref = scene_state(meshed_run(cfg, frames))
     vol = meshed_run(cfg, frames, tune_reps=1)
     r = vol.marching_cubes_tuning()
@@ -311,6 +314,29 @@ def test_tune_marching_cubes_online(hip_device):
     assert r.best_ms <= r.default_ms
     assert_same(scene_state(vol), ref, "tuned")
     vol.close()
+
+
+def test_batch_tunes_marching_cubes_over_graph_replays(hip_device):
+    """A batch's online marching-cubes tuning alternates the shapes over its
+    graph replays (a new graph each time the shape changes) and then keeps
+    one: every replay's images and the final scenes equal an untuned run."""
+    cfgs = c4_scenes(n=4, frames=20, seed=3)
+    frames = range(mc_tune_calls(1) + 2)
+    b0, img0, _ = batched(cfgs, frames, hip_device, graph=True)
+    b1, img1, _ = batched(cfgs, frames, hip_device, graph=True, mc_tune=True)
+    r = b1.marching_cubes_tuning()
+    print(f"[batch mc tune] shape {r.shape} ({r.workgroups_per_cu} wg/CU, {r.static_quarters}/4 static): "
+          f"{r.best_ms:.4f} ms vs default {r.default_ms:.4f} ms per replay")
+    assert r.done == 1 and 0 <= r.shape < A.MH_NUM_MC_SHAPES and r.candidates == A.MH_NUM_MC_SHAPES
+    assert r.best_ms <= r.default_ms and r.blocks_per_ms == 0.0
+    for f in frames:
+        for s in range(len(cfgs)):
+            assert np.array_equal(img0[f][s][0], img1[f][s][0]), f"frame {f} scene {s}: depth"
+            assert np.array_equal(img0[f][s][1], img1[f][s][1]), f"frame {f} scene {s}: normal"
+    for s in range(len(cfgs)):
+        assert_same(scene_state(b1.volumes[s]), scene_state(b0.volumes[s]), f"scene {s}")
+    b0.close()
+    b1.close()
 
 
 def test_tune_psdf_integration_changes_nothing(hip_device):
