@@ -866,7 +866,12 @@ __global__ __launch_bounds__(256) void ray_range_clear_kernel(Src src_, const Sc
     R.range[i] = i == 2 * R.tiles ? 0u : ((i & 1u) ? 0u : 0x7f800000u);  // zmin +inf, zmax 0
 }
 
-template <class Src>
+// kLds: a workgroup first merges its blocks' ranges in an LDS copy of the
+// tile table (up to kRangeLdsTiles tiles) and then sends each tile it
+// touched to the global table with one atomic per bound, instead of one per
+// block and tile.
+constexpr uint32_t kRangeLdsTiles = 4800;  // 640 x 480 pixels
+template <class Src, bool kLds>
 __global__ __launch_bounds__(256) void ray_range_kernel(Src src_, const SceneFrame* __restrict__ tab, RayConst R1) {
   const Src src = with_table(src_, tab);
   const uint32_t scene = blockIdx.y;
@@ -875,6 +880,16 @@ __global__ __launch_bounds__(256) void ray_range_kernel(Src src_, const SceneFra
   const uint32_t n = V.ctr->block_hwm;
   const float vs = V.geo.voxel_size;
   const float zcut = 0.5f * R.min_depth;
+  __shared__ uint32_t lds_range[kLds ? 2 * kRangeLdsTiles : 1];
+  uint32_t* const rng = kLds ? lds_range : R.range;
+  if (kLds) {
+    if (R.tiles > kRangeLdsTiles) {  // not launched so; the range test is then switched off
+      if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&R.range[2 * R.tiles], 1u);
+      return;
+    }
+    for (uint32_t t = threadIdx.x; t < 2 * R.tiles; t += blockDim.x) lds_range[t] = (t & 1u) ? 0u : 0x7f800000u;
+    __syncthreads();
+  }
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
     const uint64_t key = V.block_keys[p];
     if (key == kEmptyKey) continue;
@@ -910,18 +925,31 @@ __global__ __launch_bounds__(256) void ray_range_kernel(Src src_, const SceneFra
     for (int ty = y0 >> 3; ty <= (y1 >> 3); ++ty)
       for (int tx = x0 >> 3; tx <= (x1 >> 3); ++tx) {
         const uint32_t t = (uint32_t)ty * R.tiles_x + (uint32_t)tx;
-        atomicMin(&R.range[2 * t], zlo);
-        atomicMax(&R.range[2 * t + 1], zhi);
+        atomicMin(&rng[2 * t], zlo);
+        atomicMax(&rng[2 * t + 1], zhi);
       }
+  }
+  if (kLds) {
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < R.tiles; t += blockDim.x) {
+      const uint32_t lo = lds_range[2 * t], hi = lds_range[2 * t + 1];
+      if (lo != 0x7f800000u) atomicMin(&R.range[2 * t], lo);
+      if (hi != 0u) atomicMax(&R.range[2 * t + 1], hi);
+    }
   }
 }
 
 template <class Src>
-void launch_ray_range(Src src, const SceneFrame* tab, const RayConst& R, uint32_t nscenes, hipStream_t s) {
+void launch_ray_range(Src src, const SceneFrame* tab, const RayConst& R, uint32_t tiles, uint32_t nscenes,
+                      hipStream_t s) {
   ray_range_clear_kernel<<<dim3(8, nscenes), 256, 0, s>>>(src, tab, R);
-  // one lane per pool slot of a 118 k-block volume over the whole chip (the
-  // projections are independent; 128 workgroups had left most CUs idle)
-  ray_range_kernel<<<dim3(nscenes > 1 ? 256 : 1024, nscenes), 256, 0, s>>>(src, tab, R);
+  if (tiles <= kRangeLdsTiles) {
+    ray_range_kernel<Src, true><<<dim3(nscenes > 1 ? 64 : 128, nscenes), 256, 0, s>>>(src, tab, R);
+  } else {
+    // one lane per pool slot of a 118 k-block volume over the whole chip
+    // (the projections are independent; 128 workgroups had left most CUs idle)
+    ray_range_kernel<Src, false><<<dim3(nscenes > 1 ? 256 : 1024, nscenes), 256, 0, s>>>(src, tab, R);
+  }
 }
 
 template <class Src>
@@ -938,7 +966,7 @@ void raycast_shaped(Src src, const SceneFrame* tab, const RayConst& p, uint32_t 
 }
 
 void launch_ray_range_one(const View& V, const RayConst& p, hipStream_t s) {
-  launch_ray_range(One{V}, nullptr, p, 1, s);
+  launch_ray_range(One{V}, nullptr, p, p.tiles, 1, s);
 }
 
 void launch_raycast_march(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,
@@ -950,7 +978,7 @@ void launch_raycast(const View& V, const RayConst& p, float* depth, float* verte
                     hipStream_t s, const KernelClock* kc, int shape) {
   if (p.range) {
     tick(kc, s, "ray_range_kernel");
-    launch_ray_range(One{V}, nullptr, p, 1, s);
+    launch_ray_range(One{V}, nullptr, p, p.tiles, 1, s);
   }
   tick(kc, s, "raycast_kernel");
   raycast_shaped(One{V}, nullptr, p, p.cam.width, p.cam.height, 1, shape, depth, vertex, normal, s);
@@ -979,7 +1007,7 @@ void launch_raycast_backward(const View& V, const RayConst& p, const float* obs,
                              double* partials, double* out, float max_residual, float max_slope,
                              hipStream_t s) {
   dim3 grid((p.cam.width + 7) / 8, (p.cam.height + 7) / 8);
-  if (p.range) launch_ray_range(One{V}, nullptr, p, 1, s);
+  if (p.range) launch_ray_range(One{V}, nullptr, p, p.tiles, 1, s);
   raycast_backward_kernel<<<grid, 64, 0, s>>>(V, p, obs, grad_sdf, partials, max_residual,
                                               max_slope);
   slot_reduce_kernel<<<1, 64, 0, s>>>(partials, grid.x * grid.y, kGradSlots, out);
@@ -987,7 +1015,7 @@ void launch_raycast_backward(const View& V, const RayConst& p, const float* obs,
 
 void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,
                           hipStream_t s, int shape) {
-  launch_ray_range(B.src(), B.frames, RayConst{}, B.nscenes, s);
+  launch_ray_range(B.src(), B.frames, RayConst{}, ((B.width + 7) / 8) * ((B.height + 7) / 8), B.nscenes, s);
   raycast_shaped(B.src(), B.frames, RayConst{}, B.width, B.height, B.nscenes, shape, depth, vertex, normal, s);
 }
 
