@@ -117,10 +117,25 @@ __global__ void convert_color_kernel(const uint32_t* __restrict__ bgra, uint32_t
 // ------------------------------------------------------------- allocation
 // Plain-load presence test (bucket, then list).  Keys are never removed
 // inside a frame, so a hit is final; a miss is re-checked coherently.
+// The occupancy filter first (every insertion sets the key's bit, so a clear
+// bit means absent); then the bucket, its slots loaded together (one memory
+// round trip for the usual ten), then the list.
 __device__ __forceinline__ bool key_present(const View& V, uint64_t key, uint32_t first) {
-  for (uint32_t j = 0; j < V.bucket_size; ++j)
-    if (V.entries[first + j].key == key) return true;
+  if (!occ_test(V.occ, key)) return false;
   const uint32_t last = first + V.bucket_size - 1;
+  if (V.bucket_size == 10) {
+    uint64_t k[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) k[j] = V.entries[first + j].key;
+    bool hit = false;
+#pragma unroll
+    for (int j = 0; j < 10; ++j) hit = hit || k[j] == key;
+    if (hit) return true;
+  } else {
+    bool hit = false;
+    for (uint32_t j = 0; j < V.bucket_size; ++j) hit = hit || V.entries[first + j].key == key;
+    if (hit) return true;
+  }
   uint32_t i = last;
   for (uint32_t it = 0; it < V.linked_list_size; ++it) {
     const Entry e = V.entries[i];
@@ -144,13 +159,29 @@ __device__ void insert_block(const View& V, i3 p) {
   const uint32_t first = hash_bucket(p, V.bucket_count) * V.bucket_size;
   if (key_present(V, key, first)) return;
   for (uint32_t round = 0; round <= V.bucket_size; ++round) {
+    // every slot's key read before any is tested (no early exit between the
+    // loads, so they are in flight together)
     int empty = -1;
-    for (uint32_t j = 0; j < V.bucket_size; ++j) {
-      const uint64_t k = __hip_atomic_load(&V.entries[first + j].key, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-      if (k == key) return;
-      if (k == kEmptyKey && empty < 0) empty = (int)j;
+    bool seen = false;
+    if (V.bucket_size == 10) {
+      uint64_t k[10];
+#pragma unroll
+      for (int j = 0; j < 10; ++j)
+        k[j] = __hip_atomic_load(&V.entries[first + j].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+      for (int j = 9; j >= 0; --j) {
+        seen = seen || k[j] == key;
+        if (k[j] == kEmptyKey) empty = j;
+      }
+    } else {
+      for (uint32_t j = 0; j < V.bucket_size; ++j) {
+        const uint64_t k = __hip_atomic_load(&V.entries[first + j].key, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        seen = seen || k == key;
+        if (k == kEmptyKey && empty < 0) empty = (int)j;
+      }
     }
+    if (seen) return;
     if (empty < 0) break;
     Entry* e = &V.entries[first + (uint32_t)empty];
     const uint64_t old = atomicCAS((unsigned long long*)&e->key, (unsigned long long)kEmptyKey,
