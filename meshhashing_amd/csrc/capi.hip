@@ -1530,9 +1530,12 @@ void batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   launch_batch_collect(B, 1, b->stream, b->cus * 4, true);  // the list pass cleared the counts
   if (!integrate) return;
   launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
-  if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape]);
+  // with both, the meshing pass's last kernel clears the raycast's tile ranges
+  const bool both = (flags & MH_RUN_MESH) && (flags & MH_RUN_RAYCAST);
+  if (flags & MH_RUN_MESH)
+    launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape], both);
   if (flags & MH_RUN_RAYCAST)
-    launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream, b->ray_shape);
+    launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream, b->ray_shape, both);
 }
 
 int batch_tune(mh_batch* b, uint32_t reps, mh_tune_result* out) {
