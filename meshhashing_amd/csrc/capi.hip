@@ -1527,13 +1527,15 @@ void batch_free(mh_batch* b) {
 void batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   const BatchDev B = b->dev();
   launch_batch_alloc(B, b->stream);  // converts each scene's raw depth (View.raw) as it goes
-  launch_batch_collect(B, 1, b->stream, b->cus * 4, true);  // the list pass cleared the counts
+  // the list pass cleared the counts; with meshing the collect pass also
+  // stamps the blocks for the marching-cubes pass (no marking launch)
+  launch_batch_collect(B, 1, b->stream, b->cus * 4, true, (flags & MH_RUN_MESH) != 0);
   if (!integrate) return;
   launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
   // with both, the meshing pass's last kernel clears the raycast's tile ranges
   const bool both = (flags & MH_RUN_MESH) && (flags & MH_RUN_RAYCAST);
   if (flags & MH_RUN_MESH)
-    launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape], both);
+    launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape], both, true);
   if (flags & MH_RUN_RAYCAST)
     launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream, b->ray_shape, both);
 }

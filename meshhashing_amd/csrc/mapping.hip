@@ -433,9 +433,12 @@ __global__ __launch_bounds__(kListChunk) void list_insert_kernel(Src src_, const
 // wave; the worklist order is not deterministic (neither is the reference's).
 // Scene = blockIdx.y.  A batch also appends (scene, block) items to the
 // concatenated worklist that the batched block kernels run over.
+// stamp (batches): also give each collected block the frame's marching-cubes
+// pass stamp (what mc_mark_kernel would write), so the meshing pass that
+// follows in the chain needs no marking launch.
 template <class Src>
 __global__ __launch_bounds__(256) void collect_kernel(Src src_, const SceneFrame* __restrict__ tab, int frustum, uint32_t* work,
-                                                      uint32_t* work_count) {
+                                                      uint32_t* work_count, bool stamp) {
   const Src src = with_table(src_, tab);
   const uint32_t scene = blockIdx.y;
   const View& V = src.view(scene);
@@ -461,6 +464,7 @@ __global__ __launch_bounds__(256) void collect_kernel(Src src_, const SceneFrame
     if (take) {
       V.cand[out + prefix] = (int32_t)p;
       V.cand_flag[out + prefix] = 0;  // no recycling flag outlives its worklist
+      if (stamp) V.block_mark[p] = tab[scene].mc.pass;
     }
     if (work) {
       gout = __shfl(gout, 0);
@@ -1188,7 +1192,7 @@ void launch_collect(const View& V, int frustum, hipStream_t s, int grid, const K
     set_u32_kernel<<<1, 64, 0, s>>>(&V.ctr->cand_count, 0u);
   }
   tick(kc, s, "collect_kernel");
-  collect_kernel<<<grid, 256, 0, s>>>(One{V}, nullptr, frustum, nullptr, nullptr);
+  collect_kernel<<<grid, 256, 0, s>>>(One{V}, nullptr, frustum, nullptr, nullptr, false);
   tick(kc, s, nullptr);
 }
 
@@ -1237,10 +1241,11 @@ void launch_batch_alloc(const BatchDev& B, hipStream_t s) {
   list_insert_kernel<<<B.nscenes, kListChunk, 0, s>>>(B.src(), B.frames, B.work_count);
 }
 
-void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid, bool counts_cleared) {
+void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid, bool counts_cleared,
+                          bool stamp_mc) {
   if (!counts_cleared) batch_reset_counts_kernel<<<1, 256, 0, s>>>(B.src(), B.frames, B.nscenes, B.work_count);
   const uint32_t gx = (uint32_t)grid / B.nscenes > 0 ? (uint32_t)grid / B.nscenes : 1u;
-  collect_kernel<<<dim3(gx, B.nscenes), 256, 0, s>>>(B.src(), B.frames, frustum, B.work, B.work_count);
+  collect_kernel<<<dim3(gx, B.nscenes), 256, 0, s>>>(B.src(), B.frames, frustum, B.work, B.work_count, stamp_mc);
 }
 
 void launch_batch_integrate_tsdf(const BatchDev& B, hipStream_t s, int grid, int tiling) {

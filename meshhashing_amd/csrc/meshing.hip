@@ -90,6 +90,8 @@ __global__ __launch_bounds__(256) void mc_neighbours_kernel(Src src_, const Scen
                                                              McConst mc1, int32_t* __restrict__ nbt) {
   const Src src = with_table(src_, tab);
   const uint32_t total = src.count() * 27u;
+  // the surfel pass's counter of items handed out on demand (it runs next)
+  if (blockIdx.x == 0 && threadIdx.x == 0) *src.next_ctr() = 0u;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
     const uint32_t ci = i / 27u, t = i % 27u;
     uint32_t scene;
@@ -583,7 +585,8 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
   for (uint32_t ci = blockIdx.x; ci < n;) {
     // the next item, read by every lane after the shell's barriers and
     // rewritten only after the next barrier
-    if (threadIdx.x == 0) s_next = ci + gridDim.x < n_static ? ci + gridDim.x : atomicAdd(src.next_ctr(), 1u);
+    if (threadIdx.x == 0)
+      s_next = ci + gridDim.x < n_static ? ci + gridDim.x : n_static + atomicAdd(src.next_ctr(), 1u);
     uint32_t scene;
     const int32_t ptr = src.item(ci, &scene);
     if (scene != cur_scene) {
@@ -807,11 +810,9 @@ __global__ __launch_bounds__(512, 6) void mc_surfel_kernel(Src src_, const Scene
 
 // Meshed-in-this-pass marks for the writer rule.
 template <class Src>
-__global__ void mc_mark_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1, uint32_t surfel_grid,
-                               uint32_t static_q) {
+__global__ void mc_mark_kernel(Src src_, const SceneFrame* __restrict__ tab, McConst mc1) {
   const Src src = with_table(src_, tab);
   const uint32_t n = src.count();
-  if (blockIdx.x == 0 && threadIdx.x == 0) *src.next_ctr() = mc_static_items(n, surfel_grid, static_q);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     uint32_t scene;
     const int32_t ptr = src.lane_item(i, &scene);
@@ -1229,7 +1230,7 @@ void launch_marching_cubes(const View& V, const McConst& p, int32_t* nbt, hipStr
                         : surfel_grid<mc_surfel_kernel<false, One>>(cus, shape);
   const uint32_t q = (uint32_t)shape.static_quarters;
   tick(kc, s, "mc_mark_kernel");
-  mc_mark_kernel<<<cus, 256, 0, s>>>(One{V}, nullptr, p, (uint32_t)grid, q);
+  mc_mark_kernel<<<cus, 256, 0, s>>>(One{V}, nullptr, p);
   tick(kc, s, "mc_neighbours_kernel");
   mc_neighbours_kernel<<<cus * 8, 256, 0, s>>>(One{V}, nullptr, p, nbt);
   tick(kc, s, "mc_surfel_kernel");
@@ -1248,12 +1249,12 @@ void launch_gc_recycle_vertices(const View& V, hipStream_t s, int cus) {
 }
 
 void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus, McShape shape,
-                                 bool clear_ray_ranges) {
+                                 bool clear_ray_ranges, bool marks_stamped) {
   const McConst unused{};
   const int grid = B.psdf ? surfel_grid<mc_surfel_kernel<true, Many>>(cus, shape)
                           : surfel_grid<mc_surfel_kernel<false, Many>>(cus, shape);
   const uint32_t q = (uint32_t)shape.static_quarters;
-  mc_mark_kernel<<<cus, 256, 0, s>>>(B.src(), B.frames, unused, (uint32_t)grid, q);
+  if (!marks_stamped) mc_mark_kernel<<<cus, 256, 0, s>>>(B.src(), B.frames, unused);
   mc_neighbours_kernel<<<cus * 8, 256, 0, s>>>(B.src(), B.frames, unused, nbt);
   if (B.psdf)
     mc_surfel_kernel<true><<<grid, kBlockVoxels, 0, s>>>(B.src(), B.frames, unused, nbt, q);

@@ -241,7 +241,7 @@ struct Counters {
   uint32_t dda_cap;
   uint32_t list_inserts;
   uint32_t gc_free_count;  // garbage blocks left with no vertex (to be freed)
-  uint32_t mc_next;        // marching cubes: next worklist item handed out on demand
+  uint32_t mc_next;        // marching cubes: worklist items handed out on demand (after the static ones)
   uint32_t vfree_count;    // entries of vfree
   uint32_t vfree_done;     // workgroups done pushing vfree on the vertex heap
   WorkStripe work[kWorkStripes];

@@ -69,7 +69,7 @@ struct Many {
   const SceneFrame* __restrict__ f;
   const uint32_t* __restrict__ work;
   const uint32_t* work_count;
-  uint32_t* dyn;  // items handed out on demand (marching cubes)
+  uint32_t* dyn;  // items handed out on demand (marching cubes), after the static ones
   __device__ __forceinline__ const View& view(uint32_t s) const { return f[s].V; }
   __device__ __forceinline__ uint32_t count() const { return *work_count; }
   __device__ __forceinline__ uint32_t* next_ctr() const { return dyn; }
@@ -180,7 +180,10 @@ void launch_state_restore(const View& V, const float* sdf, const float* weight,
                           const uint32_t* color, hipStream_t s, int grid);
 
 void launch_batch_alloc(const BatchDev& B, hipStream_t s);
-void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid, bool counts_cleared = false);
+// stamp_mc: also stamp the collected blocks with the frame table's MC pass
+// (then launch_batch_marching_cubes(..., marks_stamped = true))
+void launch_batch_collect(const BatchDev& B, int frustum, hipStream_t s, int grid, bool counts_cleared = false,
+                          bool stamp_mc = false);
 void launch_batch_integrate_tsdf(const BatchDev& B, hipStream_t s, int grid, int tiling);
 
 // block recycling (mapping/recycle.cu, core/hash_table.h:183-255)
@@ -208,7 +211,8 @@ void launch_marching_cubes(const View& V, const McConst& p, int32_t* nbt, hipStr
 // clear_ray_ranges: the pass's last kernel also clears every scene's raycast
 // tile ranges (the frame table's RayConst.range) for a raycast that follows
 void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus,
-                                 McShape shape = kMcShapes[0], bool clear_ray_ranges = false);
+                                 McShape shape = kMcShapes[0], bool clear_ray_ranges = false,
+                                 bool marks_stamped = false);
 // mesh compaction (CompressMesh) and an exclusive scan of n uint32 (chunk_sums:
 // ceil(n / 1024) entries)
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* chunk_sums,
