@@ -1468,6 +1468,11 @@ struct mh_batch {
   uint32_t n = 0, W = 0, H = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // the raycast branch of a frame (it reads only the fused TSDF and the
+  // table, so it runs beside the meshing pass): forked from and joined back
+  // into stream with the two events
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   int cus = 256;
   SceneFrame* d_frames = nullptr;
   std::vector<SceneFrame> h_frames;
@@ -1510,6 +1515,10 @@ void batch_free(mh_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   b->drop_graph();
   mc_tune_free(b->mct);
+  if (b->side) (void)hipStreamSynchronize(b->side);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->join) (void)hipEventDestroy(b->join);
+  if (b->side) (void)hipStreamDestroy(b->side);
   for (int k = 0; k < mh_batch::kRing; ++k) {
     if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
     if (b->pinned[k]) (void)hipHostFree(b->pinned[k]);
@@ -1524,20 +1533,30 @@ void batch_free(mh_batch* b) {
 
 // The per-frame chain: convert -> alloc -> collect visible -> TSDF ->
 // [marching cubes] -> [raycast], one launch per stage over all scenes.
-void batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
+int batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   const BatchDev B = b->dev();
   launch_batch_alloc(B, b->stream);  // converts each scene's raw depth (View.raw) as it goes
   // the list pass cleared the counts; with meshing the collect pass also
   // stamps the blocks for the marching-cubes pass (no marking launch)
   launch_batch_collect(B, 1, b->stream, b->cus * 4, true, (flags & MH_RUN_MESH) != 0);
-  if (!integrate) return;
+  if (!integrate) return MH_OK;
   launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
-  // with both, the meshing pass's last kernel clears the raycast's tile ranges
-  const bool both = (flags & MH_RUN_MESH) && (flags & MH_RUN_RAYCAST);
-  if (flags & MH_RUN_MESH)
-    launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape], both, true);
-  if (flags & MH_RUN_RAYCAST)
-    launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream, b->ray_shape, both);
+  if ((flags & MH_RUN_MESH) && (flags & MH_RUN_RAYCAST)) {
+    // The raycast needs only the fused TSDF and the table, the meshing pass
+    // writes neither: the raycast runs on the side stream beside it, so each
+    // fills the other's tail (the raycast's slowest tiles end ~3x after the
+    // median).  Captured, the fork and join become graph edges.
+    HIP_TRY(hipEventRecord(b->fork, b->stream));
+    HIP_TRY(hipStreamWaitEvent(b->side, b->fork, 0));
+    launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->side, b->ray_shape);
+    launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape], false, true);
+    HIP_TRY(hipEventRecord(b->join, b->side));
+    HIP_TRY(hipStreamWaitEvent(b->stream, b->join, 0));
+    return MH_OK;
+  }
+  if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape], false, true);
+  if (flags & MH_RUN_RAYCAST) launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream, b->ray_shape);
+  return MH_OK;
 }
 
 int batch_tune(mh_batch* b, uint32_t reps, mh_tune_result* out) {
@@ -1777,6 +1796,13 @@ int mh_batch_create(uint32_t n_scenes, const mh_hash_params* hash, const mh_volu
     return bail(fail(MH_ERR_HIP, "hipStreamCreate failed"));
   }
   b->own_stream = true;
+  if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess) {
+    b->side = nullptr;
+    return bail(fail(MH_ERR_HIP, "hipStreamCreate failed"));
+  }
+  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(MH_ERR_HIP, "hipEventCreate failed"));
   size_t work_cap = 0;
   for (uint32_t s = 0; s < n_scenes; ++s) {
     mh_volume* v = nullptr;
@@ -1944,7 +1970,8 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
                          hipMemcpyHostToDevice, b->stream));
   HIP_TRY(hipEventRecord(b->ev[k], b->stream));
   if (flags & MH_RUN_TUNE) {
-    batch_chain(b, flags, false);
+    const int crc = batch_chain(b, flags, false);
+    if (crc) return crc;
     HIP_TRY(hipGetLastError());
     const int rc = batch_tune(b, 3, tune_out);
     if (rc) return rc;
@@ -2000,9 +2027,10 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
     if (!b->exec || b->graph_flags != shape) {
       b->drop_graph();
       HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      batch_chain(b, flags, true);
+      const int crc = batch_chain(b, flags, true);
       const hipError_t le = hipGetLastError();
       const hipError_t ee = hipStreamEndCapture(b->stream, &b->graph);
+      if (crc) return crc;
       HIP_TRY(le);
       HIP_TRY(ee);
       HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -2015,7 +2043,8 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
       t.pending = trial;
     }
   } else {
-    batch_chain(b, flags, true);
+    const int crc = batch_chain(b, flags, true);
+    if (crc) return crc;
   }
   HIP_TRY(hipGetLastError());
   return MH_OK;
