@@ -1549,12 +1549,12 @@ int batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
     HIP_TRY(hipEventRecord(b->fork, b->stream));
     HIP_TRY(hipStreamWaitEvent(b->side, b->fork, 0));
     launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->side, b->ray_shape);
-    launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape], false, true);
+    launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape], true);
     HIP_TRY(hipEventRecord(b->join, b->side));
     HIP_TRY(hipStreamWaitEvent(b->stream, b->join, 0));
     return MH_OK;
   }
-  if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape], false, true);
+  if (flags & MH_RUN_MESH) launch_batch_marching_cubes(B, b->d_nbt, b->stream, b->cus, kMcShapes[b->mc_shape], true);
   if (flags & MH_RUN_RAYCAST) launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream, b->ray_shape);
   return MH_OK;
 }

@@ -970,19 +970,10 @@ __global__ __launch_bounds__(256) void mc_recycle_vertices_kernel(Src src_, cons
 // so that no claim of the pass could pop one (pops and pushes of one stack
 // cannot overlap).  Workgroups (x, scene y) copy strided parts of the scene's
 // free list above the top; the last to finish moves the top.
-// With clear_ranges (batches) the same workgroups also clear each scene's
-// raycast tile ranges for the range pass that follows in the frame's chain
-// (the values the range pass starts from: zmin +inf, zmax 0, flag 0).
 template <class Src>
-__global__ __launch_bounds__(256) void mc_push_freed_kernel(Src src_, const SceneFrame* __restrict__ tab,
-                                                            bool clear_ranges) {
+__global__ __launch_bounds__(256) void mc_push_freed_kernel(Src src_, const SceneFrame* __restrict__ tab) {
   const Src src = with_table(src_, tab);
   const View& V = src.view(blockIdx.y);
-  if (clear_ranges) {
-    const RayConst& R = tab[blockIdx.y].ray;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= 2 * R.tiles; i += gridDim.x * blockDim.x)
-      R.range[i] = i == 2 * R.tiles ? 0u : ((i & 1u) ? 0u : 0x7f800000u);
-  }
   // both unchanged until every workgroup of the scene has counted itself done
   const uint32_t n = V.ctr->vfree_count;
   if (n == 0) return;  // the same for all of the scene's workgroups
@@ -1239,7 +1230,7 @@ void launch_marching_cubes(const View& V, const McConst& p, int32_t* nbt, hipStr
   else
     mc_surfel_kernel<false><<<grid, kBlockVoxels, 0, s>>>(One{V}, nullptr, p, nbt, q);
   tick(kc, s, "mc_push_freed_kernel");
-  mc_push_freed_kernel<<<kPushGrid, 256, 0, s>>>(One{V}, nullptr, false);
+  mc_push_freed_kernel<<<kPushGrid, 256, 0, s>>>(One{V}, nullptr);
   tick(kc, s, nullptr);
 }
 
@@ -1249,7 +1240,7 @@ void launch_gc_recycle_vertices(const View& V, hipStream_t s, int cus) {
 }
 
 void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus, McShape shape,
-                                 bool clear_ray_ranges, bool marks_stamped) {
+                                 bool marks_stamped) {
   const McConst unused{};
   const int grid = B.psdf ? surfel_grid<mc_surfel_kernel<true, Many>>(cus, shape)
                           : surfel_grid<mc_surfel_kernel<false, Many>>(cus, shape);
@@ -1260,7 +1251,7 @@ void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s,
     mc_surfel_kernel<true><<<grid, kBlockVoxels, 0, s>>>(B.src(), B.frames, unused, nbt, q);
   else
     mc_surfel_kernel<false><<<grid, kBlockVoxels, 0, s>>>(B.src(), B.frames, unused, nbt, q);
-  mc_push_freed_kernel<<<dim3(kPushGrid, B.nscenes), 256, 0, s>>>(B.src(), B.frames, clear_ray_ranges);
+  mc_push_freed_kernel<<<dim3(kPushGrid, B.nscenes), 256, 0, s>>>(B.src(), B.frames);
 }
 
 }  // namespace mh

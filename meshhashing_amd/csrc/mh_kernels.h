@@ -208,11 +208,9 @@ constexpr McShape kMcShapes[] = {{0, 3}, {0, 2}, {0, 4}, {2, 3}, {2, 2}, {2, 4},
 constexpr int kNumMcShapes = sizeof(kMcShapes) / sizeof(kMcShapes[0]);
 void launch_marching_cubes(const View& V, const McConst& p, int32_t* nbt, hipStream_t s, int cus,
                            const KernelClock* kc = nullptr, McShape shape = kMcShapes[0]);
-// clear_ray_ranges: the pass's last kernel also clears every scene's raycast
-// tile ranges (the frame table's RayConst.range) for a raycast that follows
+// marks_stamped: the collect pass stamped the marks (no marking launch)
 void launch_batch_marching_cubes(const BatchDev& B, int32_t* nbt, hipStream_t s, int cus,
-                                 McShape shape = kMcShapes[0], bool clear_ray_ranges = false,
-                                 bool marks_stamped = false);
+                                 McShape shape = kMcShapes[0], bool marks_stamped = false);
 // mesh compaction (CompressMesh) and an exclusive scan of n uint32 (chunk_sums:
 // ceil(n / 1024) entries)
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* chunk_sums,
@@ -236,9 +234,8 @@ void launch_ray_range_one(const View& V, const RayConst& p, hipStream_t s);
 // diagnostics: add the distinct blocks the last raycast sampled (V.ray_mark
 // == V.ray_stamp) to the work counters
 void launch_ray_mark_count(const View& V, hipStream_t s, int grid);
-// ranges_cleared: the tile ranges were cleared since the last range pass
 void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,
-                          hipStream_t s, int shape = 0, bool ranges_cleared = false);
+                          hipStream_t s, int shape = 0);
 // adjoint: 31 FP64 sums (loss, dL/dxi[6], J^T J upper triangle[21], pixels,
 // gated-out pixels, observed pixels)
 constexpr int kGradSlots = 31;

@@ -941,8 +941,8 @@ __global__ __launch_bounds__(256) void ray_range_kernel(Src src_, const SceneFra
 
 template <class Src>
 void launch_ray_range(Src src, const SceneFrame* tab, const RayConst& R, uint32_t tiles, uint32_t nscenes,
-                      hipStream_t s, bool cleared = false) {
-  if (!cleared) ray_range_clear_kernel<<<dim3(8, nscenes), 256, 0, s>>>(src, tab, R);
+                      hipStream_t s) {
+  ray_range_clear_kernel<<<dim3(8, nscenes), 256, 0, s>>>(src, tab, R);
   if (tiles <= kRangeLdsTiles) {
     ray_range_kernel<Src, true><<<dim3(nscenes > 1 ? 64 : 128, nscenes), 256, 0, s>>>(src, tab, R);
   } else {
@@ -1014,9 +1014,8 @@ void launch_raycast_backward(const View& V, const RayConst& p, const float* obs,
 }
 
 void launch_batch_raycast(const BatchDev& B, float* depth, float* vertex, float* normal,
-                          hipStream_t s, int shape, bool ranges_cleared) {
-  launch_ray_range(B.src(), B.frames, RayConst{}, ((B.width + 7) / 8) * ((B.height + 7) / 8), B.nscenes, s,
-                   ranges_cleared);
+                          hipStream_t s, int shape) {
+  launch_ray_range(B.src(), B.frames, RayConst{}, ((B.width + 7) / 8) * ((B.height + 7) / 8), B.nscenes, s);
   raycast_shaped(B.src(), B.frames, RayConst{}, B.width, B.height, B.nscenes, shape, depth, vertex, normal, s);
 }
 
