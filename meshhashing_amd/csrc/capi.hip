@@ -1541,11 +1541,16 @@ int batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   launch_batch_collect(B, 1, b->stream, b->cus * 4, true, (flags & MH_RUN_MESH) != 0);
   if (!integrate) return MH_OK;
   launch_batch_integrate_tsdf(B, b->stream, b->cus * b->grid_per_cu, b->tiling);
-  if ((flags & MH_RUN_MESH) && (flags & MH_RUN_RAYCAST)) {
-    // The raycast needs only the fused TSDF and the table, the meshing pass
-    // writes neither: the raycast runs on the side stream beside it, so each
-    // fills the other's tail (the raycast's slowest tiles end ~3x after the
-    // median).  Captured, the fork and join become graph edges.
+  // The raycast needs only the fused TSDF and the table, the meshing pass
+  // writes neither: when the raycast's waves (one per 64 pixels) fit on the
+  // chip in one round, it runs on the side stream beside meshing, so each
+  // fills the other's tail (the slowest tiles end ~3x after the median).
+  // With more waves than that there is no idle tail to fill, and sharing
+  // the chip only slowed both (64 scenes of 320 x 240: 14 %).  Captured, the
+  // fork and join become graph edges.
+  const size_t ray_waves = (size_t)b->n * ((b->W * b->H + 63) / 64);
+  const size_t resident = (size_t)kRayWavesPerSimd * 4 * (size_t)b->cus;
+  if ((flags & MH_RUN_MESH) && (flags & MH_RUN_RAYCAST) && ray_waves <= resident) {
     HIP_TRY(hipEventRecord(b->fork, b->stream));
     HIP_TRY(hipStreamWaitEvent(b->side, b->fork, 0));
     launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->side, b->ray_shape);

@@ -225,6 +225,8 @@ void launch_compact_emit(const View& V, uint32_t n, const uint32_t* blk_off, con
 // Raycast tile shapes: tile width in pixels (tile = width x 64 / width).
 constexpr int kRayTileWidths[] = {8, 16, 4, 32};
 constexpr int kNumRayShapes = sizeof(kRayTileWidths) / sizeof(kRayTileWidths[0]);
+// raycast waves resident per SIMD (the kernel is held to 96 registers)
+constexpr int kRayWavesPerSimd = 5;
 void launch_raycast(const View& V, const RayConst& p, float* depth, float* vertex, float* normal,
                     hipStream_t s, const KernelClock* kc = nullptr, int shape = 0);
 // the march alone (the tile ranges already filled)

@@ -815,7 +815,7 @@ __device__ __forceinline__ const RayConst& ray_of(const Many& m, uint32_t s, con
 // TW: tile width in pixels (the tile is TW x 64 / TW; every shape gives the
 // same per-pixel results, the autotuner picks the fastest).
 template <class Src, int TW = 8>
-__global__ __launch_bounds__(64, 5) void raycast_kernel(Src src_, const SceneFrame* __restrict__ tab, RayConst R1, float* __restrict__ depth_out,
+__global__ __launch_bounds__(64, kRayWavesPerSimd) void raycast_kernel(Src src_, const SceneFrame* __restrict__ tab, RayConst R1, float* __restrict__ depth_out,
                                                      float* __restrict__ vertex_out,
                                                      float* __restrict__ normal_out) {
   const Src src = with_table(src_, tab);
