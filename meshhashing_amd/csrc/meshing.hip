@@ -79,7 +79,8 @@ constexpr uint32_t kLookupLane = kBlockVoxels - 64;  // the surfel pass's last w
 // g, g + grid, ...) and the rest on demand from a counter, so workgroups
 // whose blocks held little surface take more of them.  static_q quarters of
 // the worklist (rounded down to whole rounds of the grid, at least one) are
-// static; the counter starts at that bound.
+// static; the counter (cleared by the neighbour-table pass) counts the items
+// handed out after them.
 __device__ __forceinline__ uint32_t mc_static_items(uint32_t n, uint32_t grid, uint32_t static_q) {
   const uint32_t k = (static_q * n / 4u) / grid;
   return (k ? k : 1u) * grid;
