@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "mh_kernels.h"
+#include "mh_tune.h"
 #include "mh_mc_tables.h"
 
 using namespace mh;
@@ -100,9 +101,9 @@ struct Scratch {
 }  // namespace
 
 // Online marching-cubes tuning (of a volume's meshing calls or a batch's
-// graph replays): trial p of n_trials runs mc_trial_shape(p); its time is
-// read at the next call, per worklist block for a volume (n_host set: frames
-// differ in size), per frame for a batch.
+// graph replays): trial p of n_trials runs mc_trial_shape(p, kNumMcShapes);
+// its time is read at the next call, per worklist block for a volume (n_host
+// set: frames differ in size), per frame for a batch.
 struct McTune {
   bool active = false;
   uint32_t trial = 0, n_trials = 0;
@@ -781,14 +782,8 @@ int mh_integrate_psdf(mh_volume* v) {
   return MH_OK;
 }
 
-// Trial p of an online tuning runs shape 0 for even p and candidate
-// 1 + ((p - 1) / 2) % (shapes - 1) for odd p, so that every candidate trial
-// sits between two trials of shape 0 on the frames just before and after.
-static int mc_trial_shape(uint32_t p) {
-  return (p & 1u) ? 1 + (int)(((p - 1u) / 2u) % (uint32_t)(kNumMcShapes - 1)) : 0;
-}
-
-// Reads the last tuning trial's time; after the last trial sets *shape.
+// Reads the last tuning trial's time; after the last trial sets *shape by
+// the rule of mc_pick.
 static int mc_tune_collect(McTune& t, hipStream_t stream, int* shape) {
   if (t.pending >= 0) {
     HIP_TRY(hipEventSynchronize(t.b));
@@ -799,36 +794,9 @@ static int mc_tune_collect(McTune& t, hipStream_t stream, int* shape) {
     t.pending = -1;
   }
   if (t.active && t.trial >= t.n_trials) {
-    // Frames differ in content as well as size, and a trial's time drifts
-    // with them (a first frame, where every block is new, costs several
-    // times a later one): each candidate trial is compared with the faster
-    // of the shape-0 trials on either side of it, and a candidate counts
-    // with its worst ratio over its trials.  Shape 0 stays unless a
-    // candidate's worst ratio is at most 0.97.
-    const std::vector<double>& tp = t.ms;
-    double worst[kNumMcShapes];
-    for (double& w : worst) w = 0.0;
-    std::vector<double> dflt;
-    for (size_t p = 0; p < tp.size(); ++p) {
-      if (!(p & 1)) {
-        if (std::isfinite(tp[p])) dflt.push_back(tp[p]);
-        continue;
-      }
-      const double r = p + 1 < tp.size() ? tp[p] / fmin(tp[p - 1], tp[p + 1]) : (double)INFINITY;
-      const int c = mc_trial_shape((uint32_t)p);
-      worst[c] = std::isfinite(r) ? fmax(worst[c], r) : (double)INFINITY;
-    }
-    int best = 0;
-    double best_r = 0.97;
-    for (int k = 1; k < kNumMcShapes; ++k)
-      if (worst[k] > 0.0 && worst[k] <= best_r) {
-        best_r = worst[k];
-        best = k;
-      }
-    // shape 0's median trial
-    std::sort(dflt.begin(), dflt.end());
-    const double default_ms = dflt.empty() ? (double)INFINITY : dflt[dflt.size() / 2];
-    const double best_ms = best ? default_ms * best_r : default_ms;
+    const McPick pick = mc_pick(t.ms, kNumMcShapes);
+    const int best = pick.shape;
+    const double default_ms = pick.default_ms, best_ms = pick.best_ms;
     *shape = best;
     t.active = false;
     t.result.done = 1;
@@ -870,7 +838,7 @@ int mh_marching_cubes(mh_volume* v, const mh_mc_params* p) {
     if (rc) return rc;
   }
   if (t.active) {
-    const int k = mc_trial_shape(t.trial++);
+    const int k = mc_trial_shape(t.trial++, kNumMcShapes);
     HIP_TRY(hipEventRecord(t.a, v->stream));
     launch_marching_cubes(v->V, c, v->mc_nbt, v->stream, v->cus, v->kc(), kMcShapes[k]);
     HIP_TRY(hipEventRecord(t.b, v->stream));
@@ -2022,7 +1990,7 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
       }
     }
     if (t.active && (flags & MH_RUN_MESH)) {
-      trial = mc_trial_shape(t.trial++);
+      trial = mc_trial_shape(t.trial++, kNumMcShapes);
       if (trial != b->mc_shape) {
         b->mc_shape = trial;
         b->drop_graph();
