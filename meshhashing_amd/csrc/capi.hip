@@ -1516,7 +1516,7 @@ int batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   // With more waves than that there is no idle tail to fill, and sharing
   // the chip only slowed both (64 scenes of 320 x 240: 14 %).  Captured, the
   // fork and join become graph edges.
-  const size_t ray_waves = (size_t)b->n * ((b->W * b->H + 63) / 64);
+  const size_t ray_waves = (size_t)b->n * (((size_t)b->W * b->H + 63) / 64);
   const size_t resident = (size_t)kRayWavesPerSimd * 4 * (size_t)b->cus;
   if ((flags & MH_RUN_MESH) && (flags & MH_RUN_RAYCAST) && ray_waves <= resident) {
     HIP_TRY(hipEventRecord(b->fork, b->stream));
