@@ -303,6 +303,7 @@ int build_ray(const mh_volume* v, const mh_raycaster_params* p, const float cTw[
     return fail(MH_ERR_ARG, "bad raycaster params (sizes, depth range, or > 65536 steps per ray)");
   R->cam = Cam{p->fx, p->fy, p->cx, p->cy, p->width, p->height, p->min_raycast_depth,
                p->max_raycast_depth};
+  R->boost = 0;
   R->min_depth = p->min_raycast_depth;
   R->max_depth = p->max_raycast_depth;
   R->step = p->raycast_step;
@@ -1501,6 +1502,13 @@ void batch_free(mh_batch* b) {
 
 // The per-frame chain: convert -> alloc -> collect visible -> TSDF ->
 // [marching cubes] -> [raycast], one launch per stage over all scenes.
+// Whether a frame's chain runs the raycast beside the meshing pass (below).
+bool batch_overlaps(const mh_batch* b, uint32_t flags) {
+  const size_t ray_waves = (size_t)b->n * (((size_t)b->W * b->H + 63) / 64);
+  const size_t resident = (size_t)kRayWavesPerSimd * 4 * (size_t)b->cus;
+  return (flags & MH_RUN_MESH) && (flags & MH_RUN_RAYCAST) && ray_waves <= resident;
+}
+
 int batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   const BatchDev B = b->dev();
   launch_batch_alloc(B, b->stream);  // converts each scene's raw depth (View.raw) as it goes
@@ -1516,9 +1524,7 @@ int batch_chain(mh_batch* b, uint32_t flags, bool integrate) {
   // With more waves than that there is no idle tail to fill, and sharing
   // the chip only slowed both (64 scenes of 320 x 240: 14 %).  Captured, the
   // fork and join become graph edges.
-  const size_t ray_waves = (size_t)b->n * (((size_t)b->W * b->H + 63) / 64);
-  const size_t resident = (size_t)kRayWavesPerSimd * 4 * (size_t)b->cus;
-  if ((flags & MH_RUN_MESH) && (flags & MH_RUN_RAYCAST) && ray_waves <= resident) {
+  if (batch_overlaps(b, flags)) {
     HIP_TRY(hipEventRecord(b->fork, b->stream));
     HIP_TRY(hipStreamWaitEvent(b->side, b->fork, 0));
     launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->side, b->ray_shape);
@@ -1916,6 +1922,28 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
   if ((flags & MH_RUN_RAYCAST) && !b->have_ray) return fail(MH_ERR_STATE, "mh_batch_set_raycast first");
   if ((flags & MH_RUN_GRAPH) && (flags & MH_RUN_TUNE))
     return fail(MH_ERR_ARG, "tuning runs eagerly; tune first, then replay the graph");
+  // an online marching-cubes tuning times whole replays, one shape each (the
+  // shape is set before the frame table, whose raycast constants depend on it)
+  int trial = -1;
+  if (flags & MH_RUN_GRAPH) {
+    McTune& t = b->mct;
+    if (t.active || t.pending >= 0) {
+      int chosen = b->mc_shape;
+      const int rc = mc_tune_collect(t, b->stream, &chosen);
+      if (rc) return rc;
+      if (chosen != b->mc_shape) {
+        b->mc_shape = chosen;
+        b->drop_graph();
+      }
+    }
+    if (t.active && (flags & MH_RUN_MESH)) {
+      trial = mc_trial_shape(t.trial++, kNumMcShapes);
+      if (trial != b->mc_shape) {
+        b->mc_shape = trial;
+        b->drop_graph();
+      }
+    }
+  }
   // this frame's table: Views (pose, raw depth), MC pass stamps, raycast poses
   for (uint32_t s = 0; s < b->n; ++s) {
     mh_volume* v = b->vols[s];
@@ -1929,6 +1957,11 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
       const int rc = build_ray(v, &b->rp[s], v->V.cTw.m, &f.ray);
       if (rc) return rc;
       f.ray.range = b->d_range + (size_t)s * ray_range_words(b->W, b->H);
+      // Beside the meshing pass the march is the longer branch once the
+      // surfel pass leaves room on the chip (fewer than all resident
+      // workgroups): its waves then issue first on a shared SIMD.  Beside an
+      // all-resident surfel pass the boost cost time instead.
+      f.ray.boost = batch_overlaps(b, flags) && kMcShapes[b->mc_shape].grid_per_cu > 0;
     }
     f.V = v->V;
     f.V.raw = b->raw + (size_t)s * b->W * b->H;
@@ -1977,25 +2010,7 @@ int mh_batch_run(mh_batch* b, const mh_mc_params* mc, uint32_t flags, mh_tune_re
       launch_batch_raycast(B, b->out_depth, b->out_vertex, b->out_normal, b->stream, b->ray_shape);
     }
   } else if (flags & MH_RUN_GRAPH) {
-    // an online marching-cubes tuning times whole replays, one shape each
     McTune& t = b->mct;
-    int trial = -1;
-    if (t.active || t.pending >= 0) {
-      int chosen = b->mc_shape;
-      const int rc = mc_tune_collect(t, b->stream, &chosen);
-      if (rc) return rc;
-      if (chosen != b->mc_shape) {
-        b->mc_shape = chosen;
-        b->drop_graph();
-      }
-    }
-    if (t.active && (flags & MH_RUN_MESH)) {
-      trial = mc_trial_shape(t.trial++, kNumMcShapes);
-      if (trial != b->mc_shape) {
-        b->mc_shape = trial;
-        b->drop_graph();
-      }
-    }
     const uint32_t shape = flags & (MH_RUN_MESH | MH_RUN_RAYCAST);
     if (!b->exec || b->graph_flags != shape) {
       b->drop_graph();

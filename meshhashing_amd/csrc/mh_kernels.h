@@ -28,6 +28,9 @@ struct RayConst {
   // non-zero, disables the range test), or null.  Filled by the range pass.
   uint32_t* range;
   uint32_t tiles_x, tiles;
+  // the march's waves raise their issue priority (a batch's raycast beside a
+  // meshing pass that leaves room on the chip)
+  int boost;
 };
 // Words of a range buffer for a W x H raycast.
 inline uint32_t ray_range_words(uint32_t w, uint32_t h) { return 2u * ((w + 7) / 8) * ((h + 7) / 8) + 1u; }

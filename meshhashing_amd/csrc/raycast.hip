@@ -819,9 +819,6 @@ __global__ __launch_bounds__(64, kRayWavesPerSimd) void raycast_kernel(Src src_,
                                                      float* __restrict__ vertex_out,
                                                      float* __restrict__ normal_out) {
   const Src src = with_table(src_, tab);
-  // Beside the meshing pass (a batch's chain) the march is the longer
-  // branch: its waves issue first on a shared SIMD.
-  __builtin_amdgcn_s_setprio(2);
   __shared__ WaveTable table;
   wave_table_init(&table);
   const uint32_t scene = blockIdx.z;
@@ -829,6 +826,7 @@ __global__ __launch_bounds__(64, kRayWavesPerSimd) void raycast_kernel(Src src_,
   // reloaded after the stores of the march (LDS table, outputs)
   const View V = src.view(scene);
   const RayConst R = ray_of(src, scene, R1);
+  if (R.boost) __builtin_amdgcn_s_setprio(2);
   constexpr int TH = 64 / TW;
   const uint32_t x = blockIdx.x * TW + (threadIdx.x % TW);
   const uint32_t y = blockIdx.y * TH + (threadIdx.x / TW);
